@@ -1,0 +1,880 @@
+/*
+ * Host orchestration of the gfx950 LBMForce platform and its C-ABI (include/lbm_amd.h).
+ *
+ * Restates, for one GPU and one domain, the host side of the reference's GPU platforms
+ * (CommonCalcLBMForceKernel, platforms/common/src/CommonLBMKernels.cpp:160-2097) behind the
+ * kernel interface CalcLBMForceKernel (openmmapi/include/LBMKernels.h:97-189), with the step
+ * semantics of the Reference platform (ReferenceLBMKernels.cpp:308-401):
+ *
+ *   beginStep:  step index <- step count; reflect the coupled particles at solid nodes.
+ *   execute:    first call after beginStep -> advanceFluid; later calls between steps -> the
+ *               coupling of the next step without its reaction (same random numbers, since they
+ *               are keyed by the step index); a repeated evaluation reuses the forces.
+ *   advanceFluid (one lattice step):
+ *      [removal due]  momentReduce(A) -> FP64 u_cm
+ *      [particles]    coupleParticles(A) -> forces, fixed-point reactions; finalizeReaction
+ *      fusedStep(A -> B)   (pull-stream + collision + Guo + reaction + noise)
+ *      [particles]    clearReaction
+ *      [bounce-back]  bounceBack(B) + wall momentum
+ *      [boundaries]   boundaryKernel(A, B) + wall momentum
+ *      swap A, B; step index++; [Mach check due] momentReduce(A) -> host
+ *
+ * Everything runs in order on one HIP stream owned by the context.
+ */
+#include "lbm_internal.h"
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <sstream>
+#include <stdexcept>
+
+namespace lbm {
+
+static thread_local std::string g_lastError;
+
+void setLastError(const std::string& message) {
+    g_lastError = message;
+}
+
+struct Error : public std::runtime_error {
+    int status;
+    Error(int s, const std::string& m) : std::runtime_error(m), status(s) {}
+};
+
+#define HIP_CHECK(call) do { \
+        hipError_t err_ = (call); \
+        if (err_ != hipSuccess) { \
+            std::stringstream msg_; \
+            msg_ << "HIP error " << hipGetErrorString(err_) << " at " << __FILE__ << ":" << __LINE__; \
+            throw Error(LBM_ERROR_RUNTIME, msg_.str()); \
+        } \
+    } while (0)
+
+static FastDiv makeFastDiv(uint32_t d) {
+    FastDiv f;
+    f.d = d;
+    f.l = 0;
+    while ((1ull << f.l) < d)
+        f.l++;
+    f.m = d <= 1 ? 0 : (uint32_t) (((1ull << 32)*((1ull << f.l) - d))/d + 1);
+    return f;
+}
+
+template <typename T>
+static T* deviceAlloc(size_t count) {
+    if (count == 0)
+        return nullptr;
+    void* ptr = nullptr;
+    HIP_CHECK(hipMalloc(&ptr, count*sizeof(T)));
+    return (T*) ptr;
+}
+
+static size_t storageSize(int storage) {
+    switch (storage) {
+    case LBM_STORAGE_FP16:
+    case LBM_STORAGE_BF16:
+        return 2;
+    case LBM_STORAGE_FP64:
+        return 8;
+    default:
+        return 4;
+    }
+}
+
+static const int HCX[NQ] = {0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 0, 0, 0, 0, 1, -1, -1, 1};
+static const int HCY[NQ] = {0, 0, 0, 1, -1, 0, 0, 1, -1, -1, 1, 1, -1, 1, -1, 0, 0, 0, 0};
+static const int HCZ[NQ] = {0, 0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 1, -1, -1, 1, 1, -1, 1, -1};
+static const double HW[NQ] = {1.0/3.0, 1.0/18.0, 1.0/18.0, 1.0/18.0, 1.0/18.0, 1.0/18.0, 1.0/18.0,
+        1.0/36.0, 1.0/36.0, 1.0/36.0, 1.0/36.0, 1.0/36.0, 1.0/36.0, 1.0/36.0, 1.0/36.0, 1.0/36.0,
+        1.0/36.0, 1.0/36.0, 1.0/36.0};
+
+struct CheckpointHeader {
+    char magic[8];
+    int32_t version, nx, ny, nz, numParticles, pad;
+};
+
+}  // namespace lbm
+
+using namespace lbm;
+
+struct lbm_context {
+    lbm_params p;
+    Geometry g;
+    size_t elem = 4;
+    hipStream_t stream = nullptr;
+    void* popA = nullptr;
+    void* popB = nullptr;
+    uint32_t* mask = nullptr;
+    std::vector<char> hostFluid;          // empty: all fluid
+    double numFluid = 0;
+    uint32_t seed = 0;
+    // Walls and boundaries.
+    int* wallNodes = nullptr;
+    int* wallLinks = nullptr;
+    int numWall = 0;
+    double staticWall[3] = {0, 0, 0};     // sum over the links of 2 c_q w_q (lattice units)
+    BoundaryNode* boundary = nullptr;
+    int numBoundary = 0;
+    // Reductions.
+    double* reducePartials = nullptr;
+    double* totals = nullptr;
+    double* ucm = nullptr;
+    double* wallPartials = nullptr;
+    int wallBounceOffset = 0, wallBoundaryOffset = 0, wallCoupleOffset = 0, wallReflectOffset = 0, wallPartialCount = 0;
+    bool wallStepDone = false;
+    bool wallOverride = false;
+    double wallOverrideValue[3] = {0, 0, 0};
+    // Particles.
+    int numParticles = 0, numSystem = 0, numSlots = 0;
+    int* particles = nullptr;
+    double* mass = nullptr;               // lattice units m/m_c
+    std::vector<int> hostParticles;
+    double* particleForce = nullptr;
+    std::vector<double> hostForce;
+    void* reaction = nullptr;             // 3N values, FP64 with FP64 storage, FP32 otherwise
+    unsigned long long* reactionFixed = nullptr;
+    int* entryNodes = nullptr;
+    double* stagePos = nullptr;
+    double* stageVel = nullptr;
+    // Timing of the fused pass with HIP events on the context's stream (diagnostics).
+    bool profile = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
+    // Step state (ReferenceLBMKernels.h:158-176).
+    bool stepPending = false, stepForcesCurrent = false;
+    int64_t stepIndex = 0;
+
+    bool fluct() const {
+        return p.fluid_fluctuations && p.fluid_kT > 0;
+    }
+    double cellMass() const {
+        return p.density*p.dx*p.dx*p.dx;
+    }
+    bool openAxis(int a) const {
+        return p.face_boundary[2*a] != LBM_BOUNDARY_PERIODIC;
+    }
+    bool removalDue(int64_t step) const {
+        return p.momentum_removal_frequency > 0 && step % p.momentum_removal_frequency == 0;
+    }
+    int nodeIndex(int i, int j, int k) const {
+        return i + p.nx*(j + p.ny*k);
+    }
+};
+
+namespace {
+
+void findWalls(lbm_context* c, std::vector<int>& nodes, std::vector<int>& links) {
+    // Bounce-back links (LBMBoundaries.h:118-149): fluid node x, direction q with x + c_q solid,
+    // without the links that cross an open face.
+    const lbm_params& p = c->p;
+    if (c->hostFluid.empty() || p.wall_scheme != LBM_WALL_BOUNCE_BACK)
+        return;
+    int size[3] = {p.nx, p.ny, p.nz};
+    int n = p.nx*p.ny*p.nz;
+    for (int node = 0; node < n; node++) {
+        if (!c->hostFluid[node])
+            continue;
+        int idx[3] = {node % p.nx, (node/p.nx) % p.ny, node/(p.nx*p.ny)};
+        int bits = 0;
+        for (int q = 1; q < NQ; q++) {
+            int cq[3] = {HCX[q], HCY[q], HCZ[q]}, t[3];
+            bool crosses = false;
+            for (int a = 0; a < 3; a++) {
+                t[a] = idx[a] + cq[a];
+                crosses = crosses || (c->openAxis(a) && (t[a] < 0 || t[a] >= size[a]));
+                t[a] = (t[a] + size[a]) % size[a];
+            }
+            if (!crosses && !c->hostFluid[c->nodeIndex(t[0], t[1], t[2])]) {
+                bits |= 1 << q;
+                for (int a = 0; a < 3; a++)
+                    c->staticWall[a] += 2.0*cq[a]*HW[q];
+            }
+        }
+        if (bits) {
+            nodes.push_back(node);
+            links.push_back(bits);
+        }
+    }
+}
+
+void findBoundaries(lbm_context* c, std::vector<BoundaryNode>& list) {
+    // Boundary nodes of regularized walls and open faces (LBMBoundaries.h:46-109).
+    const lbm_params& p = c->p;
+    bool regularized = !c->hostFluid.empty() && p.wall_scheme == LBM_WALL_REGULARIZED;
+    bool open = c->openAxis(0) || c->openAxis(1) || c->openAxis(2);
+    if (!regularized && !open)
+        return;
+    int size[3] = {p.nx, p.ny, p.nz};
+    int n = p.nx*p.ny*p.nz;
+    for (int node = 0; node < n; node++) {
+        if (!c->hostFluid.empty() && !c->hostFluid[node])
+            continue;
+        int idx[3] = {node % p.nx, (node/p.nx) % p.ny, node/(p.nx*p.ny)};
+        int solidBits = 0, outsideBits = 0;
+        for (int q = 1; q < NQ; q++) {
+            int cq[3] = {HCX[q], HCY[q], HCZ[q]}, s[3];
+            bool beyond = false;
+            for (int a = 0; a < 3; a++) {
+                s[a] = idx[a] - cq[a];
+                if (s[a] < 0 || s[a] >= size[a]) {
+                    beyond = beyond || c->openAxis(a);
+                    s[a] = (s[a] + size[a]) % size[a];
+                }
+            }
+            if (beyond)
+                outsideBits |= 1 << q;
+            else if (regularized && !c->hostFluid[c->nodeIndex(s[0], s[1], s[2])])
+                solidBits |= 1 << q;
+        }
+        if (solidBits == 0 && outsideBits == 0)
+            continue;
+        int kind = 0, face = -1;
+        if (solidBits == 0) {
+            int velocityFace = -1, densityFace = -1, numDensity = 0;
+            for (int f = 0; f < 6; f++) {
+                int a = f/2;
+                if (!c->openAxis(a) || idx[a] != (f % 2 == 0 ? 0 : size[a] - 1))
+                    continue;
+                int type = p.face_boundary[f];
+                if ((type == LBM_BOUNDARY_VELOCITY || type == LBM_BOUNDARY_DENSITY_VELOCITY) && velocityFace < 0)
+                    velocityFace = f;
+                if (type == LBM_BOUNDARY_DENSITY && numDensity++ == 0)
+                    densityFace = f;
+            }
+            if (velocityFace >= 0) {
+                kind = p.face_boundary[velocityFace] == LBM_BOUNDARY_DENSITY_VELOCITY ? 4 : 1;
+                face = velocityFace;
+            }
+            else {
+                kind = numDensity == 1 ? 2 : 3;
+                face = densityFace;
+            }
+        }
+        list.push_back(BoundaryNode{node, solidBits | outsideBits, solidBits, kind + 8*(face + 1)});
+    }
+}
+
+StepArgs stepArgs(lbm_context* c, int64_t step) {
+    StepArgs a;
+    a.g = c->g;
+    a.omega = c->p.omega;
+    a.gx = c->p.body_acceleration[0];
+    a.gy = c->p.body_acceleration[1];
+    a.gz = c->p.body_acceleration[2];
+    a.removal = c->removalDue(step) ? 1 : 0;
+    a.seed = c->seed;
+    a.step = step;
+    a.mu = 3.0*c->p.fluid_kT*c->p.dt*c->p.dt/(c->cellMass()*c->p.dx*c->p.dx);
+    a.roundSeed = (uint32_t) (c->seed*0x9E3779B9u) ^ (uint32_t) (step*0x85EBCA6Bll) ^ (uint32_t) ((uint64_t) step >> 32);
+    return a;
+}
+
+ParticleArgs particleArgs(lbm_context* c, int64_t step, bool isStep) {
+    ParticleArgs a;
+    a.g = c->g;
+    a.numParticles = c->numParticles;
+    a.stencil = c->p.interpolation_stencil;
+    a.dx = c->p.dx;
+    a.gamma = c->p.friction*c->p.dt;
+    a.kT = c->p.kT*c->p.dt*c->p.dt/(c->cellMass()*c->p.dx*c->p.dx);
+    a.velocityScale = c->p.dx/c->p.dt;
+    a.forceScale = c->cellMass()*c->p.dx/(c->p.dt*c->p.dt);
+    a.momentumScale = c->cellMass()*c->p.dx/c->p.dt;
+    a.cellMass = c->cellMass();
+    a.removal = c->removalDue(step) ? 1 : 0;
+    a.isStep = isStep ? 1 : 0;
+    a.randomForce = (a.kT > 0 && a.gamma > 0) ? 1 : 0;
+    a.seed = c->seed;
+    a.step = step;
+    a.hasSolids = c->mask != nullptr;
+    return a;
+}
+
+void reduceMoments(lbm_context* c) {
+    launchMomentReduce(c->p.storage, c->mask != nullptr, c->popA, c->mask, c->g, c->reducePartials, c->stream);
+    launchFinishReduce(c->reducePartials, c->numFluid, c->totals, c->ucm, c->stream);
+}
+
+void couple(lbm_context* c, const double* dPos, const double* dVel, int64_t step, bool isStep) {
+    ParticleArgs a = particleArgs(c, step, isStep);
+    if (a.removal)
+        reduceMoments(c);
+    double* partials = isStep ? c->wallPartials + 3*(size_t) c->wallCoupleOffset : nullptr;
+    launchCouple(c->p.storage, c->popA, c->mask, dPos, dVel, c->particles, c->mass, c->ucm, a, c->particleForce,
+            c->reactionFixed, c->entryNodes, partials, c->stream);
+}
+
+void advanceFluid(lbm_context* c, const double* dPos, const double* dVel) {
+    StepArgs a = stepArgs(c, c->stepIndex);
+    bool particles = c->numParticles > 0;
+    if (particles) {
+        couple(c, dPos, dVel, c->stepIndex, true);   // runs the removal reduction when due
+        launchFinalizeReaction(c->reactionFixed, c->reaction, c->p.storage == LBM_STORAGE_FP64, c->entryNodes,
+                c->numParticles*c->numSlots, c->g.n, c->stream);
+    }
+    else if (a.removal)
+        reduceMoments(c);
+    std::pair<hipEvent_t, hipEvent_t> ev{nullptr, nullptr};
+    if (c->profile) {
+        HIP_CHECK(hipEventCreate(&ev.first));
+        HIP_CHECK(hipEventCreate(&ev.second));
+        HIP_CHECK(hipEventRecord(ev.first, c->stream));
+    }
+    launchFused(c->p.storage, c->mask != nullptr, particles, c->fluct(), c->popA, c->popB, c->mask, c->reaction, c->ucm,
+            a, c->stream);
+    if (c->profile) {
+        HIP_CHECK(hipEventRecord(ev.second, c->stream));
+        c->events.push_back(ev);
+    }
+    if (particles)
+        launchClearReaction(c->reaction, c->p.storage == LBM_STORAGE_FP64, c->entryNodes, c->numParticles*c->numSlots,
+                c->g.n, c->stream);
+    if (c->numWall > 0)
+        launchBounceBack(c->p.storage, c->popB, c->wallNodes, c->wallLinks, c->numWall, c->g,
+                c->wallPartials + 3*(size_t) c->wallBounceOffset, c->stream);
+    if (c->numBoundary > 0) {
+        BoundaryArgs b;
+        b.g = c->g;
+        b.count = c->numBoundary;
+        b.omega = c->p.omega;
+        b.gx = a.gx;
+        b.gy = a.gy;
+        b.gz = a.gz;
+        b.removal = a.removal;
+        b.fluct = c->fluct() ? 1 : 0;
+        b.seed = c->seed;
+        b.step = c->stepIndex;
+        b.mu = a.mu;
+        for (int f = 0; f < 6; f++) {
+            for (int k = 0; k < 3; k++)
+                b.faceVelocity[f][k] = c->p.face_velocity[f][k];
+            b.faceDensity[f] = c->p.face_density[f];
+        }
+        b.roundSeed = a.roundSeed ^ 0x5BD1E995u;
+        launchBoundaries(c->p.storage, c->popA, c->popB, c->boundary, c->ucm, b,
+                c->wallPartials + 3*(size_t) c->wallBoundaryOffset, c->stream);
+    }
+    HIP_CHECK(hipGetLastError());
+    std::swap(c->popA, c->popB);
+    c->wallStepDone = true;
+    c->stepForcesCurrent = true;
+    c->stepIndex++;
+    if (c->p.mach_check_frequency > 0 && c->stepIndex % c->p.mach_check_frequency == 0) {
+        reduceMoments(c);
+        double totals[5];
+        HIP_CHECK(hipMemcpyAsync(totals, c->totals, sizeof(totals), hipMemcpyDeviceToHost, c->stream));
+        HIP_CHECK(hipStreamSynchronize(c->stream));
+        double mach = std::sqrt(3.0*totals[4]);
+        if (!(mach <= c->p.mach_number_limit)) {
+            std::stringstream msg;
+            msg << "LBMForce: the Mach number of the fluid is " << mach << " after " << c->stepIndex << " lattice steps, "
+                << "above the limit " << c->p.mach_number_limit << " (docs/theory.md, section 6). Reduce the forces on the "
+                << "fluid, the time step or the friction.";
+            throw Error(LBM_ERROR_MACH, msg.str());
+        }
+    }
+}
+
+void beginStepDevice(lbm_context* c, int64_t step, const double* dPos, double* dVel) {
+    c->stepIndex = step;
+    c->stepPending = true;
+    c->wallStepDone = false;
+    c->wallOverride = false;
+    if (c->wallPartialCount > 0)
+        launchFill(LBM_STORAGE_FP64, c->wallPartials, 3*(size_t) c->wallPartialCount, c->stream);
+    if (c->mask != nullptr && c->numParticles > 0) {
+        ParticleArgs a = particleArgs(c, step, true);
+        launchReflect(dPos, dVel, c->particles, c->mass, c->mask, a, c->wallPartials + 3*(size_t) c->wallReflectOffset,
+                c->stream);
+    }
+    HIP_CHECK(hipGetLastError());
+}
+
+/* Returns true if the forces of the coupled particles in c->particleForce are to be added. */
+void executeDevice(lbm_context* c, int64_t stepCount, bool includeForces, const double* dPos, const double* dVel) {
+    bool repeated = c->stepForcesCurrent && stepCount == c->stepIndex - 1;
+    if (c->stepPending) {
+        c->stepPending = false;
+        advanceFluid(c, dPos, dVel);
+    }
+    else if (includeForces && c->numParticles > 0 && stepCount > 0 && !repeated) {
+        c->stepForcesCurrent = false;
+        couple(c, dPos, dVel, stepCount, false);
+    }
+    HIP_CHECK(hipGetLastError());
+}
+
+void clearEvents(lbm_context* c) {
+    for (auto& e : c->events) {
+        (void) hipEventDestroy(e.first);
+        (void) hipEventDestroy(e.second);
+    }
+    c->events.clear();
+}
+
+void destroy(lbm_context* c) {
+    if (c->stream)
+        (void) hipStreamSynchronize(c->stream);
+    clearEvents(c);
+    void* buffers[] = {c->popA, c->popB, c->mask, c->wallNodes, c->wallLinks, c->boundary, c->reducePartials, c->totals,
+                       c->ucm, c->wallPartials, c->particles, c->mass, c->particleForce, c->reaction, c->reactionFixed,
+                       c->entryNodes, c->stagePos, c->stageVel};
+    for (void* b : buffers)
+        if (b)
+            (void) hipFree(b);
+    if (c->stream)
+        (void) hipStreamDestroy(c->stream);
+    delete c;
+}
+
+template <typename T>
+void upload(T* dst, const std::vector<T>& src, hipStream_t stream) {
+    if (!src.empty())
+        HIP_CHECK(hipMemcpyAsync(dst, src.data(), src.size()*sizeof(T), hipMemcpyHostToDevice, stream));
+}
+
+void create(const lbm_params* params, const int* solid, int numSolid, const int* parts, const double* masses,
+        int numParts, int numSys, lbm_context* c) {
+    const lbm_params& p = *params;
+    c->p = p;
+    if (p.nx <= 0 || p.ny <= 0 || p.nz <= 0)
+        throw Error(LBM_ERROR_INVALID, "LBMForce: the grid size must be set to positive values with setGridSize()");
+    if (p.drag_scheme != LBM_DRAG_EXPLICIT)
+        throw Error(LBM_ERROR_UNSUPPORTED, "LBMForce: the Centered drag scheme is not implemented by this build");
+    if (p.storage < LBM_STORAGE_FP32 || p.storage > LBM_STORAGE_FP64)
+        throw Error(LBM_ERROR_INVALID, "LBMForce: unknown storage type");
+    if (numParts < 0 || numSys < 0 || (numParts > 0 && (!parts || !masses)))
+        throw Error(LBM_ERROR_INVALID, "lbm_create: invalid particle arrays");
+    uint64_t n = (uint64_t) p.nx*p.ny*p.nz;
+    if (n > 0x7FFFFFFFull)
+        throw Error(LBM_ERROR_INVALID, "LBMForce: the lattice has more than 2147483647 nodes");
+    c->g.nx = p.nx;
+    c->g.ny = p.ny;
+    c->g.nz = p.nz;
+    c->g.nxy = (uint32_t) (p.nx*p.ny);
+    c->g.n = (uint32_t) n;
+    c->g.divx = makeFastDiv(p.nx);
+    c->g.divxy = makeFastDiv(c->g.nxy);
+    c->g.openX = c->openAxis(0);
+    c->g.openY = c->openAxis(1);
+    c->g.openZ = c->openAxis(2);
+    c->elem = storageSize(p.storage);
+    c->seed = (uint32_t) p.random_number_seed;
+    if (c->seed == 0)
+        c->seed = (uint32_t) std::chrono::high_resolution_clock::now().time_since_epoch().count() | 1u;
+    HIP_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    c->popA = deviceAlloc<char>(NQ*n*c->elem);
+    c->popB = deviceAlloc<char>(NQ*n*c->elem);
+    // Solid nodes.
+    c->numFluid = (double) n;
+    if (numSolid > 0) {
+        c->hostFluid.assign(n, 1);
+        std::vector<uint32_t> bits((n + 31)/32, 0);
+        for (int s = 0; s < numSolid; s++) {
+            int node = solid[s];
+            if (node < 0 || (uint64_t) node >= n)
+                throw Error(LBM_ERROR_INVALID, "LBMForce: a solid node index is out of range");
+            if (!c->hostFluid[node])
+                throw Error(LBM_ERROR_INVALID, "LBMForce: a solid node is listed more than once");
+            c->hostFluid[node] = 0;
+            bits[node >> 5] |= 1u << (node & 31);
+        }
+        c->numFluid = (double) (n - numSolid);
+        c->mask = deviceAlloc<uint32_t>(bits.size());
+        upload(c->mask, bits, c->stream);
+    }
+    std::vector<int> wallNodes, wallLinks;
+    findWalls(c, wallNodes, wallLinks);
+    c->numWall = (int) wallNodes.size();
+    c->wallNodes = deviceAlloc<int>(wallNodes.size());
+    c->wallLinks = deviceAlloc<int>(wallLinks.size());
+    upload(c->wallNodes, wallNodes, c->stream);
+    upload(c->wallLinks, wallLinks, c->stream);
+    std::vector<BoundaryNode> list;
+    findBoundaries(c, list);
+    c->numBoundary = (int) list.size();
+    c->boundary = deviceAlloc<BoundaryNode>(list.size());
+    upload(c->boundary, list, c->stream);
+    // Reductions.
+    c->reducePartials = deviceAlloc<double>(5*REDUCE_BLOCKS);
+    c->totals = deviceAlloc<double>(8);
+    c->ucm = deviceAlloc<double>(4);
+    HIP_CHECK(hipMemsetAsync(c->ucm, 0, 4*sizeof(double), c->stream));
+    // Particles.
+    c->numParticles = numParts;
+    c->numSystem = numSys;
+    int w = stencilWidth(p.interpolation_stencil);
+    c->numSlots = w*w*w;
+    auto blocks = [](int count) { return (count + BLOCK - 1)/BLOCK; };
+    int coupleGroups = BLOCK/(p.interpolation_stencil == LBM_STENCIL_KEYS ? 64 : p.interpolation_stencil ==
+            LBM_STENCIL_THREE_POINT ? 32 : p.interpolation_stencil == LBM_STENCIL_TRILINEAR ? 8 : 1);
+    c->wallBounceOffset = 0;
+    c->wallBoundaryOffset = c->wallBounceOffset + blocks(c->numWall);
+    c->wallCoupleOffset = c->wallBoundaryOffset + blocks(c->numBoundary);
+    c->wallReflectOffset = c->wallCoupleOffset + (numParts + coupleGroups - 1)/coupleGroups;
+    c->wallPartialCount = c->wallReflectOffset + blocks(numParts);
+    c->wallPartials = deviceAlloc<double>(3*(size_t) std::max(c->wallPartialCount, 1));
+    HIP_CHECK(hipMemsetAsync(c->wallPartials, 0, 3*sizeof(double)*std::max(c->wallPartialCount, 1), c->stream));
+    if (numParts > 0) {
+        c->hostParticles.assign(parts, parts + numParts);
+        std::vector<double> m(numParts);
+        for (int i = 0; i < numParts; i++) {
+            if (parts[i] < 0 || parts[i] >= numSys)
+                throw Error(LBM_ERROR_INVALID, "LBMForce: a coupled particle index is out of range");
+            m[i] = masses[i]/c->cellMass();
+        }
+        c->particles = deviceAlloc<int>(numParts);
+        c->mass = deviceAlloc<double>(numParts);
+        upload(c->particles, c->hostParticles, c->stream);
+        upload(c->mass, m, c->stream);
+        c->particleForce = deviceAlloc<double>(3*(size_t) numParts);
+        HIP_CHECK(hipMemsetAsync(c->particleForce, 0, 3*sizeof(double)*numParts, c->stream));
+        c->hostForce.assign(3*(size_t) numParts, 0.0);
+        size_t realBytes = p.storage == LBM_STORAGE_FP64 ? 8 : 4;
+        c->reaction = deviceAlloc<char>(3*n*realBytes);
+        c->reactionFixed = deviceAlloc<unsigned long long>(3*n);
+        HIP_CHECK(hipMemsetAsync(c->reaction, 0, 3*n*realBytes, c->stream));
+        HIP_CHECK(hipMemsetAsync(c->reactionFixed, 0, 3*n*sizeof(unsigned long long), c->stream));
+        c->entryNodes = deviceAlloc<int>((size_t) numParts*c->numSlots);
+        c->stagePos = deviceAlloc<double>(3*(size_t) std::max(numSys, 1));
+        c->stageVel = deviceAlloc<double>(3*(size_t) std::max(numSys, 1));
+    }
+    // The fluid at equilibrium with the initial velocity; solid nodes hold no fluid.
+    launchInitEquilibrium(p.storage, c->popA, c->mask, c->g, p.initial_velocity[0], p.initial_velocity[1],
+            p.initial_velocity[2], c->stream);
+    HIP_CHECK(hipMemcpyAsync(c->popB, c->popA, NQ*n*c->elem, hipMemcpyDeviceToDevice, c->stream));
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+}
+
+std::vector<double> getState(lbm_context* c) {
+    size_t count = NQ*(size_t) c->g.n;
+    double* tmp = deviceAlloc<double>(count);
+    std::vector<double> out(count);
+    try {
+        launchGather(c->p.storage, c->popA, tmp, c->g, c->stream);
+        HIP_CHECK(hipMemcpyAsync(out.data(), tmp, count*sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIP_CHECK(hipStreamSynchronize(c->stream));
+    }
+    catch (...) {
+        (void) hipFree(tmp);
+        throw;
+    }
+    HIP_CHECK(hipFree(tmp));
+    return out;
+}
+
+void setState(lbm_context* c, const double* state) {
+    size_t count = NQ*(size_t) c->g.n;
+    double* tmp = deviceAlloc<double>(count);
+    try {
+        HIP_CHECK(hipMemcpyAsync(tmp, state, count*sizeof(double), hipMemcpyHostToDevice, c->stream));
+        launchScatter(c->p.storage, c->popA, tmp, c->g, c->stream);
+        HIP_CHECK(hipStreamSynchronize(c->stream));
+    }
+    catch (...) {
+        (void) hipFree(tmp);
+        throw;
+    }
+    HIP_CHECK(hipFree(tmp));
+    c->stepForcesCurrent = false;
+}
+
+void wallMomentum(lbm_context* c, double out[3]) {
+    out[0] = out[1] = out[2] = 0.0;
+    if (c->wallOverride) {
+        for (int k = 0; k < 3; k++)
+            out[k] = c->wallOverrideValue[k];
+        return;
+    }
+    if (c->wallPartialCount == 0)
+        return;
+    std::vector<double> partials(3*(size_t) c->wallPartialCount);
+    HIP_CHECK(hipMemcpyAsync(partials.data(), c->wallPartials, partials.size()*sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIP_CHECK(hipStreamSynchronize(c->stream));
+    double fluid[3] = {0, 0, 0}, reflect[3] = {0, 0, 0};
+    for (int b = 0; b < c->wallReflectOffset; b++)
+        for (int k = 0; k < 3; k++)
+            fluid[k] += partials[3*(size_t) b + k];
+    for (int b = c->wallReflectOffset; b < c->wallPartialCount; b++)
+        for (int k = 0; k < 3; k++)
+            reflect[k] += partials[3*(size_t) b + k];
+    double scale = c->cellMass()*c->p.dx/c->p.dt;
+    for (int k = 0; k < 3; k++) {
+        double lattice = fluid[k] + (c->wallStepDone && c->numWall > 0 ? c->staticWall[k] : 0.0);
+        out[k] = lattice*scale + reflect[k];
+    }
+}
+
+int fail(const Error& e) {
+    g_lastError = e.what();
+    return e.status;
+}
+
+}  // namespace
+
+#define LBM_TRY(...) \
+    try { __VA_ARGS__; g_lastError.clear(); return LBM_OK; } \
+    catch (const Error& e) { return fail(e); } \
+    catch (const std::bad_alloc&) { g_lastError = "out of host memory"; return LBM_ERROR_RUNTIME; } \
+    catch (const std::exception& e) { g_lastError = e.what(); return LBM_ERROR_RUNTIME; }
+
+#define LBM_CHECK_CTX(ctx) if (!(ctx)) { g_lastError = "null context"; return LBM_ERROR_INVALID; }
+
+extern "C" {
+
+const char* lbm_last_error(void) {
+    return g_lastError.c_str();
+}
+
+int lbm_create(const lbm_params* params, const int* solid, int numSolid, const int* parts, const double* masses,
+        int numParts, int numSys, lbm_context** out) {
+    if (!params || !out) {
+        g_lastError = "lbm_create: null argument";
+        return LBM_ERROR_INVALID;
+    }
+    lbm_context* c = new lbm_context();
+    try {
+        create(params, solid, numSolid, parts, masses, numParts, numSys, c);
+    }
+    catch (const Error& e) {
+        destroy(c);
+        return fail(e);
+    }
+    catch (const std::exception& e) {
+        destroy(c);
+        g_lastError = e.what();
+        return LBM_ERROR_RUNTIME;
+    }
+    *out = c;
+    g_lastError.clear();
+    return LBM_OK;
+}
+
+int lbm_destroy(lbm_context* c) {
+    if (c)
+        destroy(c);
+    return LBM_OK;
+}
+
+int lbm_begin_step_device(lbm_context* c, int64_t step, const double* positions, double* velocities) {
+    LBM_CHECK_CTX(c);
+    LBM_TRY(beginStepDevice(c, step, positions, velocities))
+}
+
+int lbm_execute_device(lbm_context* c, int64_t step, int includeForces, const double* positions, const double* velocities,
+        double* forces) {
+    LBM_CHECK_CTX(c);
+    LBM_TRY({
+        executeDevice(c, step, includeForces != 0, positions, velocities);
+        if (includeForces && forces)
+            launchAddForces(c->particleForce, c->particles, c->numParticles, forces, c->stream);
+        HIP_CHECK(hipGetLastError());
+    })
+}
+
+int lbm_begin_step(lbm_context* c, int64_t step, const double* positions, double* velocities) {
+    LBM_CHECK_CTX(c);
+    LBM_TRY({
+        if (c->mask != nullptr && c->numParticles > 0) {
+            size_t bytes = 3*sizeof(double)*(size_t) c->numSystem;
+            HIP_CHECK(hipMemcpyAsync(c->stagePos, positions, bytes, hipMemcpyHostToDevice, c->stream));
+            HIP_CHECK(hipMemcpyAsync(c->stageVel, velocities, bytes, hipMemcpyHostToDevice, c->stream));
+            beginStepDevice(c, step, c->stagePos, c->stageVel);
+            HIP_CHECK(hipMemcpyAsync(velocities, c->stageVel, bytes, hipMemcpyDeviceToHost, c->stream));
+            HIP_CHECK(hipStreamSynchronize(c->stream));
+        }
+        else
+            beginStepDevice(c, step, nullptr, nullptr);
+    })
+}
+
+int lbm_execute(lbm_context* c, int64_t step, int includeForces, const double* positions, const double* velocities,
+        double* forces) {
+    LBM_CHECK_CTX(c);
+    LBM_TRY({
+        if (c->numParticles > 0) {
+            size_t bytes = 3*sizeof(double)*(size_t) c->numSystem;
+            HIP_CHECK(hipMemcpyAsync(c->stagePos, positions, bytes, hipMemcpyHostToDevice, c->stream));
+            HIP_CHECK(hipMemcpyAsync(c->stageVel, velocities, bytes, hipMemcpyHostToDevice, c->stream));
+        }
+        executeDevice(c, step, includeForces != 0, c->stagePos, c->stageVel);
+        if (includeForces && c->numParticles > 0 && forces) {
+            HIP_CHECK(hipMemcpyAsync(c->hostForce.data(), c->particleForce, c->hostForce.size()*sizeof(double),
+                    hipMemcpyDeviceToHost, c->stream));
+            HIP_CHECK(hipStreamSynchronize(c->stream));
+            for (int i = 0; i < c->numParticles; i++)
+                for (int k = 0; k < 3; k++)
+                    forces[3*(size_t) c->hostParticles[i] + k] += c->hostForce[3*(size_t) i + k];
+        }
+        else
+            HIP_CHECK(hipStreamSynchronize(c->stream));
+    })
+}
+
+int lbm_copy_parameters(lbm_context* c, const lbm_params* params) {
+    LBM_CHECK_CTX(c);
+    LBM_TRY({
+        if (!params)
+            throw Error(LBM_ERROR_INVALID, "lbm_copy_parameters: null argument");
+        if (params->nx != c->p.nx || params->ny != c->p.ny || params->nz != c->p.nz)
+            throw Error(LBM_ERROR_INVALID, "updateParametersInContext: the grid size cannot be changed");
+        int storage = c->p.storage;
+        c->p = *params;
+        c->p.storage = storage;
+    })
+}
+
+int lbm_get_fluid_fields(lbm_context* c, double* density, double* velocity) {
+    LBM_CHECK_CTX(c);
+    LBM_TRY({
+        size_t n = c->g.n;
+        double* dRho = density ? deviceAlloc<double>(n) : nullptr;
+        double* dU = velocity ? deviceAlloc<double>(3*n) : nullptr;
+        launchFields(c->p.storage, c->popA, c->mask, c->g, c->p.density, c->p.dx/c->p.dt, c->p.body_acceleration[0],
+                c->p.body_acceleration[1], c->p.body_acceleration[2], dRho, dU, c->stream);
+        if (density)
+            HIP_CHECK(hipMemcpyAsync(density, dRho, n*sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        if (velocity)
+            HIP_CHECK(hipMemcpyAsync(velocity, dU, 3*n*sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIP_CHECK(hipStreamSynchronize(c->stream));
+        if (dRho)
+            HIP_CHECK(hipFree(dRho));
+        if (dU)
+            HIP_CHECK(hipFree(dU));
+    })
+}
+
+int lbm_get_fluid_mach_number(lbm_context* c, double* mach) {
+    LBM_CHECK_CTX(c);
+    LBM_TRY({
+        reduceMoments(c);
+        double totals[5];
+        HIP_CHECK(hipMemcpyAsync(totals, c->totals, sizeof(totals), hipMemcpyDeviceToHost, c->stream));
+        HIP_CHECK(hipStreamSynchronize(c->stream));
+        *mach = std::sqrt(3.0*totals[4]);
+    })
+}
+
+int lbm_get_fluid_totals(lbm_context* c, double* mass, double momentum[3]) {
+    LBM_CHECK_CTX(c);
+    LBM_TRY({
+        reduceMoments(c);
+        double totals[5];
+        HIP_CHECK(hipMemcpyAsync(totals, c->totals, sizeof(totals), hipMemcpyDeviceToHost, c->stream));
+        HIP_CHECK(hipStreamSynchronize(c->stream));
+        *mass = c->numFluid + totals[0];
+        for (int k = 0; k < 3; k++)
+            momentum[k] = totals[1 + k];
+    })
+}
+
+int lbm_get_wall_force(lbm_context* c, double force[3]) {
+    LBM_CHECK_CTX(c);
+    LBM_TRY({
+        double m[3];
+        wallMomentum(c, m);
+        for (int k = 0; k < 3; k++)
+            force[k] = m[k]/c->p.dt;
+    })
+}
+
+int lbm_get_fluid_state(lbm_context* c, double* state, size_t size) {
+    LBM_CHECK_CTX(c);
+    LBM_TRY({
+        if (size != NQ*(size_t) c->g.n)
+            throw Error(LBM_ERROR_INVALID, "LBMForce: getFluidState() was called with a state of the wrong size");
+        std::vector<double> s = getState(c);
+        std::memcpy(state, s.data(), size*sizeof(double));
+    })
+}
+
+int lbm_set_fluid_state(lbm_context* c, const double* state, size_t size) {
+    LBM_CHECK_CTX(c);
+    LBM_TRY({
+        if (size != NQ*(size_t) c->g.n)
+            throw Error(LBM_ERROR_INVALID, "LBMForce: setFluidState() was called with a state of the wrong size");
+        setState(c, state);
+    })
+}
+
+int lbm_checkpoint_size(lbm_context* c, size_t* size) {
+    LBM_CHECK_CTX(c);
+    LBM_TRY(*size = sizeof(CheckpointHeader) + NQ*(size_t) c->g.n*sizeof(double) + 3*sizeof(double))
+}
+
+int lbm_create_checkpoint(lbm_context* c, void* buffer, size_t size) {
+    LBM_CHECK_CTX(c);
+    LBM_TRY({
+        size_t stateBytes = NQ*(size_t) c->g.n*sizeof(double);
+        if (size < sizeof(CheckpointHeader) + stateBytes + 3*sizeof(double))
+            throw Error(LBM_ERROR_INVALID, "lbm_create_checkpoint: buffer too small");
+        CheckpointHeader h;
+        std::memcpy(h.magic, "LBMAMDC1", 8);
+        h.version = 1;
+        h.nx = c->p.nx;
+        h.ny = c->p.ny;
+        h.nz = c->p.nz;
+        h.numParticles = c->numParticles;
+        h.pad = 0;
+        std::vector<double> s = getState(c);
+        double m[3];
+        wallMomentum(c, m);
+        char* out = (char*) buffer;
+        std::memcpy(out, &h, sizeof(h));
+        std::memcpy(out + sizeof(h), s.data(), stateBytes);
+        std::memcpy(out + sizeof(h) + stateBytes, m, 3*sizeof(double));
+    })
+}
+
+int lbm_load_checkpoint(lbm_context* c, const void* buffer, size_t size) {
+    LBM_CHECK_CTX(c);
+    LBM_TRY({
+        size_t stateBytes = NQ*(size_t) c->g.n*sizeof(double);
+        CheckpointHeader h;
+        if (size < sizeof(h))
+            throw Error(LBM_ERROR_INVALID, "LBMForce: the checkpoint is truncated");
+        std::memcpy(&h, buffer, sizeof(h));
+        if (std::memcmp(h.magic, "LBMAMDC1", 8) != 0 || h.version != 1)
+            throw Error(LBM_ERROR_INVALID, "LBMForce: the data is not a checkpoint of this LBMForce platform");
+        if (h.nx != c->p.nx || h.ny != c->p.ny || h.nz != c->p.nz || h.numParticles != c->numParticles)
+            throw Error(LBM_ERROR_INVALID, "LBMForce: the checkpoint was written for another lattice or System");
+        if (size < sizeof(h) + stateBytes + 3*sizeof(double))
+            throw Error(LBM_ERROR_INVALID, "LBMForce: the checkpoint is truncated");
+        const char* in = (const char*) buffer;
+        setState(c, (const double*) (in + sizeof(h)));
+        std::memcpy(c->wallOverrideValue, in + sizeof(h) + stateBytes, 3*sizeof(double));
+        c->wallOverride = true;
+    })
+}
+
+int lbm_synchronize(lbm_context* c) {
+    LBM_CHECK_CTX(c);
+    LBM_TRY(HIP_CHECK(hipStreamSynchronize(c->stream)))
+}
+
+int lbm_profile_fused(lbm_context* c, int enable, double* totalMs, int* count) {
+    LBM_CHECK_CTX(c);
+    LBM_TRY({
+        HIP_CHECK(hipStreamSynchronize(c->stream));
+        double total = 0.0;
+        for (auto& e : c->events) {
+            float ms = 0.0f;
+            HIP_CHECK(hipEventElapsedTime(&ms, e.first, e.second));
+            total += ms;
+        }
+        if (totalMs)
+            *totalMs = total;
+        if (count)
+            *count = (int) c->events.size();
+        clearEvents(c);
+        c->profile = enable != 0;
+    })
+}
+
+/* Internal: the device pointers and the stream of a context, for the harness (lbm_harness.hip). */
+hipStream_t lbm_internal_stream(lbm_context* c) {
+    return c->stream;
+}
+
+}  // extern "C"

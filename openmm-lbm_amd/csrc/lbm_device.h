@@ -1,0 +1,421 @@
+/*
+ * Device-side building blocks of the gfx950 LBMForce platform: the D3Q19 velocity set, the
+ * moments of a node, the regularized collision with Guo forcing written in its closed
+ * 10-moment Hermite form, the fluctuating part, Philox4x32-10 normal numbers, and the
+ * conversions of the population storage types.
+ *
+ * The collision is the one of ReferenceLBMKernels.cpp:1090-1120 (D3Q19.h:55-134): the
+ * post-collision population
+ *   C_q = feq_q(rho, u) + (1 - omega) fneq_q(Pi_neq) + S_q(u, F)/2,   u = (j + F/2)/rho,
+ * is a second-order Hermite expansion in c_q, so it is computed exactly as
+ *   C_q - w_q = w_q [drho + 3 c_q.(j + F) + 4.5 H2(c_q) : P],
+ *   P = rho u u + (1 - omega) Pi_neq + (u F + F u)/2,
+ * and Pi_neq = sum_q c c df_q - I drho/3 - j j/rho (the same moment as D3Q19.h:98-110, since
+ * sum_q H2 dfeq_q = rho u u on D3Q19).  Only the rounding differs from the reference.
+ */
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <hip/hip_fp16.h>
+#include <hip/hip_bf16.h>
+#include <stdint.h>
+#include <utility>
+#include <type_traits>
+
+namespace lbm {
+
+constexpr int NQ = 19;
+constexpr int CX[NQ] = {0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 0, 0, 0, 0, 1, -1, -1, 1};
+constexpr int CY[NQ] = {0, 0, 0, 1, -1, 0, 0, 1, -1, -1, 1, 1, -1, 1, -1, 0, 0, 0, 0};
+constexpr int CZ[NQ] = {0, 0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 1, -1, -1, 1, 1, -1, 1, -1};
+constexpr int OPP[NQ] = {0, 2, 1, 4, 3, 6, 5, 8, 7, 10, 9, 12, 11, 14, 13, 16, 15, 18, 17};
+constexpr double WQ[NQ] = {1.0/3.0, 1.0/18.0, 1.0/18.0, 1.0/18.0, 1.0/18.0, 1.0/18.0, 1.0/18.0,
+        1.0/36.0, 1.0/36.0, 1.0/36.0, 1.0/36.0, 1.0/36.0, 1.0/36.0, 1.0/36.0, 1.0/36.0, 1.0/36.0,
+        1.0/36.0, 1.0/36.0, 1.0/36.0};
+
+/* 1/sqrt(b_k) of the orthogonal basis of Lulli et al. (norms in D3Q19.h:136-137). */
+constexpr double INV_SQRT_NORM[NQ] = {1.0, 1.7320508075688772, 1.7320508075688772, 1.7320508075688772,
+        2.1213203435596424, 2.1213203435596424, 2.1213203435596424, 3.0, 3.0, 3.0,
+        3.6742346141747673, 3.6742346141747673, 3.6742346141747673, 4.242640687119285, 4.242640687119285,
+        4.242640687119285, 4.810702354423639, 5.019960159204453, 5.477225575051661};
+
+/* The basis polynomial e_k evaluated at c_q (D3Q19.h:145-169), as a constant expression. */
+constexpr double basisMode(int k, int q) {
+    double x = CX[q], y = CY[q], z = CZ[q];
+    double x2 = x*x, y2 = y*y, z2 = z*z;
+    return k == 0 ? 1.0 : k == 1 ? x : k == 2 ? y : k == 3 ? z :
+           k == 4 ? x2 - 1.0/3.0 : k == 5 ? y2 - 1.0/3.0 : k == 6 ? z2 - 1.0/3.0 :
+           k == 7 ? x*y : k == 8 ? x*z : k == 9 ? y*z :
+           k == 10 ? y*(x2 - 1.0/3.0) : k == 11 ? z*(x2 - 1.0/3.0) : k == 12 ? x*(y2 - 1.0/3.0) :
+           k == 13 ? 0.5*x*(y2 + 2.0*z2 - 1.0) : k == 14 ? 0.5*y*(x2 + 2.0*z2 - 1.0) :
+           k == 15 ? 0.5*z*(x2 + 2.0*y2 - 1.0) :
+           k == 16 ? x2*y2 - x2/3.0 - y2/3.0 + z2/6.0 + 1.0/18.0 :
+           k == 17 ? (2.0/7.0)*x2*y2 + x2*z2 - (3.0/7.0)*x2 + y2/14.0 - (2.0/7.0)*z2 + 1.0/14.0 :
+           0.4*x2*y2 + 0.4*x2*z2 + y2*z2 - 0.1*x2 - 0.4*y2 - 0.4*z2 + 0.1;
+}
+
+/* Compile-time loop: f(std::integral_constant<int, i>) for i = 0..N-1. */
+template <typename F, int... Is>
+__device__ __forceinline__ void staticForImpl(F&& f, std::integer_sequence<int, Is...>) {
+    (f(std::integral_constant<int, Is>{}), ...);
+}
+template <int N, typename F>
+__device__ __forceinline__ void staticFor(F&& f) {
+    staticForImpl(f, std::make_integer_sequence<int, N>{});
+}
+
+/* c*x for a constant integer c in {-1, 0, 1}, added to acc without ever multiplying by 0. */
+template <int C, typename R>
+__device__ __forceinline__ R addTerm(R acc, R x) {
+    if constexpr (C == 1)
+        return acc + x;
+    else if constexpr (C == -1)
+        return acc - x;
+    else
+        return acc;
+}
+
+/* Moments of one node from its 19 deviations: drho, j, and the raw second moments sum c c df. */
+template <typename R>
+struct Moments {
+    R dr, jx, jy, jz, sxx, syy, szz, sxy, sxz, syz;
+};
+
+template <typename R>
+__device__ __forceinline__ Moments<R> nodeMoments(const R f[NQ]) {
+    Moments<R> m;
+    m.dr = f[0];
+    m.jx = m.jy = m.jz = m.sxx = m.syy = m.szz = m.sxy = m.sxz = m.syz = R(0);
+    staticFor<NQ - 1>([&](auto I) {
+        constexpr int q = decltype(I)::value + 1;
+        R v = f[q];
+        m.dr += v;
+        m.jx = addTerm<CX[q]>(m.jx, v);
+        m.jy = addTerm<CY[q]>(m.jy, v);
+        m.jz = addTerm<CZ[q]>(m.jz, v);
+        m.sxx = addTerm<CX[q]*CX[q]>(m.sxx, v);
+        m.syy = addTerm<CY[q]*CY[q]>(m.syy, v);
+        m.szz = addTerm<CZ[q]*CZ[q]>(m.szz, v);
+        m.sxy = addTerm<CX[q]*CY[q]>(m.sxy, v);
+        m.sxz = addTerm<CX[q]*CZ[q]>(m.sxz, v);
+        m.syz = addTerm<CY[q]*CZ[q]>(m.syz, v);
+    });
+    return m;
+}
+
+/* Pi_neq (xx, yy, zz, xy, xz, yz) with the equilibrium at j/rho (D3Q19.h:98-110). */
+template <typename R>
+__device__ __forceinline__ void piNeq(const Moments<R>& m, R pi[6]) {
+    R rho = R(1) + m.dr;
+    R inv = R(1)/rho;
+    R third = m.dr*R(1.0/3.0);
+    pi[0] = m.sxx - third - m.jx*m.jx*inv;
+    pi[1] = m.syy - third - m.jy*m.jy*inv;
+    pi[2] = m.szz - third - m.jz*m.jz*inv;
+    pi[3] = m.sxy - m.jx*m.jy*inv;
+    pi[4] = m.sxz - m.jx*m.jz*inv;
+    pi[5] = m.syz - m.jy*m.jz*inv;
+}
+
+/*
+ * Post-collision deviations out[q] = C_q - w_q from drho, the momentum j (after the removal),
+ * Pi_neq and the force density F; stress[6] is an extra symmetric tensor added to P (the stress
+ * modes of the fluctuating part, already divided by 4.5, or zeros), ghost[q] the ghost part of
+ * the fluctuating part divided by w_q (or nullptr).
+ */
+template <typename R, bool GHOST>
+__device__ __forceinline__ void collide(R dr, R jx, R jy, R jz, const R pi[6], R fx, R fy, R fz, R omega,
+        const R stress[6], const R* ghost, R out[NQ]) {
+    R rho = R(1) + dr;
+    R inv = R(1)/rho;
+    R ux = (jx + R(0.5)*fx)*inv, uy = (jy + R(0.5)*fy)*inv, uz = (jz + R(0.5)*fz)*inv;
+    R relax = R(1) - omega;
+    R pxx = rho*ux*ux + relax*pi[0] + ux*fx + stress[0];
+    R pyy = rho*uy*uy + relax*pi[1] + uy*fy + stress[1];
+    R pzz = rho*uz*uz + relax*pi[2] + uz*fz + stress[2];
+    R pxy = rho*ux*uy + relax*pi[3] + R(0.5)*(ux*fy + uy*fx) + stress[3];
+    R pxz = rho*ux*uz + relax*pi[4] + R(0.5)*(ux*fz + uz*fx) + stress[4];
+    R pyz = rho*uy*uz + relax*pi[5] + R(0.5)*(uy*fz + uz*fy) + stress[5];
+    R mx = jx + fx, my = jy + fy, mz = jz + fz;
+    R t = (pxx + pyy + pzz)*R(1.0/3.0);
+    // Shared parts of the diagonal terms: dr + 4.5 (P_aa + P_bb - T) for the three planes.
+    R base0 = dr - R(4.5)*t;
+    R ax = base0 + R(4.5)*pxx, ay = base0 + R(4.5)*pyy, az = base0 + R(4.5)*pzz;
+    R dxy = ax + R(4.5)*pyy, dxz = ax + R(4.5)*pzz, dyz = ay + R(4.5)*pzz;
+    R sxy = R(9)*pxy, sxz = R(9)*pxz, syz = R(9)*pyz;
+    R m3x = R(3)*mx, m3y = R(3)*my, m3z = R(3)*mz;
+    R e[NQ];
+    e[0] = base0;
+    e[1] = ax + m3x;  e[2] = ax - m3x;
+    e[3] = ay + m3y;  e[4] = ay - m3y;
+    e[5] = az + m3z;  e[6] = az - m3z;
+    e[7] = dxy + sxy + (m3x + m3y);   e[8] = dxy + sxy - (m3x + m3y);
+    e[9] = dxy - sxy + (m3x - m3y);   e[10] = dxy - sxy - (m3x - m3y);
+    e[11] = dyz + syz + (m3y + m3z);  e[12] = dyz + syz - (m3y + m3z);
+    e[13] = dyz - syz + (m3y - m3z);  e[14] = dyz - syz - (m3y - m3z);
+    e[15] = dxz + sxz + (m3x + m3z);  e[16] = dxz + sxz - (m3x + m3z);
+    e[17] = dxz - sxz - (m3x - m3z);  e[18] = dxz - sxz + (m3x - m3z);
+#pragma unroll
+    for (int q = 0; q < NQ; q++) {
+        R v = e[q];
+        if constexpr (GHOST)
+            v += ghost[q];
+        out[q] = R(WQ[q])*v;
+    }
+}
+
+/*
+ * Ghost part of the fluctuating collision divided by w_q: g_q = amp * sum_{k=10..18} e_k(c_q)/sqrt(b_k) r[k-4]
+ * (D3Q19.h:179-186), with the constant basis folded at compile time.
+ */
+template <typename R>
+__device__ __forceinline__ void ghostNoise(const R r[16], R amp, R g[NQ]) {
+    staticFor<NQ>([&](auto I) {
+        constexpr int q = decltype(I)::value;
+        R sum = R(0);
+        staticFor<9>([&](auto K) {
+            constexpr int k = decltype(K)::value + 10;
+            constexpr double coef = basisMode(k, q)*INV_SQRT_NORM[k];
+            if constexpr (coef != 0.0)
+                sum += R(coef)*r[k - 4];
+        });
+        g[q] = amp*sum;
+    });
+}
+
+/* The stress modes (k = 4..9) of the fluctuating part as a tensor added to P: the contribution
+   w_q sum_k e_k/sqrt(b_k) amp r_k equals w_q 4.5 H2:S with S = diag(a4, a5, a6)/4.5 and
+   off-diagonal a7/9, a8/9, a9/9 (a_k = amp r_k/sqrt(b_k)). */
+template <typename R>
+__device__ __forceinline__ void stressNoise(const R r[16], R amp, R s[6]) {
+    R d = amp*R(INV_SQRT_NORM[4]/4.5), o = amp*R(INV_SQRT_NORM[7]/9.0);
+    s[0] = d*r[0];
+    s[1] = d*r[1];
+    s[2] = d*r[2];
+    s[3] = o*r[3];
+    s[4] = o*r[4];
+    s[5] = o*r[5];
+}
+
+/* Stress part of the fluctuating collision, per population (w_q included): D3Q19::fluctuation() with
+   a zero ghost amplitude. */
+template <typename R>
+__device__ __forceinline__ void stressNoisePopulations(const R r[16], R amp, R out[NQ]) {
+    staticFor<NQ>([&](auto I) {
+        constexpr int q = decltype(I)::value;
+        R sum = R(0);
+        staticFor<6>([&](auto K) {
+            constexpr int k = decltype(K)::value + 4;
+            constexpr double coef = basisMode(k, q)*INV_SQRT_NORM[k];
+            if constexpr (coef != 0.0)
+                sum += R(coef)*r[k - 4];
+        });
+        out[q] = R(WQ[q])*amp*sum;
+    });
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* Philox4x32-10 (Salmon et al., SC'11) and Box-Muller normals.                               */
+/* ------------------------------------------------------------------------------------------ */
+constexpr uint32_t PHILOX_M0 = 0xD2511F53u, PHILOX_M1 = 0xCD9E8D57u;
+constexpr uint32_t PHILOX_W0 = 0x9E3779B9u, PHILOX_W1 = 0xBB67AE85u;
+constexpr uint32_t TAG_FLUID = 0x464C5549u, TAG_BOUNDARY = 0x424F554Eu, TAG_PARTICLE = 0x50415254u;
+
+struct uint4x {
+    uint32_t x, y, z, w;
+};
+
+__device__ __forceinline__ uint4x philox(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0, uint32_t k1) {
+#pragma unroll
+    for (int round = 0; round < 10; round++) {
+        uint64_t p0 = (uint64_t) PHILOX_M0*c0;
+        uint64_t p1 = (uint64_t) PHILOX_M1*c2;
+        uint32_t n0 = (uint32_t) (p1 >> 32) ^ c1 ^ k0;
+        uint32_t n2 = (uint32_t) (p0 >> 32) ^ c3 ^ k1;
+        c1 = (uint32_t) p1;
+        c3 = (uint32_t) p0;
+        c0 = n0;
+        c2 = n2;
+        k0 += PHILOX_W0;
+        k1 += PHILOX_W1;
+    }
+    return {c0, c1, c2, c3};
+}
+
+/* Four normals from one Philox block: u1 = (a + 1) 2^-32, u2 = b 2^-32. */
+template <typename R>
+__device__ __forceinline__ void boxMuller4(uint4x b, R* out) {
+    if constexpr (std::is_same<R, double>::value) {
+        const double s = 2.3283064365386963e-10, twoPi = 6.283185307179586476925287;
+        double r0 = sqrt(-2.0*log(((double) b.x + 1.0)*s)), a0 = twoPi*((double) b.y*s);
+        double r1 = sqrt(-2.0*log(((double) b.z + 1.0)*s)), a1 = twoPi*((double) b.w*s);
+        out[0] = r0*cos(a0);
+        out[1] = r0*sin(a0);
+        out[2] = r1*cos(a1);
+        out[3] = r1*sin(a1);
+    }
+    else {
+        const float s = 2.3283064365386963e-10f, twoPi = 6.28318530717958647f;
+        float r0 = __fsqrt_rn(-2.0f*__logf(fmaf((float) b.x, s, s)));
+        float r1 = __fsqrt_rn(-2.0f*__logf(fmaf((float) b.z, s, s)));
+        float a0 = twoPi*((float) b.y*s), a1 = twoPi*((float) b.w*s);
+        out[0] = r0*__cosf(a0);
+        out[1] = r0*__sinf(a0);
+        out[2] = r1*__cosf(a1);
+        out[3] = r1*__sinf(a1);
+    }
+}
+
+/* count normals (a multiple of 4 rounded up) for (seed, tag, index, step). */
+template <typename R, int COUNT>
+__device__ __forceinline__ void drawNormals(uint32_t seed, uint32_t tag, uint32_t index, int64_t step, R* out) {
+#pragma unroll
+    for (int sub = 0; sub < (COUNT + 3)/4; sub++) {
+        uint4x b = philox(index, (uint32_t) (uint64_t) step, (uint32_t) ((uint64_t) step >> 32), (uint32_t) sub, seed, tag);
+        boxMuller4<R>(b, out + 4*sub);
+    }
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* Storage types.                                                                             */
+/* ------------------------------------------------------------------------------------------ */
+template <typename S> struct StorageTraits;
+template <> struct StorageTraits<float> { using Real = float; };
+template <> struct StorageTraits<double> { using Real = double; };
+template <> struct StorageTraits<__half> { using Real = float; };
+template <> struct StorageTraits<__hip_bfloat16> { using Real = float; };
+
+template <typename S>
+__device__ __forceinline__ typename StorageTraits<S>::Real loadPop(const S* p) {
+    if constexpr (std::is_same<S, __half>::value)
+        return __half2float(*p);
+    else if constexpr (std::is_same<S, __hip_bfloat16>::value)
+        return __uint_as_float(((uint32_t) __builtin_bit_cast(uint16_t, *p)) << 16);
+    else
+        return *p;
+}
+
+/* Integer hash for the stochastic rounding of reduced-precision stores (a murmur3-style finalizer). */
+__device__ __forceinline__ uint32_t mixBits(uint32_t h) {
+    h ^= h >> 16;
+    h *= 0x7FEB352Du;
+    h ^= h >> 15;
+    h *= 0x846CA68Bu;
+    h ^= h >> 16;
+    return h;
+}
+
+/* Store with stochastic rounding for FP16/BF16 (gfx950 v_cvt_sr_*), plain stores otherwise. */
+template <typename S>
+__device__ __forceinline__ void storePop(S* p, typename StorageTraits<S>::Real v, uint32_t rnd) {
+    if constexpr (std::is_same<S, __half>::value) {
+        typedef _Float16 half2v __attribute__((ext_vector_type(2)));
+        half2v h = {0, 0};
+        h = __builtin_amdgcn_cvt_sr_f16_f32(h, v, rnd, false);
+        *reinterpret_cast<_Float16*>(p) = h[0];
+    }
+    else if constexpr (std::is_same<S, __hip_bfloat16>::value) {
+        typedef __bf16 bf2v __attribute__((ext_vector_type(2)));
+        bf2v h = {0, 0};
+        h = __builtin_amdgcn_cvt_sr_bf16_f32(h, v, rnd, false);
+        *reinterpret_cast<__bf16*>(p) = h[0];
+    }
+    else {
+        (void) rnd;
+        *p = v;
+    }
+}
+
+template <typename S>
+constexpr bool isReduced() {
+    return std::is_same<S, __half>::value || std::is_same<S, __hip_bfloat16>::value;
+}
+
+/* ------------------------------------------------------------------------------------------ */
+/* Lattice geometry.                                                                          */
+/* ------------------------------------------------------------------------------------------ */
+
+/* Unsigned division by an invariant integer (Granlund and Montgomery 1994, the round-up
+   variant with a 33-bit multiplier): q = (t + ((n - t) >> 1)) >> (l - 1), t = mulhi(m, n). */
+struct FastDiv {
+    uint32_t d, m, l;
+};
+
+__device__ __forceinline__ uint32_t fastDiv(uint32_t n, FastDiv f) {
+    if (f.d == 1)
+        return n;
+    uint32_t t = __umulhi(f.m, n);
+    return (t + ((n - t) >> 1)) >> (f.l - 1);
+}
+
+struct Geometry {
+    uint32_t nx, ny, nz, nxy, n;
+    FastDiv divx, divxy;
+    int openX, openY, openZ;
+};
+
+/* The offsets that take a node to its neighbours one step along -c (the pull source) on a periodic lattice. */
+struct Neighbours {
+    int xm, xp, ym, yp, zm, zp;   // offsets to (i-1), (i+1), (j-1), (j+1), (k-1), (k+1), wrapped
+};
+
+__device__ __forceinline__ void nodeCoords(const Geometry& g, uint32_t node, uint32_t& i, uint32_t& j, uint32_t& k) {
+    k = fastDiv(node, g.divxy);
+    uint32_t rem = node - k*g.nxy;
+    j = fastDiv(rem, g.divx);
+    i = rem - j*g.nx;
+}
+
+__device__ __forceinline__ Neighbours neighbours(const Geometry& g, uint32_t i, uint32_t j, uint32_t k) {
+    Neighbours nb;
+    nb.xm = i > 0 ? -1 : (int) g.nx - 1;
+    nb.xp = i + 1 < g.nx ? 1 : 1 - (int) g.nx;
+    nb.ym = j > 0 ? -(int) g.nx : (int) (g.nxy - g.nx);
+    nb.yp = j + 1 < g.ny ? (int) g.nx : (int) g.nx - (int) g.nxy;
+    nb.zm = k > 0 ? -(int) g.nxy : (int) (g.n - g.nxy);
+    nb.zp = k + 1 < g.nz ? (int) g.nxy : (int) g.nxy - (int) g.n;
+    return nb;
+}
+
+/* Offset of the node x - c_q (the pull source of direction q) from x. */
+template <int Q>
+__device__ __forceinline__ int pullOffset(const Neighbours& nb) {
+    int off = 0;
+    if constexpr (CX[Q] == 1) off += nb.xm;
+    if constexpr (CX[Q] == -1) off += nb.xp;
+    if constexpr (CY[Q] == 1) off += nb.ym;
+    if constexpr (CY[Q] == -1) off += nb.yp;
+    if constexpr (CZ[Q] == 1) off += nb.zm;
+    if constexpr (CZ[Q] == -1) off += nb.zp;
+    return off;
+}
+
+/* Offset of the node x + c_q from x. */
+template <int Q>
+__device__ __forceinline__ int pushOffset(const Neighbours& nb) {
+    return pullOffset<OPP[Q]>(nb);
+}
+
+/* Pull the 19 pre-collision deviations of a node: f_q(x) = post-collision f_q stored at x - c_q. */
+template <typename S>
+__device__ __forceinline__ void pullNode(const S* __restrict__ src, uint32_t n, uint32_t node, const Neighbours& nb,
+        typename StorageTraits<S>::Real f[NQ]) {
+    staticFor<NQ>([&](auto I) {
+        constexpr int q = decltype(I)::value;
+        f[q] = loadPop(src + (size_t) q*n + (uint32_t) ((int) node + pullOffset<q>(nb)));
+    });
+}
+
+/* Solid-node bitmask (bit node&31 of word node>>5). */
+__device__ __forceinline__ bool isSolid(const uint32_t* __restrict__ mask, uint32_t node) {
+    return (mask[node >> 5] >> (node & 31)) & 1u;
+}
+
+/* Block index remapped so that each XCD (blocks are dealt round-robin to the 8 XCDs) works on one
+   contiguous range of the lattice and keeps its neighbouring rows in its own L2. */
+__device__ __forceinline__ uint32_t xcdBlock(uint32_t b, uint32_t nb) {
+    uint32_t per = nb >> 3, rem = nb & 7, xcd = b & 7, idx = b >> 3;
+    return xcd < rem ? xcd*(per + 1) + idx : rem*(per + 1) + (xcd - rem)*per + idx;
+}
+
+}  // namespace lbm

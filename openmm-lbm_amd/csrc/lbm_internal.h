@@ -1,0 +1,102 @@
+/*
+ * Internal declarations shared by the host orchestration (lbm_context.hip) and the kernel
+ * launchers (lbm_fluid.hip, lbm_particles.hip).  Not part of the public C-ABI.
+ */
+#pragma once
+
+#include "lbm_device.h"
+#include "../../include/lbm_amd.h"
+#include <string>
+#include <vector>
+
+namespace lbm {
+
+constexpr int BLOCK = 256;
+constexpr int REDUCE_BLOCKS = 1024;       // fixed grid of the FP64 reductions: fixed summation order
+constexpr double FIXED_SCALE = 1125899906842624.0;   // 2^50: fixed-point scale of the particle reactions
+
+/* Arguments of one fused pull-stream/collide step. */
+struct StepArgs {
+    Geometry g;
+    double omega;
+    double gx, gy, gz;            // body acceleration, lattice units
+    int removal;                  // 1: subtract rho*ucm from j
+    uint32_t seed;
+    int64_t step;
+    double mu;                    // 3 kT in lattice units (fluctuations)
+    uint32_t roundSeed;           // stochastic-rounding seed of the step
+};
+
+/* One boundary node of a regularized wall or an open face (LBMBoundaries.h:46-109). */
+struct BoundaryNode {
+    int node;
+    int unknown;     // bits 1<<q of the unknown directions
+    int solid;       // bits 1<<q of the unknown directions whose source is solid
+    int kindFace;    // kind + 8*(face + 1)
+};
+
+struct BoundaryArgs {
+    Geometry g;
+    int count;
+    double omega, gx, gy, gz;
+    int removal;
+    int fluct;
+    uint32_t seed;
+    int64_t step;
+    double mu;
+    double faceVelocity[6][3];
+    double faceDensity[6];
+    uint32_t roundSeed;
+};
+
+struct ParticleArgs {
+    Geometry g;
+    int numParticles;
+    int stencil;            // LBM_STENCIL_*
+    double dx;              // nm
+    double gamma;           // friction*dt
+    double kT;              // lattice units
+    double velocityScale;   // dx/dt
+    double forceScale;      // m_c dx/dt^2
+    double momentumScale;   // m_c dx/dt (reactions at solid nodes -> wall momentum)
+    double cellMass;
+    int removal;
+    int isStep;
+    int randomForce;
+    uint32_t seed;
+    int64_t step;
+    int hasSolids;
+};
+
+/* Host-side launchers (defined in lbm_fluid.hip / lbm_particles.hip). */
+void launchFused(int storage, bool solids, bool reaction, bool fluct, const void* src, void* dst, const uint32_t* mask,
+        const void* reactionField, const double* ucm, const StepArgs& a, hipStream_t stream);
+void launchMomentReduce(int storage, bool solids, const void* src, const uint32_t* mask, const Geometry& g,
+        double* partials, hipStream_t stream);
+void launchFinishReduce(const double* partials, double numFluid, double* totals, double* ucm, hipStream_t stream);
+void launchBounceBack(int storage, void* dst, const int* wallNodes, const int* wallLinks, int count, const Geometry& g,
+        double* partials, hipStream_t stream);
+void launchBoundaries(int storage, const void* src, void* dst, const BoundaryNode* nodes, const double* ucm,
+        const BoundaryArgs& a, double* partials, hipStream_t stream);
+void launchGather(int storage, const void* src, double* out, const Geometry& g, hipStream_t stream);
+void launchScatter(int storage, void* dst, const double* in, const Geometry& g, hipStream_t stream);
+void launchFields(int storage, const void* src, const uint32_t* mask, const Geometry& g, double density, double velocityScale,
+        double gx, double gy, double gz, double* rhoOut, double* uOut, hipStream_t stream);
+void launchInitEquilibrium(int storage, void* dst, const uint32_t* mask, const Geometry& g, double ux, double uy,
+        double uz, hipStream_t stream);
+void launchFill(int storage, void* dst, size_t count, hipStream_t stream);
+void launchReduceWall(const double* partials, int count, double* out, hipStream_t stream);
+
+void launchReflect(const double* positions, double* velocities, const int* particles, const double* mass,
+        const uint32_t* mask, const ParticleArgs& a, double* partials, hipStream_t stream);
+void launchCouple(int storage, const void* src, const uint32_t* mask, const double* positions, const double* velocities,
+        const int* particles, const double* mass, const double* ucm, const ParticleArgs& a, double* particleForce,
+        unsigned long long* reactionFixed, int* entryNodes, double* partials, hipStream_t stream);
+void launchFinalizeReaction(unsigned long long* reactionFixed, void* reaction, bool fp64, const int* entryNodes, int count,
+        uint32_t n, hipStream_t stream);
+void launchClearReaction(void* reaction, bool fp64, const int* entryNodes, int count, uint32_t n, hipStream_t stream);
+void launchAddForces(const double* particleForce, const int* particles, int count, double* forces, hipStream_t stream);
+
+__host__ __device__ int stencilWidth(int stencil);
+
+}  // namespace lbm

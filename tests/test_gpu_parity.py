@@ -1,0 +1,265 @@
+"""Parity of the gfx950 library with the FP64 CPU oracle (same setups, same Philox streams), through the C-ABI.
+
+Tolerances: FP32 storage and arithmetic, 1e-5 relative (max |gpu - oracle| / max |oracle|) for density,
+velocity and coupled-particle forces at every checked step; FP64 storage 1e-11.
+"""
+import numpy as np
+import pytest
+
+from lbm_amd import LBMForce
+from parity import Setup, equilibrium_state, lockstep, rel_err, total_moments
+
+pytestmark = pytest.mark.gpu
+
+TOL32 = 1e-5
+
+
+def shear_wave_setup(n=32, tau=0.8, amplitude=0.01, g=1e-5, dt=0.01, dx=1.0):
+    """32^3 periodic D3Q19, u_x = 0.01 sin(2 pi z/32), uniform Guo force 1e-5 (lattice units) along x."""
+    s = Setup((n, n, n), dx=dx, dt=dt, tau=tau)
+    z = np.arange(n*n*n)//(n*n)
+    u = np.zeros((n*n*n, 3))
+    u[:, 0] = amplitude*np.sin(2*np.pi*z/n)
+    s.state = equilibrium_state(n*n*n, 1.0, u)
+    s.configure.append(lambda f: f.setBodyAcceleration((g*dx/(dt*dt), 0, 0)))
+    return s
+
+
+def config1_setup(n=32, num_beads=64, stencil=LBMForce.Trilinear, seed=1234):
+    """The CPU-runnable configuration: 32^3 periodic, dx = 1 nm, tau = 0.8, shear wave 0.01 sin(2 pi z/32), Guo force
+    1e-5 along x, 64 free beads (100 Da) at seeded uniform positions, v = 0, gamma = 10/ps, trilinear, T = 0."""
+    s = shear_wave_setup(n)
+    rng = np.random.default_rng(seed)
+    s.add_particles(rng.uniform(0, n, size=(num_beads, 3)), mass=100.0)
+
+    def conf(f):
+        f.setFriction(10.0)
+        f.setTemperature(0.0)
+        f.setInterpolationStencil(stencil)
+    s.configure.append(conf)
+    return s
+
+
+def test_fluid_periodic_parity():
+    worst, ref, gpu = lockstep(shear_wave_setup(), 100, check_every=5, forces=False)
+    assert worst["density"] < TOL32 and worst["velocity"] < TOL32, worst
+
+
+def test_config1_parity_1000_steps():
+    """Every step for 1000 steps: fields, bead forces and bead positions agree to 1e-5."""
+    worst, ref, gpu = lockstep(config1_setup(), 1000, check_every=1)
+    print("config[1] worst relative errors:", worst)
+    assert worst["density"] < TOL32, worst
+    assert worst["velocity"] < TOL32, worst
+    assert worst["force"] < TOL32, worst
+
+
+@pytest.mark.parametrize("stencil", [LBMForce.NearestNode, LBMForce.Trilinear, LBMForce.ThreePoint, LBMForce.Keys])
+def test_stencil_coupling_parity(stencil):
+    s = config1_setup(n=16, num_beads=24, stencil=stencil, seed=7)
+    worst, ref, gpu = lockstep(s, 60, check_every=3)
+    assert max(worst.values()) < TOL32, worst
+
+
+def test_node_assignment_bit_exact():
+    """Particle-to-node assignment at ties: positions on half-integer planes go to the same node as on the CPU."""
+    s = config1_setup(n=8, num_beads=0, stencil=LBMForce.NearestNode)
+    pos = np.array([[0.5, 1.5, 2.5], [7.5, 7.4999999999, 0.5000000001], [-0.5, 8.5, 16.5], [3.0, 3.0, 3.0]])
+    s.add_particles(pos, mass=100.0, velocities=np.ones((4, 3))*0.1)
+    worst, ref, gpu = lockstep(s, 3, check_every=1)
+    assert worst["force"] < TOL32, worst
+
+
+def test_removal_parity():
+    s = shear_wave_setup(n=12)
+    s.configure.append(lambda f: f.setFluidMomentumRemovalFrequency(3))
+    worst, ref, gpu = lockstep(s, 20, check_every=1, forces=False)
+    assert worst["density"] < TOL32 and worst["velocity"] < TOL32, worst
+
+
+def wall_setup(scheme, g=1e-5, tau=0.7):
+    nx, ny, nz = 6, 10, 5
+    s = Setup((nx, ny, nz), dx=0.5, dt=0.01, tau=tau)
+    solid = [i + nx*(j + ny*k) for k in range(nz) for j in range(ny) for i in range(nx)
+             if j == 0 or (2 <= i < 4 and 3 <= j < 5 and 1 <= k < 3)]
+
+    def conf(f):
+        f.setSolidNodes(solid)
+        f.setWallScheme(scheme)
+        f.setInitialFluidVelocity((0.02*0.5/0.01, -0.01*0.5/0.01, 0.005*0.5/0.01))
+        f.setBodyAcceleration((g*0.5/1e-4, 2*g*0.5/1e-4, -g*0.5/1e-4))
+    s.configure.append(conf)
+    return s
+
+
+@pytest.mark.parametrize("scheme", [LBMForce.BounceBack, LBMForce.Regularized])
+def test_wall_parity(scheme):
+    def wall(step, ref, gpu):
+        w0, w1 = ref.getWallForce(), gpu.getWallForce()
+        assert rel_err(w1, w0) < 1e-5, (step, w0, w1)
+    worst, ref, gpu = lockstep(wall_setup(scheme), 40, check_every=1, forces=False, callback=wall)
+    assert worst["density"] < TOL32 and worst["velocity"] < TOL32, worst
+
+
+@pytest.mark.parametrize("faces", ["velocity", "density", "density_velocity", "mixed"])
+def test_face_parity(faces):
+    s = Setup((5, 7, 6), dx=0.5, dt=0.01, tau=0.8)
+    U = 0.01*0.5/0.01
+
+    def conf(f):
+        if faces == "velocity":
+            f.setFaceBoundary(LBMForce.ZMin, LBMForce.Velocity)
+            f.setFaceBoundary(LBMForce.ZMax, LBMForce.Velocity)
+            f.setFaceVelocity(LBMForce.ZMax, (U, 0.3*U, 0))
+        elif faces == "density":
+            f.setFaceBoundary(LBMForce.YMin, LBMForce.Velocity)
+            f.setFaceVelocity(LBMForce.YMin, (0, U, 0))
+            f.setFaceBoundary(LBMForce.YMax, LBMForce.Density)
+        elif faces == "density_velocity":
+            f.setFaceBoundary(LBMForce.XMin, LBMForce.DensityVelocity)
+            f.setFaceBoundary(LBMForce.XMax, LBMForce.DensityVelocity)
+            f.setFaceVelocity(LBMForce.XMin, (U, 0, 0))
+            f.setFaceDensity(LBMForce.XMax, 602.214*1.001)
+        else:
+            for face in range(6):
+                f.setFaceBoundary(face, [LBMForce.Velocity, LBMForce.Density, LBMForce.DensityVelocity][face % 3])
+            f.setFaceVelocity(LBMForce.XMin, (U, 0, 0))
+            f.setFaceDensity(LBMForce.YMin, 602.214*0.999)
+    s.configure.append(conf)
+    worst, ref, gpu = lockstep(s, 30, check_every=1, forces=False)
+    assert worst["density"] < TOL32 and worst["velocity"] < TOL32, worst
+
+
+def test_regularized_walls_with_faces_parity():
+    nx, ny, nz = 6, 8, 7
+    s = Setup((nx, ny, nz), dx=0.5, dt=0.01, tau=0.9)
+    solid = [i + nx*(j + ny*k) for k in range(nz) for j in range(ny) for i in range(nx) if j == 0]
+
+    def conf(f):
+        f.setSolidNodes(solid)
+        f.setWallScheme(LBMForce.Regularized)
+        f.setFaceBoundary(LBMForce.ZMin, LBMForce.Velocity)
+        f.setFaceBoundary(LBMForce.ZMax, LBMForce.Density)
+        f.setFaceVelocity(LBMForce.ZMin, (0, 0, 0.5))
+    s.configure.append(conf)
+    worst, ref, gpu = lockstep(s, 25, check_every=1, forces=False)
+    assert worst["density"] < TOL32 and worst["velocity"] < TOL32, worst
+
+
+def test_fluctuations_same_stream_parity():
+    """With the same Philox stream the fluctuating fluid follows the oracle's trajectory (FP32 normals from hardware
+    transcendentals agree with the FP64 ones to ~1e-6 relative)."""
+    s = shear_wave_setup(n=10, amplitude=0.0, g=0.0)
+
+    def conf(f):
+        f.setFluidFluctuations(True)
+        f.setTemperature(300.0)
+        f.setRandomNumberSeed(4321)
+    s.configure.append(conf)
+    s.dx = 2.0
+    worst, ref, gpu = lockstep(s, 20, check_every=1, forces=False)
+    assert worst["velocity"] < 1e-4 and worst["density"] < 1e-5, worst
+
+
+def test_fluctuating_walls_and_particles_parity():
+    nx, ny, nz = 8, 8, 8
+    s = Setup((nx, ny, nz), dx=2.0, dt=0.01, tau=0.8)
+    solid = [i + nx*(j + ny*k) for k in range(nz) for j in range(ny) for i in range(nx) if k == 0]
+    rng = np.random.default_rng(3)
+    s.add_particles(rng.uniform([0, 0, 3], [16, 16, 13], size=(10, 3)), mass=200.0)
+
+    def conf(f):
+        f.setSolidNodes(solid)
+        f.setWallScheme(LBMForce.Regularized)
+        f.setFluidFluctuations(True)
+        f.setTemperature(300.0)
+        f.setRandomNumberSeed(99)
+        f.setFriction(5.0)
+        f.setInterpolationStencil(LBMForce.ThreePoint)
+    s.configure.append(conf)
+    worst, ref, gpu = lockstep(s, 15, check_every=1)
+    assert worst["velocity"] < 1e-4 and worst["density"] < 1e-5 and worst["force"] < 1e-4, worst
+
+
+def test_fp64_storage_parity():
+    s = config1_setup(n=12, num_beads=10, stencil=LBMForce.Keys)
+    worst, ref, gpu = lockstep(s, 30, check_every=1, storage=LBMForce.FP64)
+    assert max(worst.values()) < 1e-11, worst
+
+
+def test_state_round_trip_and_checkpoint():
+    s = config1_setup(n=10, num_beads=5)
+    gpu = s.build()
+    gpu.step(5)
+    state = gpu.getFluidState()
+    gpu.setFluidState(state)
+    assert np.array_equal(gpu.getFluidState(), state)
+    ck = gpu.createCheckpoint()
+    gpu.step(3)
+    after = gpu.getFluidState()
+    gpu.loadCheckpoint(ck)
+    assert np.array_equal(gpu.getFluidState(), state)
+    gpu.setStepCount(5)
+    gpu.positions, gpu.velocities = gpu.positions, gpu.velocities
+
+
+def test_checkpoint_portable_between_oracle_and_gpu():
+    import oracle_lib
+    s = shear_wave_setup(n=8)
+    ref = s.build(oracle_lib.load_oracle())
+    ref.step(4)
+    gpu = s.build()
+    gpu.loadCheckpoint(ref.createCheckpoint())
+    for c in (ref, gpu):
+        c.setStepCount(4)
+        c.step(4)
+    r0, u0 = ref.getFluidFields()
+    r1, u1 = gpu.getFluidFields()
+    assert rel_err(u1, u0) < TOL32 and rel_err(r1, r0) < TOL32
+
+
+def test_queries_do_not_advance_fluid():
+    s = config1_setup(n=8, num_beads=4)
+    a = s.build()
+    b = s.build()
+    a.step(3)
+    b.step(1)
+    b.getForces()
+    b.getFluidMachNumber()
+    b.step(2)
+    assert np.array_equal(a.getFluidState(), b.getFluidState())
+
+
+def test_mach_check_raises():
+    from lbm_amd import LBMError
+    s = shear_wave_setup(n=8, amplitude=0.0, g=0.0)
+    n = 8**3
+    u = np.zeros((n, 3))
+    u[:, 0] = 0.25
+    s.state = equilibrium_state(n, 1.0, u)
+    s.configure.append(lambda f: f.setMachCheckFrequency(2))
+    gpu = s.build()
+    gpu.step(1)
+    with pytest.raises(LBMError, match="Mach number"):
+        gpu.step(1)
+
+
+def test_deterministic_runs():
+    s = config1_setup(n=16, num_beads=40, stencil=LBMForce.Keys)
+    a = s.build()
+    b = s.build()
+    a.step(20)
+    b.step(20)
+    assert np.array_equal(a.getFluidState(), b.getFluidState())
+    assert np.array_equal(a.positions, b.positions)
+
+
+def test_fluid_totals_fp64():
+    s = shear_wave_setup(n=16, g=0.0)
+    gpu = s.build()
+    m0, p0 = gpu.getFluidTotals()
+    gpu.step(50)
+    m1, p1 = gpu.getFluidTotals()
+    mass_ref, p_ref = total_moments(gpu.getFluidState())
+    assert abs(m1 - mass_ref) < 1e-9*mass_ref
+    assert abs(m1 - m0) < 1e-6*m0
