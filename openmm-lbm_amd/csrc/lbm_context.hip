@@ -12,9 +12,8 @@
  *               are keyed by the step index); a repeated evaluation reuses the forces.
  *   advanceFluid (one lattice step):
  *      [removal due]  momentReduce(A) -> FP64 u_cm
- *      [particles]    coupleParticles(A) -> forces, fixed-point reactions; finalizeReaction
- *      fusedStep(A -> B)   (pull-stream + collision + Guo + reaction + noise)
- *      [particles]    clearReaction
+ *      [particles]    coupleParticles(A) -> forces, fixed-point reactions, touched mask
+ *      fusedStep(A -> B)   (pull-stream + collision + Guo + reaction (consumed) + noise)
  *      [bounce-back]  bounceBack(B) + wall momentum
  *      [boundaries]   boundaryKernel(A, B) + wall momentum
  *      swap A, B; step index++; [Mach check due] momentReduce(A) -> host
@@ -132,9 +131,8 @@ struct lbm_context {
     std::vector<int> hostParticles;
     double* particleForce = nullptr;
     std::vector<double> hostForce;
-    void* reaction = nullptr;             // 3N values, FP64 with FP64 storage, FP32 otherwise
-    unsigned long long* reactionFixed = nullptr;
-    int* entryNodes = nullptr;
+    unsigned long long* reactionFixed = nullptr;   // 3N fixed-point accumulators of the particle reactions
+    uint32_t* touched = nullptr;                   // one bit per node: accumulators holding a reaction
     double* stagePos = nullptr;
     double* stageVel = nullptr;
     // Timing of the fused pass with HIP events on the context's stream (diagnostics).
@@ -301,17 +299,14 @@ void couple(lbm_context* c, const double* dPos, const double* dVel, int64_t step
         reduceMoments(c);
     double* partials = isStep ? c->wallPartials + 3*(size_t) c->wallCoupleOffset : nullptr;
     launchCouple(c->p.storage, c->popA, c->mask, dPos, dVel, c->particles, c->mass, c->ucm, a, c->particleForce,
-            c->reactionFixed, c->entryNodes, partials, c->stream);
+            c->reactionFixed, c->touched, partials, c->stream);
 }
 
 void advanceFluid(lbm_context* c, const double* dPos, const double* dVel) {
     StepArgs a = stepArgs(c, c->stepIndex);
     bool particles = c->numParticles > 0;
-    if (particles) {
+    if (particles)
         couple(c, dPos, dVel, c->stepIndex, true);   // runs the removal reduction when due
-        launchFinalizeReaction(c->reactionFixed, c->reaction, c->p.storage == LBM_STORAGE_FP64, c->entryNodes,
-                c->numParticles*c->numSlots, c->g.n, c->stream);
-    }
     else if (a.removal)
         reduceMoments(c);
     std::pair<hipEvent_t, hipEvent_t> ev{nullptr, nullptr};
@@ -320,15 +315,12 @@ void advanceFluid(lbm_context* c, const double* dPos, const double* dVel) {
         HIP_CHECK(hipEventCreate(&ev.second));
         HIP_CHECK(hipEventRecord(ev.first, c->stream));
     }
-    launchFused(c->p.storage, c->mask != nullptr, particles, c->fluct(), c->popA, c->popB, c->mask, c->reaction, c->ucm,
-            a, c->stream);
+    launchFused(c->p.storage, c->mask != nullptr, particles, c->fluct(), c->popA, c->popB, c->mask, c->reactionFixed,
+            c->touched, c->ucm, a, c->stream);
     if (c->profile) {
         HIP_CHECK(hipEventRecord(ev.second, c->stream));
         c->events.push_back(ev);
     }
-    if (particles)
-        launchClearReaction(c->reaction, c->p.storage == LBM_STORAGE_FP64, c->entryNodes, c->numParticles*c->numSlots,
-                c->g.n, c->stream);
     if (c->numWall > 0)
         launchBounceBack(c->p.storage, c->popB, c->wallNodes, c->wallLinks, c->numWall, c->g,
                 c->wallPartials + 3*(size_t) c->wallBounceOffset, c->stream);
@@ -417,8 +409,8 @@ void destroy(lbm_context* c) {
         (void) hipStreamSynchronize(c->stream);
     clearEvents(c);
     void* buffers[] = {c->popA, c->popB, c->mask, c->wallNodes, c->wallLinks, c->boundary, c->reducePartials, c->totals,
-                       c->ucm, c->wallPartials, c->particles, c->mass, c->particleForce, c->reaction, c->reactionFixed,
-                       c->entryNodes, c->stagePos, c->stageVel};
+                       c->ucm, c->wallPartials, c->particles, c->mass, c->particleForce, c->reactionFixed,
+                       c->touched, c->stagePos, c->stageVel};
     for (void* b : buffers)
         if (b)
             (void) hipFree(b);
@@ -530,12 +522,10 @@ void create(const lbm_params* params, const int* solid, int numSolid, const int*
         c->particleForce = deviceAlloc<double>(3*(size_t) numParts);
         HIP_CHECK(hipMemsetAsync(c->particleForce, 0, 3*sizeof(double)*numParts, c->stream));
         c->hostForce.assign(3*(size_t) numParts, 0.0);
-        size_t realBytes = p.storage == LBM_STORAGE_FP64 ? 8 : 4;
-        c->reaction = deviceAlloc<char>(3*n*realBytes);
         c->reactionFixed = deviceAlloc<unsigned long long>(3*n);
-        HIP_CHECK(hipMemsetAsync(c->reaction, 0, 3*n*realBytes, c->stream));
         HIP_CHECK(hipMemsetAsync(c->reactionFixed, 0, 3*n*sizeof(unsigned long long), c->stream));
-        c->entryNodes = deviceAlloc<int>((size_t) numParts*c->numSlots);
+        c->touched = deviceAlloc<uint32_t>((n + 31)/32);
+        HIP_CHECK(hipMemsetAsync(c->touched, 0, ((n + 31)/32)*sizeof(uint32_t), c->stream));
         c->stagePos = deviceAlloc<double>(3*(size_t) std::max(numSys, 1));
         c->stageVel = deviceAlloc<double>(3*(size_t) std::max(numSys, 1));
     }

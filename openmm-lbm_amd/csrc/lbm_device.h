@@ -75,6 +75,25 @@ __device__ __forceinline__ R addTerm(R acc, R x) {
         return acc;
 }
 
+/*
+ * w_q*v.  In FP32 the weights 1/3, 1/18, 1/36 round up by 3.0e-8 and 7.4e-9 relative: multiplied in
+ * every collision, that bias would scale the momentum by (1 + 7.4e-9) per step, 7e-6 after 1000
+ * steps.  The weight is split into w_hi = float(w) and w_lo = float(w - w_hi), and w*v is formed
+ * as fma(v, w_hi, v*w_lo), exact to about 1e-15 relative before the final rounding.
+ */
+template <typename R>
+__device__ __forceinline__ R weighted(int q, R v) {
+    if constexpr (std::is_same<R, float>::value) {
+        constexpr float hi0 = (float) (1.0/3.0), hi1 = (float) (1.0/18.0), hi2 = (float) (1.0/36.0);
+        constexpr float lo0 = (float) (1.0/3.0 - (double) hi0), lo1 = (float) (1.0/18.0 - (double) hi1),
+                lo2 = (float) (1.0/36.0 - (double) hi2);
+        float hi = q == 0 ? hi0 : (q < 7 ? hi1 : hi2), lo = q == 0 ? lo0 : (q < 7 ? lo1 : lo2);
+        return fmaf(v, hi, v*lo);
+    }
+    else
+        return R(WQ[q])*v;
+}
+
 /* Moments of one node from its 19 deviations: drho, j, and the raw second moments sum c c df. */
 template <typename R>
 struct Moments {
@@ -137,9 +156,9 @@ __device__ __forceinline__ void collide(R dr, R jx, R jy, R jz, const R pi[6], R
     R pxz = rho*ux*uz + relax*pi[4] + R(0.5)*(ux*fz + uz*fx) + stress[4];
     R pyz = rho*uy*uz + relax*pi[5] + R(0.5)*(uy*fz + uz*fy) + stress[5];
     R mx = jx + fx, my = jy + fy, mz = jz + fz;
-    R t = (pxx + pyy + pzz)*R(1.0/3.0);
-    // Shared parts of the diagonal terms: dr + 4.5 (P_aa + P_bb - T) for the three planes.
-    R base0 = dr - R(4.5)*t;
+    // Shared parts of the diagonal terms: dr + 4.5 (P_aa + P_bb - T), T = tr(P)/3, for the three planes
+    // (4.5 T = 1.5 tr(P): no rounded 1/3).
+    R base0 = dr - R(1.5)*(pxx + pyy + pzz);
     R ax = base0 + R(4.5)*pxx, ay = base0 + R(4.5)*pyy, az = base0 + R(4.5)*pzz;
     R dxy = ax + R(4.5)*pyy, dxz = ax + R(4.5)*pzz, dyz = ay + R(4.5)*pzz;
     R sxy = R(9)*pxy, sxz = R(9)*pxz, syz = R(9)*pyz;
@@ -160,7 +179,7 @@ __device__ __forceinline__ void collide(R dr, R jx, R jy, R jz, const R pi[6], R
         R v = e[q];
         if constexpr (GHOST)
             v += ghost[q];
-        out[q] = R(WQ[q])*v;
+        out[q] = weighted<R>(q, v);
     }
 }
 
@@ -242,48 +261,70 @@ __device__ __forceinline__ uint4x philox(uint32_t c0, uint32_t c1, uint32_t c2, 
     return {c0, c1, c2, c3};
 }
 
-/* Four normals from one Philox block: u1 = (a + 1) 2^-32, u2 = b 2^-32. */
+/*
+ * A pair of normals by the Box-Muller transform of a 32-bit word a (radius) and a 16-bit word b
+ * (angle): u1 = (a + 1) 2^-32 in (0, 1], u2 = b 2^-16 in [0, 1), r = sqrt(-2 ln u1),
+ * (r cos 2 pi u2, r sin 2 pi u2).  The angle needs no more than 16 bits, so three Philox blocks
+ * (12 words: 8 radii, 4 words of two angles each) give 16 normals.
+ */
 template <typename R>
-__device__ __forceinline__ void boxMuller4(uint4x b, R* out) {
+__device__ __forceinline__ void boxMullerPair(uint32_t a, uint32_t b, R* out) {
     if constexpr (std::is_same<R, double>::value) {
-        const double s = 2.3283064365386963e-10, twoPi = 6.283185307179586476925287;
-        double r0 = sqrt(-2.0*log(((double) b.x + 1.0)*s)), a0 = twoPi*((double) b.y*s);
-        double r1 = sqrt(-2.0*log(((double) b.z + 1.0)*s)), a1 = twoPi*((double) b.w*s);
-        out[0] = r0*cos(a0);
-        out[1] = r0*sin(a0);
-        out[2] = r1*cos(a1);
-        out[3] = r1*sin(a1);
+        const double twoPi = 6.283185307179586476925287;
+        double r = sqrt(-2.0*log(((double) a + 1.0)*2.3283064365386963e-10));
+        double angle = twoPi*((double) b*1.52587890625e-05);
+        out[0] = r*cos(angle);
+        out[1] = r*sin(angle);
     }
     else {
-        const float s = 2.3283064365386963e-10f, twoPi = 6.28318530717958647f;
-        float r0 = __fsqrt_rn(-2.0f*__logf(fmaf((float) b.x, s, s)));
-        float r1 = __fsqrt_rn(-2.0f*__logf(fmaf((float) b.z, s, s)));
-        float a0 = twoPi*((float) b.y*s), a1 = twoPi*((float) b.w*s);
-        out[0] = r0*__cosf(a0);
-        out[1] = r0*__sinf(a0);
-        out[2] = r1*__cosf(a1);
-        out[3] = r1*__sinf(a1);
+        const float twoPi = 6.28318530717958647f;
+        float r = sqrtf(-2.0f*__logf(fmaf((float) a, 2.3283064365386963e-10f, 2.3283064365386963e-10f)));
+        float angle = twoPi*((float) b*1.52587890625e-05f);
+        out[0] = r*__cosf(angle);
+        out[1] = r*__sinf(angle);
     }
 }
 
-/* count normals (a multiple of 4 rounded up) for (seed, tag, index, step). */
+/* COUNT (<= 20) normals for (seed, tag, index, step): blocks of 16 normals from three Philox calls with
+   counters (index, step_lo, step_hi, 3*block + c). */
 template <typename R, int COUNT>
 __device__ __forceinline__ void drawNormals(uint32_t seed, uint32_t tag, uint32_t index, int64_t step, R* out) {
+    uint32_t lo = (uint32_t) (uint64_t) step, hi = (uint32_t) ((uint64_t) step >> 32);
 #pragma unroll
-    for (int sub = 0; sub < (COUNT + 3)/4; sub++) {
-        uint4x b = philox(index, (uint32_t) (uint64_t) step, (uint32_t) ((uint64_t) step >> 32), (uint32_t) sub, seed, tag);
-        boxMuller4<R>(b, out + 4*sub);
+    for (int block = 0; block < (COUNT + 15)/16; block++) {
+        uint32_t w[12];
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            uint4x b = philox(index, lo, hi, (uint32_t) (3*block + c), seed, tag);
+            w[4*c] = b.x;
+            w[4*c + 1] = b.y;
+            w[4*c + 2] = b.z;
+            w[4*c + 3] = b.w;
+        }
+#pragma unroll
+        for (int p = 0; p < 8; p++) {
+            if (16*block + 2*p >= COUNT)
+                break;
+            uint32_t angle = (w[8 + p/2] >> (16*(p & 1))) & 0xFFFFu;
+            boxMullerPair<R>(w[p], angle, out + 16*block + 2*p);
+        }
     }
 }
 
 /* ------------------------------------------------------------------------------------------ */
 /* Storage types.                                                                             */
 /* ------------------------------------------------------------------------------------------ */
+/* Real: the type of values loaded from and stored to the storage (and of the reaction field).
+   Compute: the type of the moments and of the collision.  FP32 storage collides in FP64: summing
+   the moments of ~1e-3 deviations in FP32 injects ~1e-9 of rounding into j at every step, which
+   over 1000 steps drifts the velocity by ~5e-6 relative; in FP64 only the storage rounds (~1e-7).
+   On CDNA the non-packed FP64 vector rate equals the FP32 one and the pass is memory-bound.  FP16
+   and BF16 collide in FP32: their storage rounding dominates. */
 template <typename S> struct StorageTraits;
-template <> struct StorageTraits<float> { using Real = float; };
-template <> struct StorageTraits<double> { using Real = double; };
-template <> struct StorageTraits<__half> { using Real = float; };
-template <> struct StorageTraits<__hip_bfloat16> { using Real = float; };
+template <> struct StorageTraits<float> { using Real = float; using Compute = double; };
+template <> struct StorageTraits<double> { using Real = double; using Compute = double; };
+template <> struct StorageTraits<__half> { using Real = float; using Compute = float; };
+template <> struct StorageTraits<__hip_bfloat16> { using Real = float; using Compute = float; };
 
 template <typename S>
 __device__ __forceinline__ typename StorageTraits<S>::Real loadPop(const S* p) {
@@ -329,6 +370,42 @@ __device__ __forceinline__ void storePop(S* p, typename StorageTraits<S>::Real v
 template <typename S>
 constexpr bool isReduced() {
     return std::is_same<S, __half>::value || std::is_same<S, __hip_bfloat16>::value;
+}
+
+/* Streaming (nontemporal) variants for the fused pass: every population is touched once per step,
+   so its lines should not displace others in the caches. */
+template <typename S>
+__device__ __forceinline__ typename StorageTraits<S>::Real loadPopStream(const S* p) {
+    if constexpr (std::is_same<S, __half>::value) {
+        uint16_t bits = __builtin_nontemporal_load(reinterpret_cast<const uint16_t*>(p));
+        return (float) __builtin_bit_cast(_Float16, bits);
+    }
+    else if constexpr (std::is_same<S, __hip_bfloat16>::value) {
+        uint16_t bits = __builtin_nontemporal_load(reinterpret_cast<const uint16_t*>(p));
+        return __uint_as_float(((uint32_t) bits) << 16);
+    }
+    else
+        return __builtin_nontemporal_load(p);
+}
+
+template <typename S>
+__device__ __forceinline__ void storePopStream(S* p, typename StorageTraits<S>::Real v, uint32_t rnd) {
+    if constexpr (std::is_same<S, __half>::value) {
+        typedef _Float16 half2v __attribute__((ext_vector_type(2)));
+        half2v h = {0, 0};
+        h = __builtin_amdgcn_cvt_sr_f16_f32(h, v, rnd, false);
+        __builtin_nontemporal_store(__builtin_bit_cast(uint16_t, h[0]), reinterpret_cast<uint16_t*>(p));
+    }
+    else if constexpr (std::is_same<S, __hip_bfloat16>::value) {
+        typedef __bf16 bf2v __attribute__((ext_vector_type(2)));
+        bf2v h = {0, 0};
+        h = __builtin_amdgcn_cvt_sr_bf16_f32(h, v, rnd, false);
+        __builtin_nontemporal_store(__builtin_bit_cast(uint16_t, h[0]), reinterpret_cast<uint16_t*>(p));
+    }
+    else {
+        (void) rnd;
+        __builtin_nontemporal_store(v, p);
+    }
 }
 
 /* ------------------------------------------------------------------------------------------ */
@@ -403,6 +480,15 @@ __device__ __forceinline__ void pullNode(const S* __restrict__ src, uint32_t n, 
     staticFor<NQ>([&](auto I) {
         constexpr int q = decltype(I)::value;
         f[q] = loadPop(src + (size_t) q*n + (uint32_t) ((int) node + pullOffset<q>(nb)));
+    });
+}
+
+template <typename S>
+__device__ __forceinline__ void pullNodeStream(const S* __restrict__ src, uint32_t n, uint32_t node, const Neighbours& nb,
+        typename StorageTraits<S>::Real f[NQ]) {
+    staticFor<NQ>([&](auto I) {
+        constexpr int q = decltype(I)::value;
+        f[q] = loadPopStream(src + (size_t) q*n + (uint32_t) ((int) node + pullOffset<q>(nb)));
     });
 }
 

@@ -23,51 +23,81 @@ namespace lbm {
 /* ------------------------------------------------------------------------------------------ */
 /* The fused pull-stream + regularized collision + Guo forcing (+ Philox noise) pass.          */
 /* ------------------------------------------------------------------------------------------ */
-template <typename S, bool SOLIDS, bool REACTION, bool FLUCT>
-__global__ void __launch_bounds__(BLOCK) fusedStep(const S* __restrict__ src, S* __restrict__ dst,
-        const uint32_t* __restrict__ mask, const typename StorageTraits<S>::Real* __restrict__ reaction,
-        const double* __restrict__ ucm,
-        StepArgs a) {
+template <typename S, bool SOLIDS, bool REACTION, bool FLUCT, int THREADS>
+__global__ void __launch_bounds__(THREADS) fusedStep(const S* __restrict__ src, S* __restrict__ dst,
+        const uint32_t* __restrict__ mask, unsigned long long* __restrict__ reactionFixed, uint32_t* __restrict__ touched,
+        const double* __restrict__ ucm, StepArgs a) {
     using R = typename StorageTraits<S>::Real;
-    uint32_t node = xcdBlock(blockIdx.x, gridDim.x)*BLOCK + threadIdx.x;
+    using C = typename StorageTraits<S>::Compute;
+    uint32_t node = blockIdx.x*THREADS + threadIdx.x;
     if (node >= a.g.n)
         return;
+    uint32_t touchedWord = 0;
+    if constexpr (REACTION) {
+        // The word of the touched mask covers 32 nodes of this wave: every lane reads it, then its first lane
+        // (always inside the lattice, since words are 32-aligned) clears it for the next step.
+        touchedWord = touched[node >> 5];
+        if ((node & 31) == 0 && touchedWord != 0)
+            touched[node >> 5] = 0;
+    }
     if constexpr (SOLIDS)
         if (isSolid(mask, node))
             return;
     uint32_t i, j, k;
     nodeCoords(a.g, node, i, j, k);
     Neighbours nb = neighbours(a.g, i, j, k);
-    R f[NQ];
-    pullNode(src, a.g.n, node, nb, f);
-    Moments<R> m = nodeMoments(f);
-    R pi[6];
+    R fr[NQ];
+    pullNodeStream(src, a.g.n, node, nb, fr);
+    C f[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; q++)
+        f[q] = (C) fr[q];
+    Moments<C> m = nodeMoments(f);
+    C pi[6];
     piNeq(m, pi);
-    R rho = R(1) + m.dr;
-    R jx = m.jx, jy = m.jy, jz = m.jz;
+    C rho = C(1) + m.dr;
+    C jx = m.jx, jy = m.jy, jz = m.jz;
     if (a.removal) {
-        jx -= rho*R(ucm[0]);
-        jy -= rho*R(ucm[1]);
-        jz -= rho*R(ucm[2]);
+        jx -= rho*C(ucm[0]);
+        jy -= rho*C(ucm[1]);
+        jz -= rho*C(ucm[2]);
     }
-    R fx = rho*R(a.gx), fy = rho*R(a.gy), fz = rho*R(a.gz);
+    C fx = rho*C(a.gx), fy = rho*C(a.gy), fz = rho*C(a.gz);
     if constexpr (REACTION) {
-        fx += R(reaction[node]);
-        fy += R(reaction[a.g.n + node]);
-        fz += R(reaction[2*(size_t) a.g.n + node]);
+        if ((touchedWord >> (node & 31)) & 1u) {
+            // Reaction of the coupled particles on this node (fixed point), consumed and cleared.
+            C r[3];
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                unsigned long long* p = reactionFixed + (size_t) c*a.g.n + node;
+                r[c] = C((double) (long long) *p*(1.0/FIXED_SCALE));
+                *p = 0ull;
+            }
+            fx += r[0];
+            fy += r[1];
+            fz += r[2];
+        }
     }
-    R stress[6] = {R(0), R(0), R(0), R(0), R(0), R(0)};
-    R ghost[NQ];
-    R omega = R(a.omega);
+    C stress[6] = {C(0), C(0), C(0), C(0), C(0), C(0)};
+    C ghost[NQ];
+    C omega = C(a.omega);
     if constexpr (FLUCT) {
-        R r[16];
-        drawNormals<R, 16>(a.seed, TAG_FLUID, node, a.step, r);
-        R mu = R(a.mu);
-        stressNoise(r, R(sqrt(mu*rho*omega*(R(2) - omega))), stress);
-        ghostNoise(r, R(sqrt(mu*rho)), ghost);
+        // The random part in FP32 (FP64 with FP64 storage): its precision is that of a random number.
+        using N = typename std::conditional<std::is_same<S, double>::value, double, float>::type;
+        N r[16], s[6], g[NQ];
+        drawNormals<N, 16>(a.seed, TAG_FLUID, node, a.step, r);
+        N mu = N(a.mu), rhoN = N(rho), om = N(a.omega);
+        stressNoise(r, N(sqrt(mu*rhoN*om*(N(2) - om))), s);
+        ghostNoise(r, N(sqrt(mu*rhoN)), g);
+#pragma unroll
+        for (int c = 0; c < 6; c++)
+            stress[c] = C(s[c]);
+#pragma unroll
+        for (int q = 0; q < NQ; q++)
+            ghost[q] = C(g[q]);
     }
-    R out[NQ];
-    collide<R, FLUCT>(m.dr, jx, jy, jz, pi, fx, fy, fz, omega, stress, ghost, out);
+    C out[NQ];
+    collide<C, FLUCT>(m.dr, jx, jy, jz, pi, fx, fy, fz, omega, stress, ghost, out);
     uint32_t rnd = 0;
     if constexpr (isReduced<S>())
         rnd = mixBits(node ^ a.roundSeed);
@@ -78,7 +108,7 @@ __global__ void __launch_bounds__(BLOCK) fusedStep(const S* __restrict__ src, S*
             r = (rnd ^ (uint32_t) (q*0x9E3779B9u))*0x85EBCA6Bu;
             r ^= r >> 13;
         }
-        storePop(dst + (size_t) q*a.g.n + node, out[q], r);
+        storePopStream(dst + (size_t) q*a.g.n + node, (R) out[q], r);
     }
 }
 
@@ -536,26 +566,26 @@ static inline unsigned gridFor(uint64_t n) {
     }
 
 template <typename S, bool SOL, bool REA>
-static void launchFusedT(bool fluct, const void* src, void* dst, const uint32_t* mask, const void* reactionVoid,
-        const double* ucm, const StepArgs& a, hipStream_t stream) {
-    unsigned grid = gridFor(a.g.n);
-    const typename StorageTraits<S>::Real* reaction = (const typename StorageTraits<S>::Real*) reactionVoid;
+static void launchFusedT(bool fluct, const void* src, void* dst, const uint32_t* mask, unsigned long long* reactionFixed,
+        uint32_t* touched, const double* ucm, const StepArgs& a, hipStream_t stream) {
+    constexpr int T = FUSED_THREADS;
+    unsigned grid = (unsigned) ((a.g.n + T - 1)/T);
     if (fluct)
-        fusedStep<S, SOL, REA, true><<<grid, BLOCK, 0, stream>>>((const S*) src, (S*) dst, mask, reaction, ucm, a);
+        fusedStep<S, SOL, REA, true, T><<<grid, T, 0, stream>>>((const S*) src, (S*) dst, mask, reactionFixed, touched, ucm, a);
     else
-        fusedStep<S, SOL, REA, false><<<grid, BLOCK, 0, stream>>>((const S*) src, (S*) dst, mask, reaction, ucm, a);
+        fusedStep<S, SOL, REA, false, T><<<grid, T, 0, stream>>>((const S*) src, (S*) dst, mask, reactionFixed, touched, ucm, a);
 }
 
 void launchFused(int storage, bool solids, bool reaction, bool fluct, const void* src, void* dst, const uint32_t* mask,
-        const void* reactionField, const double* ucm, const StepArgs& a, hipStream_t stream) {
+        unsigned long long* reactionFixed, uint32_t* touched, const double* ucm, const StepArgs& a, hipStream_t stream) {
     LBM_DISPATCH_STORAGE(storage, S, {
         if (solids) {
-            if (reaction) launchFusedT<S, true, true>(fluct, src, dst, mask, reactionField, ucm, a, stream);
-            else launchFusedT<S, true, false>(fluct, src, dst, mask, reactionField, ucm, a, stream);
+            if (reaction) launchFusedT<S, true, true>(fluct, src, dst, mask, reactionFixed, touched, ucm, a, stream);
+            else launchFusedT<S, true, false>(fluct, src, dst, mask, reactionFixed, touched, ucm, a, stream);
         }
         else {
-            if (reaction) launchFusedT<S, false, true>(fluct, src, dst, mask, reactionField, ucm, a, stream);
-            else launchFusedT<S, false, false>(fluct, src, dst, mask, reactionField, ucm, a, stream);
+            if (reaction) launchFusedT<S, false, true>(fluct, src, dst, mask, reactionFixed, touched, ucm, a, stream);
+            else launchFusedT<S, false, false>(fluct, src, dst, mask, reactionFixed, touched, ucm, a, stream);
         }
     })
 }

@@ -12,6 +12,7 @@
 namespace lbm {
 
 constexpr int BLOCK = 256;
+constexpr int FUSED_THREADS = 512;         // work-group size of the fused pass (measured best on MI355X)
 constexpr int REDUCE_BLOCKS = 1024;       // fixed grid of the FP64 reductions: fixed summation order
 constexpr double FIXED_SCALE = 1125899906842624.0;   // 2^50: fixed-point scale of the particle reactions
 
@@ -70,7 +71,7 @@ struct ParticleArgs {
 
 /* Host-side launchers (defined in lbm_fluid.hip / lbm_particles.hip). */
 void launchFused(int storage, bool solids, bool reaction, bool fluct, const void* src, void* dst, const uint32_t* mask,
-        const void* reactionField, const double* ucm, const StepArgs& a, hipStream_t stream);
+        unsigned long long* reactionFixed, uint32_t* touched, const double* ucm, const StepArgs& a, hipStream_t stream);
 void launchMomentReduce(int storage, bool solids, const void* src, const uint32_t* mask, const Geometry& g,
         double* partials, hipStream_t stream);
 void launchFinishReduce(const double* partials, double numFluid, double* totals, double* ucm, hipStream_t stream);
@@ -91,10 +92,7 @@ void launchReflect(const double* positions, double* velocities, const int* parti
         const uint32_t* mask, const ParticleArgs& a, double* partials, hipStream_t stream);
 void launchCouple(int storage, const void* src, const uint32_t* mask, const double* positions, const double* velocities,
         const int* particles, const double* mass, const double* ucm, const ParticleArgs& a, double* particleForce,
-        unsigned long long* reactionFixed, int* entryNodes, double* partials, hipStream_t stream);
-void launchFinalizeReaction(unsigned long long* reactionFixed, void* reaction, bool fp64, const int* entryNodes, int count,
-        uint32_t n, hipStream_t stream);
-void launchClearReaction(void* reaction, bool fp64, const int* entryNodes, int count, uint32_t n, hipStream_t stream);
+        unsigned long long* reactionFixed, uint32_t* touched, double* partials, hipStream_t stream);
 void launchAddForces(const double* particleForce, const int* particles, int count, double* forces, hipStream_t stream);
 
 __host__ __device__ int stencilWidth(int stencil);

@@ -12,8 +12,9 @@
  *    64-bit fixed-point atomics (2^-50 lattice units): integer addition is associative, so the
  *    per-node sums are bit-identical from run to run whatever the order of the atomics, which is
  *    the reference's determinism guarantee (docs/architecture.md:223-227) without its sort.
- *  - finalizeReaction / clearReaction: convert the touched nodes to the FP32 reaction field that the
- *    fused pass reads, and clear them after it (lbmCoupling.cc:1852-1877).
+ *    Each touched node is flagged in a one-bit-per-node mask; the fused pass converts the accumulators
+ *    of the flagged nodes and clears them and the mask (the reference's sumCellReactions and
+ *    clearCellReactions, lbmCoupling.cc:251-272, 1852-1877, folded into the lattice pass).
  */
 #include "lbm_internal.h"
 
@@ -165,7 +166,7 @@ template <typename S, int LANES>
 __global__ void __launch_bounds__(BLOCK) coupleParticles(const S* __restrict__ src, const uint32_t* __restrict__ mask,
         const double* __restrict__ positions, const double* __restrict__ velocities, const int* __restrict__ particles,
         const double* __restrict__ mass, const double* __restrict__ ucm, ParticleArgs a, double* __restrict__ particleForce,
-        unsigned long long* __restrict__ reactionFixed, int* __restrict__ entryNodes, double* __restrict__ partials) {
+        unsigned long long* __restrict__ reactionFixed, uint32_t* __restrict__ touched, double* __restrict__ partials) {
     constexpr int GROUPS = BLOCK/LANES;
     int lane = threadIdx.x % LANES;
     int i = blockIdx.x*GROUPS + threadIdx.x/LANES;
@@ -255,19 +256,25 @@ __global__ void __launch_bounds__(BLOCK) coupleParticles(const S* __restrict__ s
                 particleForce[3*(size_t) i + k] = f[k]*a.forceScale;
     }
     double wallAcc[3] = {0, 0, 0};
-    if (a.isStep) {
-        if (active && lane < numSlots)
-            entryNodes[(size_t) i*numSlots + lane] = inStencil ? (int) node : -1;
-        if (inStencil) {
-            for (int k = 0; k < 3; k++) {
-                double r = -weight*f[k];
+    if (a.isStep && inStencil) {
+        // The reaction -w F: on a fluid node into its fixed-point accumulators (flagged in the touched mask, which
+        // the fused pass reads and clears); on a solid node it leaves the fluid and goes to the wall
+        // (ReferenceLBMKernels.cpp:959-971).
+        bool any = false;
+        for (int k = 0; k < 3; k++) {
+            double r = -weight*f[k];
+            if (solid)
+                wallAcc[k] = r*a.momentumScale;
+            else {
                 long long fixed = llrint(r*FIXED_SCALE);
-                if (fixed != 0)
+                if (fixed != 0) {
                     atomicAdd(reactionFixed + (size_t) k*a.g.n + node, (unsigned long long) fixed);
-                if (solid)
-                    wallAcc[k] = r*a.momentumScale;
+                    any = true;
+                }
             }
         }
+        if (any)
+            atomicOr(touched + (node >> 5), 1u << (node & 31));
     }
     if (partials != nullptr) {
         __shared__ double lds[BLOCK][3];
@@ -283,35 +290,6 @@ __global__ void __launch_bounds__(BLOCK) coupleParticles(const S* __restrict__ s
                 partials[3*blockIdx.x + k] = s[k];
         }
     }
-}
-
-template <typename T>
-__global__ void __launch_bounds__(BLOCK) finalizeReaction(unsigned long long* __restrict__ reactionFixed, T* __restrict__ reaction,
-        const int* __restrict__ entryNodes, int count, uint32_t n) {
-    int e = blockIdx.x*BLOCK + threadIdx.x;
-    if (e >= count)
-        return;
-    int node = entryNodes[e];
-    if (node < 0)
-        return;
-    for (int k = 0; k < 3; k++) {
-        unsigned long long old = atomicExch(reactionFixed + (size_t) k*n + node, 0ull);
-        if (old != 0)
-            reaction[(size_t) k*n + node] = (T) ((double) (long long) old*(1.0/FIXED_SCALE));
-    }
-}
-
-template <typename T>
-__global__ void __launch_bounds__(BLOCK) clearReaction(T* __restrict__ reaction, const int* __restrict__ entryNodes,
-        int count, uint32_t n) {
-    int e = blockIdx.x*BLOCK + threadIdx.x;
-    if (e >= count)
-        return;
-    int node = entryNodes[e];
-    if (node < 0)
-        return;
-    for (int k = 0; k < 3; k++)
-        reaction[(size_t) k*n + node] = T(0);
 }
 
 __global__ void __launch_bounds__(BLOCK) addForces(const double* __restrict__ particleForce, const int* __restrict__ particles,
@@ -353,71 +331,52 @@ int coupleBlocks(int stencil, int numParticles) {
 template <typename S>
 static void launchCoupleT(const void* src, const uint32_t* mask, const double* positions, const double* velocities,
         const int* particles, const double* mass, const double* ucm, const ParticleArgs& a, double* particleForce,
-        unsigned long long* reactionFixed, int* entryNodes, double* partials, hipStream_t stream) {
+        unsigned long long* reactionFixed, uint32_t* touched, double* partials, hipStream_t stream) {
     unsigned grid = (unsigned) coupleBlocks(a.stencil, a.numParticles);
     const S* s = (const S*) src;
     switch (coupleLanes(a.stencil)) {
     case 8:
         coupleParticles<S, 8><<<grid, BLOCK, 0, stream>>>(s, mask, positions, velocities, particles, mass, ucm, a,
-                particleForce, reactionFixed, entryNodes, partials);
+                particleForce, reactionFixed, touched, partials);
         break;
     case 32:
         coupleParticles<S, 32><<<grid, BLOCK, 0, stream>>>(s, mask, positions, velocities, particles, mass, ucm, a,
-                particleForce, reactionFixed, entryNodes, partials);
+                particleForce, reactionFixed, touched, partials);
         break;
     case 64:
         coupleParticles<S, 64><<<grid, BLOCK, 0, stream>>>(s, mask, positions, velocities, particles, mass, ucm, a,
-                particleForce, reactionFixed, entryNodes, partials);
+                particleForce, reactionFixed, touched, partials);
         break;
     default:
         coupleParticles<S, 1><<<grid, BLOCK, 0, stream>>>(s, mask, positions, velocities, particles, mass, ucm, a,
-                particleForce, reactionFixed, entryNodes, partials);
+                particleForce, reactionFixed, touched, partials);
         break;
     }
 }
 
 void launchCouple(int storage, const void* src, const uint32_t* mask, const double* positions, const double* velocities,
         const int* particles, const double* mass, const double* ucm, const ParticleArgs& a, double* particleForce,
-        unsigned long long* reactionFixed, int* entryNodes, double* partials, hipStream_t stream) {
+        unsigned long long* reactionFixed, uint32_t* touched, double* partials, hipStream_t stream) {
     if (a.numParticles == 0)
         return;
     switch (storage) {
     case LBM_STORAGE_FP16:
         launchCoupleT<__half>(src, mask, positions, velocities, particles, mass, ucm, a, particleForce, reactionFixed,
-                entryNodes, partials, stream);
+                touched, partials, stream);
         break;
     case LBM_STORAGE_BF16:
         launchCoupleT<__hip_bfloat16>(src, mask, positions, velocities, particles, mass, ucm, a, particleForce,
-                reactionFixed, entryNodes, partials, stream);
+                reactionFixed, touched, partials, stream);
         break;
     case LBM_STORAGE_FP64:
         launchCoupleT<double>(src, mask, positions, velocities, particles, mass, ucm, a, particleForce, reactionFixed,
-                entryNodes, partials, stream);
+                touched, partials, stream);
         break;
     default:
         launchCoupleT<float>(src, mask, positions, velocities, particles, mass, ucm, a, particleForce, reactionFixed,
-                entryNodes, partials, stream);
+                touched, partials, stream);
         break;
     }
-}
-
-void launchFinalizeReaction(unsigned long long* reactionFixed, void* reaction, bool fp64, const int* entryNodes, int count,
-        uint32_t n, hipStream_t stream) {
-    if (count == 0)
-        return;
-    if (fp64)
-        finalizeReaction<double><<<gridFor(count), BLOCK, 0, stream>>>(reactionFixed, (double*) reaction, entryNodes, count, n);
-    else
-        finalizeReaction<float><<<gridFor(count), BLOCK, 0, stream>>>(reactionFixed, (float*) reaction, entryNodes, count, n);
-}
-
-void launchClearReaction(void* reaction, bool fp64, const int* entryNodes, int count, uint32_t n, hipStream_t stream) {
-    if (count == 0)
-        return;
-    if (fp64)
-        clearReaction<double><<<gridFor(count), BLOCK, 0, stream>>>((double*) reaction, entryNodes, count, n);
-    else
-        clearReaction<float><<<gridFor(count), BLOCK, 0, stream>>>((float*) reaction, entryNodes, count, n);
 }
 
 void launchAddForces(const double* particleForce, const int* particles, int count, double* forces, hipStream_t stream) {

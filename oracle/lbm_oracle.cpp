@@ -250,30 +250,27 @@ void philox4x32(const uint32_t ctrIn[4], const uint32_t keyIn[2], uint32_t out[4
     out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
 }
 
-// Four normal numbers for (seed, tag, index, step, sub): u1 = (a + 1) 2^-32 in (0, 1],
-// u2 = b 2^-32 in [0, 1), r = sqrt(-2 ln u1), (r cos 2 pi u2, r sin 2 pi u2).
-void normals4(uint32_t seed, uint32_t tag, uint32_t index, int64_t step, uint32_t sub, double out[4]) {
-    uint32_t ctr[4] = {index, (uint32_t) (uint64_t) step, (uint32_t) ((uint64_t) step >> 32), sub};
-    uint32_t key[2] = {seed, tag};
-    uint32_t bits[4];
-    philox4x32(ctr, key, bits);
-    const double twoPi = 6.283185307179586476925287;
-    const double scale = 2.3283064365386963e-10;   // 2^-32
-    for (int p = 0; p < 2; p++) {
-        double u1 = ((double) bits[2*p] + 1.0)*scale;
-        double u2 = (double) bits[2*p+1]*scale;
-        double r = std::sqrt(-2.0*std::log(u1));
-        out[2*p] = r*std::cos(twoPi*u2);
-        out[2*p+1] = r*std::sin(twoPi*u2);
-    }
-}
-
+// Normal numbers for (seed, tag, index, step), in blocks of 16: three Philox blocks with counters
+// (index, step_lo, step_hi, 3*block + c) give 12 words; normals 2p and 2p+1 of the block are the Box-Muller
+// pair of the radius word w[p] and the 16-bit angle word (w[8 + p/2] >> 16*(p%2)) & 0xFFFF:
+// u1 = (a + 1) 2^-32 in (0, 1], u2 = b 2^-16 in [0, 1), r = sqrt(-2 ln u1), (r cos 2 pi u2, r sin 2 pi u2).
 void drawNormals(uint32_t seed, uint32_t tag, uint32_t index, int64_t step, int count, double* out) {
-    double four[4];
-    for (int sub = 0; 4*sub < count; sub++) {
-        normals4(seed, tag, index, step, (uint32_t) sub, four);
-        for (int m = 0; m < 4 && 4*sub + m < count; m++)
-            out[4*sub + m] = four[m];
+    const double twoPi = 6.283185307179586476925287;
+    for (int block = 0; 16*block < count; block++) {
+        uint32_t w[12];
+        for (int c = 0; c < 3; c++) {
+            uint32_t ctr[4] = {index, (uint32_t) (uint64_t) step, (uint32_t) ((uint64_t) step >> 32), (uint32_t) (3*block + c)};
+            uint32_t key[2] = {seed, tag};
+            philox4x32(ctr, key, w + 4*c);
+        }
+        for (int p = 0; p < 8 && 16*block + 2*p < count; p++) {
+            uint32_t angle = (w[8 + p/2] >> (16*(p & 1))) & 0xFFFFu;
+            double r = std::sqrt(-2.0*std::log(((double) w[p] + 1.0)*2.3283064365386963e-10));
+            double a = twoPi*((double) angle*1.52587890625e-05);
+            out[16*block + 2*p] = r*std::cos(a);
+            if (16*block + 2*p + 1 < count)
+                out[16*block + 2*p + 1] = r*std::sin(a);
+        }
     }
 }
 
