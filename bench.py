@@ -73,6 +73,22 @@ def cpu_baseline(workload_name, budget_steps=1):
                       f"host {platform.processor() or platform.machine()}, nproc {os.cpu_count()}"}
 
 
+def copy_calibration(mib=1024, repeats=5):
+    """A plain device-to-device copy of known size (hipMemcpy's copy kernel), to scale the FETCH_SIZE and WRITE_SIZE
+    counters of rocprofv3: the factor is the known bytes over what the counters read for it."""
+    hip = ctypes.CDLL("libamdhip64.so")
+    nbytes = mib*1024*1024
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(nbytes)) == 0
+    for _ in range(repeats):
+        assert hip.hipMemcpy(b, a, ctypes.c_size_t(nbytes), 3) == 0   # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print(json.dumps({"copy_calibration_bytes": nbytes, "repeats": repeats}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -81,7 +97,12 @@ def main():
     ap.add_argument("--workload", default="config2")
     ap.add_argument("--storage", default="fp32")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a 1 GiB device copy (for the counter scale of rocprofv3 --pmc)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration()
+        return 0
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU (the whole lattice in one domain); --gpus {args.gpus} is not "
               f"supported", file=sys.stderr)
@@ -115,6 +136,12 @@ def main():
     touched_fraction = min(1.0, len(w.masses)*slots/n) if len(w.masses) else 0.0
     bpn = bytes_per_node(args.storage, True, touched_fraction)
     achieved = bpn*n/(fused_ms*1e-3)/1e9
+    # HBM bytes per launch of the fused pass, measured by rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE (scaled by a
+    # copy of known size) for this workload and storage, where such a measurement is committed.
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", f"fusedStep_{args.workload}_{args.storage}.traffic.json")
+    if os.path.exists(tfile):
+        traffic = round(json.load(open(tfile))["traffic_bytes_per_launch"])
     result = {
         "metric": "MLUPS & coupled steps/s (HBM GB/s %) 256^3+20k beads; FP16/MLP vs plain @err",
         "value": round(mlups, 2),
@@ -131,8 +158,9 @@ def main():
         "data": "synthetic",
         "config": dict(w.description, storage=args.storage),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved/HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "fusedStep", "kernel_ms": round(fused_ms, 4), "bytes_per_node": bpn},
+                     "frac": round(achieved/HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "kernel": "fusedStep", "kernel_ms": round(fused_ms, 4), "bytes_per_node": bpn,
+                     "algorithmic_bytes_per_launch": round(bpn*n)},
     }
     if not args.no_cpu_baseline:
         print("bench: timing the CPU oracle baseline", file=sys.stderr, flush=True)

@@ -81,9 +81,9 @@ __device__ __forceinline__ R addTerm(R acc, R x) {
  * steps.  The weight is split into w_hi = float(w) and w_lo = float(w - w_hi), and w*v is formed
  * as fma(v, w_hi, v*w_lo), exact to about 1e-15 relative before the final rounding.
  */
-template <typename R>
+template <typename R, bool SPLIT>
 __device__ __forceinline__ R weighted(int q, R v) {
-    if constexpr (std::is_same<R, float>::value) {
+    if constexpr (std::is_same<R, float>::value && SPLIT) {
         constexpr float hi0 = (float) (1.0/3.0), hi1 = (float) (1.0/18.0), hi2 = (float) (1.0/36.0);
         constexpr float lo0 = (float) (1.0/3.0 - (double) hi0), lo1 = (float) (1.0/18.0 - (double) hi1),
                 lo2 = (float) (1.0/36.0 - (double) hi2);
@@ -142,7 +142,7 @@ __device__ __forceinline__ void piNeq(const Moments<R>& m, R pi[6]) {
  * modes of the fluctuating part, already divided by 4.5, or zeros), ghost[q] the ghost part of
  * the fluctuating part divided by w_q (or nullptr).
  */
-template <typename R, bool GHOST>
+template <typename R, bool GHOST, bool SPLIT = true>
 __device__ __forceinline__ void collide(R dr, R jx, R jy, R jz, const R pi[6], R fx, R fy, R fz, R omega,
         const R stress[6], const R* ghost, R out[NQ]) {
     R rho = R(1) + dr;
@@ -179,7 +179,7 @@ __device__ __forceinline__ void collide(R dr, R jx, R jy, R jz, const R pi[6], R
         R v = e[q];
         if constexpr (GHOST)
             v += ghost[q];
-        out[q] = weighted<R>(q, v);
+        out[q] = weighted<R, SPLIT>(q, v);
     }
 }
 
@@ -277,11 +277,13 @@ __device__ __forceinline__ void boxMullerPair(uint32_t a, uint32_t b, R* out) {
         out[1] = r*sin(angle);
     }
     else {
-        const float twoPi = 6.28318530717958647f;
-        float r = sqrtf(-2.0f*__logf(fmaf((float) a, 2.3283064365386963e-10f, 2.3283064365386963e-10f)));
-        float angle = twoPi*((float) b*1.52587890625e-05f);
-        out[0] = r*__cosf(angle);
-        out[1] = r*__sinf(angle);
+        // Hardware transcendentals: v_log_f32 is log2 (so -2 ln u1 = -2 ln 2 log2 u1), v_sin_f32/v_cos_f32
+        // take the angle in revolutions (sin 2 pi u2), v_sqrt_f32 is within 1 ulp.
+        float u1 = fmaf((float) a, 2.3283064365386963e-10f, 2.3283064365386963e-10f);
+        float r = __builtin_amdgcn_sqrtf(-1.3862943611198906f*__builtin_amdgcn_logf(u1));
+        float turns = (float) b*1.52587890625e-05f;
+        out[0] = r*__builtin_amdgcn_cosf(turns);
+        out[1] = r*__builtin_amdgcn_sinf(turns);
     }
 }
 

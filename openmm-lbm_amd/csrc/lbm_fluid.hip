@@ -87,8 +87,17 @@ __global__ void __launch_bounds__(THREADS) fusedStep(const S* __restrict__ src, 
         N r[16], s[6], g[NQ];
         drawNormals<N, 16>(a.seed, TAG_FLUID, node, a.step, r);
         N mu = N(a.mu), rhoN = N(rho), om = N(a.omega);
-        stressNoise(r, N(sqrt(mu*rhoN*om*(N(2) - om))), s);
-        ghostNoise(r, N(sqrt(mu*rhoN)), g);
+        N stressAmp, ghostAmp;
+        if constexpr (std::is_same<N, float>::value) {
+            stressAmp = __builtin_amdgcn_sqrtf(mu*rhoN*om*(2.0f - om));
+            ghostAmp = __builtin_amdgcn_sqrtf(mu*rhoN);
+        }
+        else {
+            stressAmp = sqrt(mu*rhoN*om*(N(2) - om));
+            ghostAmp = sqrt(mu*rhoN);
+        }
+        stressNoise(r, stressAmp, s);
+        ghostNoise(r, ghostAmp, g);
 #pragma unroll
         for (int c = 0; c < 6; c++)
             stress[c] = C(s[c]);
@@ -97,7 +106,8 @@ __global__ void __launch_bounds__(THREADS) fusedStep(const S* __restrict__ src, 
             ghost[q] = C(g[q]);
     }
     C out[NQ];
-    collide<C, FLUCT>(m.dr, jx, jy, jz, pi, fx, fy, fz, omega, stress, ghost, out);
+    // The exact weights matter where the storage keeps FP32 or FP64 precision; FP16/BF16 rounding is far coarser.
+    collide<C, FLUCT, !isReduced<S>()>(m.dr, jx, jy, jz, pi, fx, fy, fz, omega, stress, ghost, out);
     uint32_t rnd = 0;
     if constexpr (isReduced<S>())
         rnd = mixBits(node ^ a.roundSeed);
