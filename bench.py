@@ -1,6 +1,6 @@
 """Benchmark of the gfx950 LBMForce platform on its headline workload.
 
-python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload config2|config2-fluid|periodic]
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload config2|config2-fluid|periodic|config3|config4]
                 [--storage fp32|fp16|bf16] [--no-cpu-baseline]
 
 A step is one coupled integration step: beginStep (reflection), the extra forces, the kernel's lattice step
@@ -89,6 +89,133 @@ def copy_calibration(mib=1024, repeats=5):
     print(json.dumps({"copy_calibration_bytes": nbytes, "repeats": repeats}), flush=True)
 
 
+def precision_study(steps, n=256, num_beads=20000):
+    """The precision form: config2 with fluctuations off, run with FP64, FP32, FP16 and BF16 storage of the
+    populations.  Reports, against the FP64 run (which agrees with the FP64 CPU oracle to 1e-12), the error of the mean
+    velocity profile u_x(z) (plane averages, max over z relative to the max of the profile), of the bead forces (max
+    and RMS relative to max |F|), the FP64 drift of total mass and momentum, and the speed of each storage."""
+    import numpy as np
+    from lbm_amd import LBMForce, _lib, workloads
+    results = {}
+    profiles = {}
+    forces = {}
+    for name, st in (("fp64", LBMForce.FP64), ("fp32", LBMForce.FP32), ("fp16", LBMForce.FP16), ("bf16", LBMForce.BF16)):
+        w = workloads.config2(n=n, num_beads=num_beads, fluctuations=False)
+        w.configure.append(lambda f: f.setTemperature(0.0))
+        ctx = w.build(storage=st)
+        dp = w.device_particles(ctx)
+        m0, p0 = ctx.getFluidTotals()
+        _lib.check(ctx.lib, ctx.lib.lbm_profile_fused(ctx.handle, 1, None, None))
+        t0 = time.perf_counter()
+        dp.step(steps)
+        _lib.check(ctx.lib, ctx.lib.lbm_synchronize(ctx.handle))
+        elapsed = time.perf_counter() - t0
+        total = ctypes.c_double()
+        count = ctypes.c_int()
+        _lib.check(ctx.lib, ctx.lib.lbm_profile_fused(ctx.handle, 0, ctypes.byref(total), ctypes.byref(count)))
+        m1, p1 = ctx.getFluidTotals()
+        dp.pull()
+        rho, u = ctx.getFluidFields()
+        ux = u[:, 0].reshape(n, n*n).mean(axis=1)
+        profiles[name] = ux
+        forces[name] = ctx.getForces()[:num_beads]
+        results[name] = {"ms_per_step": 1e3*elapsed/steps, "fused_ms": total.value/max(1, count.value),
+                         "mass_drift": (m1 - m0)/m0, "momentum_x": p1[0]}
+        print(f"precision: {name} done ({results[name]['ms_per_step']:.3f} ms/step)", file=sys.stderr, flush=True)
+        del dp, ctx
+    ref_u, ref_f = profiles["fp64"], forces["fp64"]
+    for name in results:
+        results[name]["profile_rel_err"] = float(np.abs(profiles[name] - ref_u).max()/np.abs(ref_u).max())
+        df = np.linalg.norm(forces[name] - ref_f, axis=1)
+        fmax = np.linalg.norm(ref_f, axis=1).max()
+        results[name]["force_max_rel_err"] = float(df.max()/fmax)
+        results[name]["force_rms_rel_err"] = float(np.sqrt((df**2).mean())/np.sqrt((np.linalg.norm(ref_f, axis=1)**2).mean()))
+        results[name]["speedup_vs_fp32"] = results["fp32"]["ms_per_step"]/results[name]["ms_per_step"]
+        results[name]["fused_speedup_vs_fp32"] = results["fp32"]["fused_ms"]/results[name]["fused_ms"]
+    return {"metric": "precision form: storage speed vs FP32 and error vs FP64 (config2, fluctuations off)",
+            "value": round(results["fp16"]["speedup_vs_fp32"], 3), "unit": "x FP32 coupled steps/s (FP16 storage)",
+            "steps": steps, "higher_is_better": True, "dtype": "f16-storage/f32",
+            "config": {"workload": f"config3 = config2 {n}^3, {num_beads} beads Keys, fluctuations off, T=0",
+                       "reference": "FP64 storage run on the GPU (FP64 oracle parity 1e-12)"},
+            "storages": {k: {kk: (round(vv, 8) if isinstance(vv, float) else vv) for kk, vv in v.items()}
+                         for k, v in results.items()}}
+
+
+def batch_study(steps, warmup, sequential_steps=20):
+    """config4: 64 uneven periodic replicas stepped together (hipGraph replay, work-group size picked by the
+    autotuner).  value: aggregate MLUPS over all replicas.  Also times the same replicas run one after another
+    through the single-lattice path (sequential_steps steps each), the way the reference runs them."""
+    from lbm_amd import workloads
+    from lbm_amd.batch import Batch
+    from lbm_amd.harness import DeviceParticles
+    from lbm_amd.context import Context, System, VerletIntegrator
+    replicas, rows, description = workloads.config4(steps=steps)
+    print(f"bench: building config4 ({len(replicas)} replicas, {description['total_nodes']} nodes, "
+          f"{description['total_beads']} beads)", file=sys.stderr, flush=True)
+    batch = Batch(replicas)
+    tune, best = batch.autotune(20)
+    print(f"bench: autotune ms/step {tune} -> {best} threads", file=sys.stderr, flush=True)
+    batch.step(warmup)
+    batch.synchronize()
+    t0 = time.perf_counter()
+    batch.step(steps)
+    batch.synchronize()
+    elapsed = time.perf_counter() - t0
+    batch.profileFused(True)
+    batch.step(20)
+    fused_total, fused_count = batch.profileFused(False)
+    fused_ms = fused_total/max(fused_count, 1)
+    nodes = description["total_nodes"]
+    beads = description["total_beads"]
+    steps_per_s = steps/elapsed
+    mlups = nodes*steps_per_s/1e6
+    # Touched nodes of the trilinear stencil: at most 8 per bead.
+    bpn = bytes_per_node("fp32", False, min(1.0, 8*beads/nodes))
+    achieved = bpn*nodes/(fused_ms*1e-3)/1e9
+
+    # The same replicas one after another through the single-lattice path (one GPU, one replica at a time).
+    seq_ms = 0.0
+    for rep in replicas:
+        system = System(rep.box)
+        for m in rep.masses:
+            system.addParticle(m)
+        ctx = Context(system, VerletIntegrator(rep.stepSize), rep.force)
+        ctx.setPositions(rep.positions)
+        ctx.setVelocities(rep.velocities)
+        dp = DeviceParticles(ctx)
+        dp.step(5)
+        ctx.lib.lbm_synchronize(ctx.handle)
+        t = time.perf_counter()
+        dp.step(sequential_steps)
+        ctx.lib.lbm_synchronize(ctx.handle)
+        seq_ms += 1e3*(time.perf_counter() - t)/sequential_steps
+        del dp, ctx
+    return {
+        "metric": "MLUPS & coupled steps/s (HBM GB/s %) 256^3+20k beads; FP16/MLP vs plain @err",
+        "value": round(mlups, 2),
+        "unit": "MLUPS",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": warmup,
+        "ms_per_step": round(1e3*elapsed/steps, 4),
+        "coupled_steps_per_s": round(steps_per_s, 2),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": dict(description, autotune_ms_per_step={str(k): round(v, 4) for k, v in tune.items()},
+                       threads=best),
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved/HBM_PEAK_GBS, 4), "traffic": None, "kernel": "fusedBatch",
+                     "kernel_ms": round(fused_ms, 4), "bytes_per_node": bpn,
+                     "algorithmic_bytes_per_launch": round(bpn*nodes)},
+        "sequential_single_lattice": {"ms_per_step_all_replicas": round(seq_ms, 4),
+                                      "speedup_of_batch": round(seq_ms/(1e3*elapsed/steps), 3),
+                                      "steps_per_replica_timed": sequential_steps},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -107,6 +234,12 @@ def main():
         print(f"bench.py: this build runs on one GPU (the whole lattice in one domain); --gpus {args.gpus} is not "
               f"supported", file=sys.stderr)
         return 2
+    if args.workload == "config3":
+        print(json.dumps(precision_study(args.steps)), flush=True)
+        return 0
+    if args.workload == "config4":
+        print(json.dumps(batch_study(args.steps, args.warmup)), flush=True)
+        return 0
 
     from lbm_amd import _lib
     w = make_workload(args.workload)

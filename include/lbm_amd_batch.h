@@ -1,0 +1,58 @@
+/*
+ * lbm_amd_batch.h -- many independent replicas stepped together on one GPU.
+ *
+ * No counterpart in the reference's kernel interface: the reference runs the replicas of a
+ * seed-averaged study as separate runs (examples/cocomo/kick.py:30,105 --repeats).  Here R periodic
+ * lattices of different sizes, relaxation times, temperatures, seeds and particle counts advance in
+ * one fused launch per step over a table of work-groups, with the particles coupled and integrated
+ * (leapfrog Verlet, coupling force only) on the device; a step is captured once in a hipGraph and
+ * replayed.  FP32 storage; periodic faces, no solid nodes, explicit drag.
+ */
+#ifndef LBM_AMD_BATCH_H_
+#define LBM_AMD_BATCH_H_
+
+#include "lbm_amd.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct lbm_batch lbm_batch;
+
+/*
+ * params[r]: the lattice parameters of replica r (lbm_compute_lattice_parameters), all with the same
+ * interpolation stencil and the same fluid_fluctuations setting; num_particles[r]: its coupled particles;
+ * masses (Da), positions (nm), velocities (nm/ps): all particles, replica after replica (xyz per particle).
+ */
+int lbm_batch_create(int num_replicas, const lbm_params* params, const int* num_particles, const double* masses,
+        const double* positions, const double* velocities, lbm_batch** out);
+
+/* Time each work-group size of the fused pass over a few steps (the state advances) and keep the fastest.
+   times_ms (3 entries, may be NULL) receives the time per step of 256, 512 and 1024 threads; *best the choice. */
+int lbm_batch_autotune(lbm_batch* batch, int steps_per_trial, double* times_ms, int* best);
+
+/* Advance every replica by num_steps steps (asynchronous; replays the captured graph). */
+int lbm_batch_step(lbm_batch* batch, int num_steps);
+
+int lbm_batch_synchronize(lbm_batch* batch);
+
+/* As lbm_profile_fused: while enabled, steps are launched one by one (not from the graph) with events around each
+   fused pass; disabling synchronizes and returns their total time and count. */
+int lbm_batch_profile_fused(lbm_batch* batch, int enable, double* total_ms, int* count);
+
+/* Density (Da/nm^3) and velocity (nm/ps, xyz) of replica r, as lbm_get_fluid_fields. */
+int lbm_batch_get_fluid_fields(lbm_batch* batch, int replica, double* density, double* velocity);
+
+/* The particles of all replicas (either pointer may be NULL). */
+int lbm_batch_get_particles(lbm_batch* batch, double* positions, double* velocities);
+
+/* Total nodes of all replicas, and the step count. */
+int lbm_batch_info(lbm_batch* batch, int64_t* total_nodes, int64_t* step);
+
+int lbm_batch_destroy(lbm_batch* batch);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LBM_AMD_BATCH_H_ */

@@ -16,7 +16,7 @@
  * bounceBack and computeWallExchange (lbmFluid.cc:358-403; ReferenceLBMKernels.cpp:1156-1205),
  * applyBoundaries (lbmFluid.cc:437-602; ReferenceLBMKernels.cpp:1207-1343).
  */
-#include "lbm_internal.h"
+#include "lbm_fused.h"
 
 namespace lbm {
 
@@ -27,99 +27,45 @@ template <typename S, bool SOLIDS, bool REACTION, bool FLUCT, int THREADS>
 __global__ void __launch_bounds__(THREADS) fusedStep(const S* __restrict__ src, S* __restrict__ dst,
         const uint32_t* __restrict__ mask, unsigned long long* __restrict__ reactionFixed, uint32_t* __restrict__ touched,
         const double* __restrict__ ucm, StepArgs a) {
-    using R = typename StorageTraits<S>::Real;
-    using C = typename StorageTraits<S>::Compute;
     uint32_t node = blockIdx.x*THREADS + threadIdx.x;
     if (node >= a.g.n)
         return;
-    uint32_t touchedWord = 0;
-    if constexpr (REACTION) {
-        // The word of the touched mask covers 32 nodes of this wave: every lane reads it, then its first lane
-        // (always inside the lattice, since words are 32-aligned) clears it for the next step.
-        touchedWord = touched[node >> 5];
-        if ((node & 31) == 0 && touchedWord != 0)
-            touched[node >> 5] = 0;
+    fusedNode<S, SOLIDS, REACTION, FLUCT>(src, dst, mask, reactionFixed, touched, ucm, a, node);
+}
+
+/*
+ * The same pass over every replica of a batch in one launch: work-group b covers THREADS nodes of replica
+ * tileReplica[b].  The step index comes from device memory, so one captured launch serves every step.
+ */
+template <bool FLUCT, int THREADS>
+__global__ void __launch_bounds__(THREADS) fusedBatch(const ReplicaDesc* __restrict__ table,
+        const uint16_t* __restrict__ tileReplica, const int64_t* __restrict__ stepCounter) {
+    const ReplicaDesc* d = table + tileReplica[blockIdx.x];
+    uint32_t node = (blockIdx.x - d->tileBegin)*THREADS + threadIdx.x;
+    StepArgs a = d->step;
+    if (node >= a.g.n)
+        return;
+    a.step = *stepCounter;
+    a.roundSeed = 0;
+    const float* src = (a.step & 1) ? d->popB : d->popA;
+    float* dst = (a.step & 1) ? d->popA : d->popB;
+    fusedNode<float, false, true, FLUCT>(src, dst, nullptr, d->reactionFixed, d->touched, nullptr, a, node);
+}
+
+void launchFusedBatch(int threads, bool fluct, const ReplicaDesc* table, const uint16_t* tileReplica, uint32_t numTiles,
+        const int64_t* stepCounter, hipStream_t stream) {
+#define LBM_BATCH_CASE(T) \
+    case T: \
+        if (fluct) fusedBatch<true, T><<<numTiles, T, 0, stream>>>(table, tileReplica, stepCounter); \
+        else fusedBatch<false, T><<<numTiles, T, 0, stream>>>(table, tileReplica, stepCounter); \
+        break;
+    switch (threads) {
+    LBM_BATCH_CASE(256)
+    LBM_BATCH_CASE(1024)
+    default:
+    LBM_BATCH_CASE(512)
     }
-    if constexpr (SOLIDS)
-        if (isSolid(mask, node))
-            return;
-    uint32_t i, j, k;
-    nodeCoords(a.g, node, i, j, k);
-    Neighbours nb = neighbours(a.g, i, j, k);
-    R fr[NQ];
-    pullNodeStream(src, a.g.n, node, nb, fr);
-    C f[NQ];
-#pragma unroll
-    for (int q = 0; q < NQ; q++)
-        f[q] = (C) fr[q];
-    Moments<C> m = nodeMoments(f);
-    C pi[6];
-    piNeq(m, pi);
-    C rho = C(1) + m.dr;
-    C jx = m.jx, jy = m.jy, jz = m.jz;
-    if (a.removal) {
-        jx -= rho*C(ucm[0]);
-        jy -= rho*C(ucm[1]);
-        jz -= rho*C(ucm[2]);
-    }
-    C fx = rho*C(a.gx), fy = rho*C(a.gy), fz = rho*C(a.gz);
-    if constexpr (REACTION) {
-        if ((touchedWord >> (node & 31)) & 1u) {
-            // Reaction of the coupled particles on this node (fixed point), consumed and cleared.
-            C r[3];
-#pragma unroll
-            for (int c = 0; c < 3; c++) {
-                unsigned long long* p = reactionFixed + (size_t) c*a.g.n + node;
-                r[c] = C((double) (long long) *p*(1.0/FIXED_SCALE));
-                *p = 0ull;
-            }
-            fx += r[0];
-            fy += r[1];
-            fz += r[2];
-        }
-    }
-    C stress[6] = {C(0), C(0), C(0), C(0), C(0), C(0)};
-    C ghost[NQ];
-    C omega = C(a.omega);
-    if constexpr (FLUCT) {
-        // The random part in FP32 (FP64 with FP64 storage): its precision is that of a random number.
-        using N = typename std::conditional<std::is_same<S, double>::value, double, float>::type;
-        N r[16], s[6], g[NQ];
-        drawNormals<N, 16>(a.seed, TAG_FLUID, node, a.step, r);
-        N mu = N(a.mu), rhoN = N(rho), om = N(a.omega);
-        N stressAmp, ghostAmp;
-        if constexpr (std::is_same<N, float>::value) {
-            stressAmp = __builtin_amdgcn_sqrtf(mu*rhoN*om*(2.0f - om));
-            ghostAmp = __builtin_amdgcn_sqrtf(mu*rhoN);
-        }
-        else {
-            stressAmp = sqrt(mu*rhoN*om*(N(2) - om));
-            ghostAmp = sqrt(mu*rhoN);
-        }
-        stressNoise(r, stressAmp, s);
-        ghostNoise(r, ghostAmp, g);
-#pragma unroll
-        for (int c = 0; c < 6; c++)
-            stress[c] = C(s[c]);
-#pragma unroll
-        for (int q = 0; q < NQ; q++)
-            ghost[q] = C(g[q]);
-    }
-    C out[NQ];
-    // The exact weights matter where the storage keeps FP32 or FP64 precision; FP16/BF16 rounding is far coarser.
-    collide<C, FLUCT, !isReduced<S>()>(m.dr, jx, jy, jz, pi, fx, fy, fz, omega, stress, ghost, out);
-    uint32_t rnd = 0;
-    if constexpr (isReduced<S>())
-        rnd = mixBits(node ^ a.roundSeed);
-#pragma unroll
-    for (int q = 0; q < NQ; q++) {
-        uint32_t r = 0;
-        if constexpr (isReduced<S>()) {
-            r = (rnd ^ (uint32_t) (q*0x9E3779B9u))*0x85EBCA6Bu;
-            r ^= r >> 13;
-        }
-        storePopStream(dst + (size_t) q*a.g.n + node, (R) out[q], r);
-    }
+#undef LBM_BATCH_CASE
 }
 
 /* ------------------------------------------------------------------------------------------ */

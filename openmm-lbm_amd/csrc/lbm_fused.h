@@ -1,0 +1,114 @@
+/*
+ * The per-node body of the fused pull-stream + regularized collision + Guo forcing (+ Philox noise) pass,
+ * shared by the single-lattice kernel (lbm_fluid.hip) and the batched-replica kernel (lbm_batch.hip).
+ * Reference counterparts: computeFluidMoments + collideAndStream (lbmFluid.cc:89-139, 230-310;
+ * ReferenceLBMKernels.cpp:430-469, 1066-1120).
+ */
+#pragma once
+
+#include "lbm_internal.h"
+
+namespace lbm {
+
+/*
+ * One node: pull C_q(x - c_q) from src, form the moments, subtract the removal velocity, add the body force and the
+ * particle reaction (consumed from the fixed-point accumulators where the touched bit is set), collide, add the
+ * fluctuating part, store C_q(x) to dst at x.  Returns without writing for solid nodes.
+ */
+template <typename S, bool SOLIDS, bool REACTION, bool FLUCT>
+__device__ __forceinline__ void fusedNode(const S* __restrict__ src, S* __restrict__ dst, const uint32_t* __restrict__ mask,
+        unsigned long long* __restrict__ reactionFixed, uint32_t* __restrict__ touched, const double* __restrict__ ucm,
+        const StepArgs& a, uint32_t node) {
+    using R = typename StorageTraits<S>::Real;
+    using C = typename StorageTraits<S>::Compute;
+    uint32_t touchedWord = 0;
+    if constexpr (REACTION) {
+        // The word of the touched mask covers 32 nodes of this wave: every lane reads it, then its first lane
+        // (always inside the lattice, since words are 32-aligned) clears it for the next step.
+        touchedWord = touched[node >> 5];
+        if ((node & 31) == 0 && touchedWord != 0)
+            touched[node >> 5] = 0;
+    }
+    if constexpr (SOLIDS)
+        if (isSolid(mask, node))
+            return;
+    uint32_t i, j, k;
+    nodeCoords(a.g, node, i, j, k);
+    Neighbours nb = neighbours(a.g, i, j, k);
+    R fr[NQ];
+    pullNodeStream(src, a.g.n, node, nb, fr);
+    C f[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; q++)
+        f[q] = (C) fr[q];
+    Moments<C> m = nodeMoments(f);
+    C pi[6];
+    piNeq(m, pi);
+    C rho = C(1) + m.dr;
+    C jx = m.jx, jy = m.jy, jz = m.jz;
+    if (a.removal) {
+        jx -= rho*C(ucm[0]);
+        jy -= rho*C(ucm[1]);
+        jz -= rho*C(ucm[2]);
+    }
+    C fx = rho*C(a.gx), fy = rho*C(a.gy), fz = rho*C(a.gz);
+    if constexpr (REACTION) {
+        if ((touchedWord >> (node & 31)) & 1u) {
+            // Reaction of the coupled particles on this node (fixed point), consumed and cleared.
+            C r[3];
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                unsigned long long* p = reactionFixed + (size_t) c*a.g.n + node;
+                r[c] = C((double) (long long) *p*(1.0/FIXED_SCALE));
+                *p = 0ull;
+            }
+            fx += r[0];
+            fy += r[1];
+            fz += r[2];
+        }
+    }
+    C stress[6] = {C(0), C(0), C(0), C(0), C(0), C(0)};
+    C ghost[NQ];
+    C omega = C(a.omega);
+    if constexpr (FLUCT) {
+        // The random part in FP32 (FP64 with FP64 storage): its precision is that of a random number.
+        using N = typename std::conditional<std::is_same<S, double>::value, double, float>::type;
+        N r[16], s[6], g[NQ];
+        drawNormals<N, 16>(a.seed, TAG_FLUID, node, a.step, r);
+        N mu = N(a.mu), rhoN = N(rho), om = N(a.omega);
+        N stressAmp, ghostAmp;
+        if constexpr (std::is_same<N, float>::value) {
+            stressAmp = __builtin_amdgcn_sqrtf(mu*rhoN*om*(2.0f - om));
+            ghostAmp = __builtin_amdgcn_sqrtf(mu*rhoN);
+        }
+        else {
+            stressAmp = sqrt(mu*rhoN*om*(N(2) - om));
+            ghostAmp = sqrt(mu*rhoN);
+        }
+        stressNoise(r, stressAmp, s);
+        ghostNoise(r, ghostAmp, g);
+#pragma unroll
+        for (int c = 0; c < 6; c++)
+            stress[c] = C(s[c]);
+#pragma unroll
+        for (int q = 0; q < NQ; q++)
+            ghost[q] = C(g[q]);
+    }
+    C out[NQ];
+    // The exact weights matter where the storage keeps FP32 or FP64 precision; FP16/BF16 rounding is far coarser.
+    collide<C, FLUCT, !isReduced<S>()>(m.dr, jx, jy, jz, pi, fx, fy, fz, omega, stress, ghost, out);
+    uint32_t rnd = 0;
+    if constexpr (isReduced<S>())
+        rnd = mixBits(node ^ a.roundSeed);
+#pragma unroll
+    for (int q = 0; q < NQ; q++) {
+        uint32_t r = 0;
+        if constexpr (isReduced<S>()) {
+            r = (rnd ^ (uint32_t) (q*0x9E3779B9u))*0x85EBCA6Bu;
+            r ^= r >> 13;
+        }
+        storePopStream(dst + (size_t) q*a.g.n + node, (R) out[q], r);
+    }
+}
+
+}  // namespace lbm

@@ -69,7 +69,31 @@ struct ParticleArgs {
     int hasSolids;
 };
 
+/*
+ * One replica of a batch (lbm_batch.hip): a periodic lattice with its own parameters, FP32 populations, particles
+ * and reaction accumulators.  The populations of step s are in popA for even s and popB for odd s, so that every
+ * step of a batch is the same sequence of launches (the step index is read from device memory).
+ */
+struct ReplicaDesc {
+    StepArgs step;               // geometry, omega, body force, seed, mu (step field set in the kernels)
+    ParticleArgs part;           // coupling parameters (step field set in the kernels)
+    float* popA;
+    float* popB;
+    unsigned long long* reactionFixed;
+    uint32_t* touched;
+    double dt;                   // ps
+    uint32_t tileBegin;          // first work-group of this replica in the fused grid
+    int particleBegin;           // first particle of this replica in the batch arrays
+};
+
 /* Host-side launchers (defined in lbm_fluid.hip / lbm_particles.hip). */
+void launchFusedBatch(int threads, bool fluct, const ReplicaDesc* table, const uint16_t* tileReplica, uint32_t numTiles,
+        const int64_t* stepCounter, hipStream_t stream);
+void launchCoupleBatch(int stencil, const ReplicaDesc* table, const uint16_t* particleReplica, int numParticles,
+        const double* positions, const double* velocities, const double* mass, double* forces,
+        const int64_t* stepCounter, hipStream_t stream);
+void launchVerletBatch(const ReplicaDesc* table, const uint16_t* particleReplica, int numParticles, double* positions,
+        double* velocities, const double* forces, const double* invMass, int64_t* stepCounter, hipStream_t stream);
 void launchFused(int storage, bool solids, bool reaction, bool fluct, const void* src, void* dst, const uint32_t* mask,
         unsigned long long* reactionFixed, uint32_t* touched, const double* ucm, const StepArgs& a, hipStream_t stream);
 void launchMomentReduce(int storage, bool solids, const void* src, const uint32_t* mask, const Geometry& g,

@@ -162,19 +162,20 @@ __global__ void __launch_bounds__(BLOCK) reflectParticles(const double* __restri
 }
 
 /* ------------------------------------------------------------------------------------------ */
+/*
+ * The coupling of one particle by a group of LANES lanes (one lane per stencil slot; every lane of the group calls
+ * it, inactive ones with active = false).  x, v: the particle's position (nm) and velocity (nm/ps); index: its
+ * coupled-particle index (the counter of its random numbers); m: its mass in lattice units.  The force (kJ/mol/nm) is
+ * written to forceOut by the first lane; in a lattice step the reaction goes to the accumulators and, on solid nodes,
+ * into wallAcc (lattice momentum x momentumScale).
+ */
 template <typename S, int LANES>
-__global__ void __launch_bounds__(BLOCK) coupleParticles(const S* __restrict__ src, const uint32_t* __restrict__ mask,
-        const double* __restrict__ positions, const double* __restrict__ velocities, const int* __restrict__ particles,
-        const double* __restrict__ mass, const double* __restrict__ ucm, ParticleArgs a, double* __restrict__ particleForce,
-        unsigned long long* __restrict__ reactionFixed, uint32_t* __restrict__ touched, double* __restrict__ partials) {
-    constexpr int GROUPS = BLOCK/LANES;
-    int lane = threadIdx.x % LANES;
-    int i = blockIdx.x*GROUPS + threadIdx.x/LANES;
-    bool active = i < a.numParticles;
+__device__ __forceinline__ void coupleGroup(const S* __restrict__ src, const uint32_t* __restrict__ mask,
+        const double* x, const double* vel, uint32_t index, double m, bool active, int lane,
+        const double* __restrict__ ucm, const ParticleArgs& a, double* forceOut,
+        unsigned long long* __restrict__ reactionFixed, uint32_t* __restrict__ touched, double wallAcc[3]) {
     int w = stencilWidth(a.stencil);
     int numSlots = w*w*w;
-    int part = active ? particles[i] : 0;
-    const double* x = positions + 3*(size_t) part;
     int slot = lane;
     bool inStencil = active && slot < numSlots;
     int size[3] = {(int) a.g.nx, (int) a.g.ny, (int) a.g.nz};
@@ -239,23 +240,21 @@ __global__ void __launch_bounds__(BLOCK) coupleParticles(const S* __restrict__ s
     // Every lane of the group now holds u; each computes the force (identical values).
     double f[3] = {0, 0, 0};
     if (active) {
-        double m = mass[i];
         double noise[4] = {0, 0, 0, 0};
         if (a.randomForce)
-            drawNormals<double, 4>(a.seed, TAG_PARTICLE, (uint32_t) i, a.step, noise);
+            drawNormals<double, 4>(a.seed, TAG_PARTICLE, index, a.step, noise);
         double u[3] = {ux, uy, uz};
         double amp = sqrt(2.0*a.gamma*m*a.kT);
         for (int k = 0; k < 3; k++) {
-            double v = velocities[3*(size_t) part + k]/a.velocityScale;
+            double v = vel[k]/a.velocityScale;
             f[k] = (v - u[k])*(-a.gamma*m);
             if (a.randomForce)
                 f[k] += noise[k]*amp;
         }
         if (lane == 0)
             for (int k = 0; k < 3; k++)
-                particleForce[3*(size_t) i + k] = f[k]*a.forceScale;
+                forceOut[k] = f[k]*a.forceScale;
     }
-    double wallAcc[3] = {0, 0, 0};
     if (a.isStep && inStencil) {
         // The reaction -w F: on a fluid node into its fixed-point accumulators (flagged in the touched mask, which
         // the fused pass reads and clears); on a solid node it leaves the fluid and goes to the wall
@@ -276,6 +275,22 @@ __global__ void __launch_bounds__(BLOCK) coupleParticles(const S* __restrict__ s
         if (any)
             atomicOr(touched + (node >> 5), 1u << (node & 31));
     }
+}
+
+template <typename S, int LANES>
+__global__ void __launch_bounds__(BLOCK) coupleParticles(const S* __restrict__ src, const uint32_t* __restrict__ mask,
+        const double* __restrict__ positions, const double* __restrict__ velocities, const int* __restrict__ particles,
+        const double* __restrict__ mass, const double* __restrict__ ucm, ParticleArgs a, double* __restrict__ particleForce,
+        unsigned long long* __restrict__ reactionFixed, uint32_t* __restrict__ touched, double* __restrict__ partials) {
+    constexpr int GROUPS = BLOCK/LANES;
+    int lane = threadIdx.x % LANES;
+    int i = blockIdx.x*GROUPS + threadIdx.x/LANES;
+    bool active = i < a.numParticles;
+    int part = active ? particles[i] : 0;
+    double m = active ? mass[i] : 0.0;
+    double wallAcc[3] = {0, 0, 0};
+    coupleGroup<S, LANES>(src, mask, positions + 3*(size_t) part, velocities + 3*(size_t) part, (uint32_t) i, m, active,
+            lane, ucm, a, particleForce + 3*(size_t) (active ? i : 0), reactionFixed, touched, wallAcc);
     if (partials != nullptr) {
         __shared__ double lds[BLOCK][3];
         for (int k = 0; k < 3; k++)
@@ -290,6 +305,51 @@ __global__ void __launch_bounds__(BLOCK) coupleParticles(const S* __restrict__ s
                 partials[3*blockIdx.x + k] = s[k];
         }
     }
+}
+
+/* Batched replicas: the coupling of every particle of every replica in one launch (particles of one replica are
+   contiguous; particleReplica gives the replica of each). */
+template <int LANES>
+__global__ void __launch_bounds__(BLOCK) coupleBatch(const ReplicaDesc* __restrict__ table,
+        const uint16_t* __restrict__ particleReplica, int numParticles, const double* __restrict__ positions,
+        const double* __restrict__ velocities, const double* __restrict__ mass, double* __restrict__ forces,
+        const int64_t* __restrict__ stepCounter) {
+    constexpr int GROUPS = BLOCK/LANES;
+    int lane = threadIdx.x % LANES;
+    int p = blockIdx.x*GROUPS + threadIdx.x/LANES;
+    bool active = p < numParticles;
+    int pp = active ? p : 0;
+    const ReplicaDesc* d = table + particleReplica[pp];
+    ParticleArgs a = d->part;
+    a.step = *stepCounter;
+    a.isStep = 1;
+    const float* src = (a.step & 1) ? d->popB : d->popA;
+    double wallAcc[3] = {0, 0, 0};
+    coupleGroup<float, LANES>(src, nullptr, positions + 3*(size_t) pp, velocities + 3*(size_t) pp,
+            (uint32_t) (pp - d->particleBegin), active ? mass[pp] : 0.0, active, lane, nullptr, a, forces + 3*(size_t) pp,
+            d->reactionFixed, d->touched, wallAcc);
+}
+
+/* Leapfrog Verlet of the batch (the coupling force is the only force).  The step counter is advanced by advanceStep,
+   launched after this kernel on the same stream. */
+__global__ void __launch_bounds__(BLOCK) verletBatch(const ReplicaDesc* __restrict__ table,
+        const uint16_t* __restrict__ particleReplica, int numParticles, double* __restrict__ positions,
+        double* __restrict__ velocities, const double* __restrict__ forces, const double* __restrict__ invMass) {
+    int p = blockIdx.x*BLOCK + threadIdx.x;
+    if (p >= numParticles)
+        return;
+    double dt = table[particleReplica[p]].dt;
+    for (int k = 0; k < 3; k++) {
+        size_t idx = 3*(size_t) p + k;
+        double v = velocities[idx] + dt*forces[idx]*invMass[p];
+        velocities[idx] = v;
+        positions[idx] += dt*v;
+    }
+}
+
+__global__ void advanceStep(int64_t* stepCounter) {
+    if (threadIdx.x == 0)
+        *stepCounter += 1;
 }
 
 __global__ void __launch_bounds__(BLOCK) addForces(const double* __restrict__ particleForce, const int* __restrict__ particles,
@@ -377,6 +437,40 @@ void launchCouple(int storage, const void* src, const uint32_t* mask, const doub
                 touched, partials, stream);
         break;
     }
+}
+
+void launchCoupleBatch(int stencil, const ReplicaDesc* table, const uint16_t* particleReplica, int numParticles,
+        const double* positions, const double* velocities, const double* mass, double* forces,
+        const int64_t* stepCounter, hipStream_t stream) {
+    if (numParticles == 0)
+        return;
+    unsigned grid = (unsigned) coupleBlocks(stencil, numParticles);
+    switch (coupleLanes(stencil)) {
+    case 8:
+        coupleBatch<8><<<grid, BLOCK, 0, stream>>>(table, particleReplica, numParticles, positions, velocities, mass,
+                forces, stepCounter);
+        break;
+    case 32:
+        coupleBatch<32><<<grid, BLOCK, 0, stream>>>(table, particleReplica, numParticles, positions, velocities, mass,
+                forces, stepCounter);
+        break;
+    case 64:
+        coupleBatch<64><<<grid, BLOCK, 0, stream>>>(table, particleReplica, numParticles, positions, velocities, mass,
+                forces, stepCounter);
+        break;
+    default:
+        coupleBatch<1><<<grid, BLOCK, 0, stream>>>(table, particleReplica, numParticles, positions, velocities, mass,
+                forces, stepCounter);
+        break;
+    }
+}
+
+void launchVerletBatch(const ReplicaDesc* table, const uint16_t* particleReplica, int numParticles, double* positions,
+        double* velocities, const double* forces, const double* invMass, int64_t* stepCounter, hipStream_t stream) {
+    if (numParticles > 0)
+        verletBatch<<<gridFor(numParticles), BLOCK, 0, stream>>>(table, particleReplica, numParticles, positions,
+                velocities, forces, invMass);
+    advanceStep<<<1, 64, 0, stream>>>(stepCounter);
 }
 
 void launchAddForces(const double* particleForce, const int* particles, int count, double* forces, hipStream_t stream) {

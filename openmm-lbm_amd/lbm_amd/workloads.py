@@ -142,3 +142,43 @@ def config2(n=256, num_beads=20000, fluctuations=True, stencil=LBMForce.Keys, se
                      "lattice": [n, n, n], "beads": num_beads, "stencil": "keys", "fluctuations": bool(fluctuations),
                      "walls": "bounce-back z=0,255", "tau": 0.6, "dx_nm": w.dx, "dt_ps": w.dt}
     return w
+
+
+def config4(num_replicas=64, steps=1000, seed=4242, edges=(32, 48, 64, 96), max_beads=2000):
+    """64 independent periodic cubes: edge drawn from {32, 48, 64, 96}, tau uniform in [0.55, 1.0], 50-2000 beads of
+    100 Da (uniform positions, v = 0, gamma = 10/ps, trilinear), fluctuating fluid with kT uniform in [1.5, 3.0]
+    kJ/mol (particles at the same temperature) and a Philox seed per replica; dx = 1 nm, dt = 0.01 ps,
+    rho0 = 602.214 Da/nm^3; FP32."""
+    from .batch import Replica
+    rng = np.random.default_rng(seed)
+    dx, dt = 1.0, 0.01
+    replicas = []
+    rows = []
+    for r in range(num_replicas):
+        n = int(rng.choice(edges))
+        tau = float(rng.uniform(0.55, 1.0))
+        beads = int(rng.integers(50, max_beads + 1))
+        kT = float(rng.uniform(1.5, 3.0))
+        rseed = int(rng.integers(1, 2**31 - 1))
+        force = LBMForce()
+        force.setGridSize(n, n, n)
+        force.setFluidDensity(602.214)
+        force.setKinematicViscosity((tau - 0.5)/3.0*dx*dx/dt)
+        force.setFluidMomentumRemovalFrequency(0)
+        force.setFriction(10.0)
+        force.setTemperature(kT/BOLTZ)
+        force.setFluidFluctuations(True)
+        force.setRandomNumberSeed(rseed)
+        force.setInterpolationStencil(LBMForce.Trilinear)
+        for i in range(beads):
+            force.addParticle(i)
+        pos = rng.uniform(0, n*dx, size=(beads, 3))
+        replicas.append(Replica(force, (n*dx, n*dx, n*dx), dt, np.full(beads, 100.0), pos))
+        rows.append({"edge": n, "tau": tau, "beads": beads, "kT": kT, "seed": rseed})
+    nodes = sum(row["edge"]**3 for row in rows)
+    beads = sum(row["beads"] for row in rows)
+    description = {"workload": f"config4 {num_replicas} periodic replicas, edges {sorted(set(edges))}, tau U[0.55,1.0], "
+                               f"50-{max_beads} beads trilinear, fluctuating kT U[1.5,3.0], per-replica seeds",
+                   "replicas": num_replicas, "total_nodes": nodes, "total_beads": beads, "steps": steps,
+                   "dx_nm": dx, "dt_ps": dt}
+    return replicas, rows, description

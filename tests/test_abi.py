@@ -25,7 +25,9 @@ def test_headers_declare_entry_points():
 def test_gpu_library_exports_everything_declared():
     import lbm_amd
     lib = ctypes.CDLL(lbm_amd.LIBRARY_PATH)
-    for header in ("lbm_amd.h", "lbm_amd_harness.h"):
+    headers = sorted(h for h in os.listdir(os.path.join(ROOT, "include")) if h.endswith(".h"))
+    assert {"lbm_amd.h", "lbm_amd_harness.h", "lbm_amd_batch.h"} <= set(headers)
+    for header in headers:
         for name in declared_functions(header):
             assert hasattr(lib, name), f"{name} declared in {header} is not exported"
 

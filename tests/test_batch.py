@@ -1,0 +1,139 @@
+"""Batched replicas (include/lbm_amd_batch.h).
+
+On the GPU, a batch of uneven replicas (different shapes, relaxation times, temperatures, seeds and particle counts,
+fluctuations on) must reproduce, bit for bit, the same replicas run one by one through the single-lattice C-ABI
+with the device Verlet integrator: both use the same node update, the same Philox streams (per-replica seed, step and
+node or particle index) and fixed-point reaction sums, so any difference is a bug in the batching (tile table,
+step parity, particle offsets, graph replay).  The single-lattice path is itself pinned to the oracle by the parity
+tests.  On the CPU, the argument checks that run before any device work.
+"""
+import numpy as np
+import pytest
+
+from lbm_amd import Context, LBMError, LBMForce, System, VerletIntegrator
+from lbm_amd.batch import Batch, Replica
+from lbm_amd.harness import DeviceParticles
+from lbm_amd.workloads import BOLTZ
+
+
+def make_replica(grid, tau, beads, kT, seed, stencil=LBMForce.Trilinear, dx=1.0, dt=0.01, fluct=True,
+                 rng_seed=0):
+    nx, ny, nz = grid
+    force = LBMForce()
+    force.setGridSize(nx, ny, nz)
+    force.setFluidDensity(602.214)
+    force.setKinematicViscosity((tau - 0.5)/3.0*dx*dx/dt)
+    force.setFluidMomentumRemovalFrequency(0)
+    force.setFriction(10.0)
+    force.setTemperature(kT/BOLTZ)
+    force.setFluidFluctuations(fluct)
+    force.setRandomNumberSeed(seed)
+    force.setInterpolationStencil(stencil)
+    for i in range(beads):
+        force.addParticle(i)
+    rng = np.random.default_rng(rng_seed)
+    pos = rng.uniform(0, 1, size=(beads, 3))*np.array(grid)*dx
+    vel = rng.normal(0, 0.1, size=(beads, 3))
+    return Replica(force, (nx*dx, ny*dx, nz*dx), dt, np.full(beads, 100.0), pos, vel)
+
+
+def uneven_replicas():
+    return [
+        make_replica((12, 12, 12), 0.62, 9, 2.5, 11, rng_seed=1),
+        make_replica((16, 12, 10), 0.95, 30, 1.5, 12345, rng_seed=2),
+        make_replica((8, 8, 8), 0.55, 0, 3.0, 777, rng_seed=3),
+        make_replica((20, 9, 14), 0.8, 3, 2.0, 31, rng_seed=4),
+    ]
+
+
+def run_single(replica, steps):
+    """The same replica through the single-lattice C-ABI and the device Verlet integrator."""
+    system = System(replica.box)
+    masses = replica.masses if len(replica.masses) else [1.0]
+    for m in masses:
+        system.addParticle(m)
+    ctx = Context(system, VerletIntegrator(replica.stepSize), replica.force)
+    if len(replica.masses):
+        ctx.setPositions(replica.positions)
+        ctx.setVelocities(replica.velocities)
+    else:
+        ctx.setPositions(np.array([[0.1, 0.2, 0.3]]))
+        ctx.setVelocities(np.zeros((1, 3)))
+    dp = DeviceParticles(ctx)
+    dp.step(steps)
+    dp.pull()
+    rho, u = ctx.getFluidFields()
+    return rho, u, ctx.positions[:len(replica.masses)].copy(), ctx.velocities[:len(replica.masses)].copy()
+
+
+@pytest.mark.gpu
+def test_batch_matches_single_replicas_bit_for_bit():
+    replicas = uneven_replicas()
+    batch = Batch(replicas)
+    nodes, _ = batch.info()
+    assert nodes == sum(int(np.prod(r.force.getGridSize())) for r in replicas)
+    batch.step(27)                # two graph replays and seven single steps
+    batch.synchronize()
+    particles = batch.getParticles()
+    for r, replica in enumerate(replicas):
+        rho, u, pos, vel = run_single(replica, 27)
+        brho, bu = batch.getFluidFields(r)
+        bpos, bvel = particles[r]
+        print(f"replica {r} grid {replica.force.getGridSize()}: max|drho| {np.abs(brho - rho).max():.3e} "
+              f"max|du| {np.abs(bu - u).max():.3e}", flush=True)
+        assert np.array_equal(brho, rho), r
+        assert np.array_equal(bu, u), r
+        assert np.array_equal(bpos, pos), r
+        assert np.array_equal(bvel, vel), r
+
+
+@pytest.mark.gpu
+def test_batch_autotune_keeps_the_state():
+    replicas = uneven_replicas()[:2]
+    batch = Batch(replicas)
+    times, best = batch.autotune(10)      # 3 x (10 warm-up + 10 timed) steps
+    print("autotune ms/step:", times, "best", best, flush=True)
+    assert best in times and all(t > 0 for t in times.values())
+    batch.step(15)
+    _, step = batch.info()
+    assert step == 75
+    for r, replica in enumerate(replicas):
+        rho, u, pos, vel = run_single(replica, 75)
+        brho, bu = batch.getFluidFields(r)
+        assert np.array_equal(brho, rho) and np.array_equal(bu, u), r
+
+
+@pytest.mark.gpu
+def test_batch_fluid_temperature():
+    """Equipartition of the fluctuating replicas: the velocity variance of a node, averaged over the lattice and
+    over late steps, is kT/(rho dx^3) of each replica within 5% (the oracle gives 0.994 and 0.997 of it at 12^3,
+    tau = 0.6 and 0.8, over the same sampling)."""
+    replicas = [make_replica((16, 16, 16), 0.8, 0, 2.5, 5), make_replica((12, 12, 20), 0.6, 0, 1.5, 6)]
+    batch = Batch(replicas)
+    batch.step(200)
+    samples = [[] for _ in replicas]
+    for _ in range(20):
+        batch.step(10)
+        for r in range(len(replicas)):
+            _, u = batch.getFluidFields(r)
+            samples[r].append(np.var(u - u.mean(axis=0), axis=0).mean())
+    for r, replica in enumerate(replicas):
+        kT = replica.force.getTemperature()*BOLTZ
+        expected = kT/(602.214*1.0)          # (nm/ps)^2: kT / cell mass
+        measured = float(np.mean(samples[r]))
+        print(f"replica {r}: <u^2> {measured:.4e} expected {expected:.4e}", flush=True)
+        assert abs(measured/expected - 1) < 0.05
+
+
+@pytest.mark.parametrize("change, message", [
+    (lambda f: f.setInterpolationStencil(LBMForce.Keys), "same interpolation stencil"),
+    (lambda f: f.setFluidFluctuations(False), "fluctuations on"),
+    (lambda f: (f.setFaceBoundary(0, LBMForce.Velocity), f.setFaceBoundary(1, LBMForce.Velocity)), "periodic"),
+    (lambda f: f.setFluidMomentumRemovalFrequency(10), "momentum removal"),
+    (lambda f: f.setStorage(LBMForce.FP16), "FP32"),
+])
+def test_batch_rejects_what_it_does_not_support(change, message):
+    replicas = uneven_replicas()[:2]
+    change(replicas[1].force)
+    with pytest.raises(LBMError, match=message):
+        Batch(replicas)
