@@ -1,0 +1,259 @@
+/*
+ * Coupling kernels of the learned closure (include/lbm_amd_closure.h).
+ *
+ *  closureFeatures  one group of 32 lanes per particle: lanes 0-26 pull the 19 populations of the 27 nodes around the
+ *                   nearest node and form u = j/rho (FP64, as the stencil coupling does); lane 27 writes the offset,
+ *                   lane 28 the particle velocity, lanes 29-31 the zero padding.  The particle is out of distribution
+ *                   if a normalized velocity exceeds its bound, a neighbour is solid, or the neighbourhood leaves the
+ *                   lattice at an open face.
+ *  closureTargets   u_Keys(x) - u(nearest) with the stencil interpolation of the coupling (training pairs).
+ *  closureCouple    one lane per particle: the nearest-node drag with u(nearest) + du and its reaction on the nearest
+ *                   node, exactly as the nearest-node stencil coupling computes them (ReferenceLBMKernels.cpp:503-553);
+ *                   with du = 0 the two are identical bit for bit.
+ *  closureFallback  the Keys coupling of the out-of-distribution particles (ReferenceLBMKernels.cpp:643-697): fixed
+ *                   grid, each group walks the particles with a fixed stride, so the wall partial sums have a fixed
+ *                   order.
+ */
+#include "lbm_coupling.h"
+#include "lbm_mlp.h"
+
+namespace lbm {
+
+namespace {
+
+constexpr int FEATURE_LANES = 32;
+constexpr int FALLBACK_MAX_BLOCKS = 512;
+
+template <typename S>
+__global__ void __launch_bounds__(BLOCK) closureFeatures(const S* __restrict__ src, const uint32_t* __restrict__ mask,
+        const double* __restrict__ positions, const double* __restrict__ velocities, const int* __restrict__ particles,
+        const double* __restrict__ ucm, ParticleArgs a, ClosureNorm norm, _Float16* __restrict__ X,
+        float* __restrict__ raw, uint32_t* __restrict__ centreNode, double* __restrict__ centreVelocity,
+        uint8_t* __restrict__ ood) {
+    constexpr int GROUPS = BLOCK/FEATURE_LANES;
+    int lane = threadIdx.x % FEATURE_LANES;
+    int i = blockIdx.x*GROUPS + threadIdx.x/FEATURE_LANES;
+    bool active = i < a.numParticles;
+    bool out = false;
+    if (active) {
+        int part = particles[i];
+        const double* x = positions + 3*(size_t) part;
+        int size[3] = {(int) a.g.nx, (int) a.g.ny, (int) a.g.nz};
+        int openAxis[3] = {a.g.openX, a.g.openY, a.g.openZ};
+        _Float16* xi = X + (size_t) i*MLP_IN_PAD;
+        float* ri = raw ? raw + (size_t) i*MLP_IN : nullptr;
+        if (lane < 27) {
+            int d[3] = {lane % 3 - 1, (lane/3) % 3 - 1, lane/9 - 1};
+            int idx[3];
+            for (int k = 0; k < 3; k++) {
+                double s = wrapCoordinate(x[k], a.dx, size[k]);
+                int t = nearestIndex(s, size[k]) + d[k];
+                if (t < 0 || t >= size[k]) {
+                    if (openAxis[k])
+                        out = true;
+                    t = (t + size[k]) % size[k];
+                }
+                idx[k] = t;
+            }
+            uint32_t node = (uint32_t) (idx[0] + (int) a.g.nx*(idx[1] + (int) a.g.ny*idx[2]));
+            double u[3] = {0, 0, 0};
+            if (solidAt(mask, node))
+                out = true;
+            else
+                nodeVelocity(src, a.g, node, ucm, a.removal, 1.0, u);
+            if (lane == 13) {
+                centreNode[i] = node;
+                for (int c = 0; c < 3; c++)
+                    centreVelocity[3*(size_t) i + c] = u[c];
+            }
+            for (int c = 0; c < 3; c++) {
+                float r = (float) u[c];
+                float f = r*norm.uScale;
+                if (!(fabsf(f) <= norm.uBound))
+                    out = true;
+                xi[3 + 3*lane + c] = (_Float16) f;
+                if (ri)
+                    ri[3 + 3*lane + c] = r;
+            }
+        }
+        else if (lane == 27) {
+            for (int c = 0; c < 3; c++) {
+                double s = wrapCoordinate(x[c], a.dx, size[c]);
+                float r = (float) (2.0*(s - floor(s + 0.5)));
+                xi[c] = (_Float16) r;
+                if (ri)
+                    ri[c] = r;
+            }
+        }
+        else if (lane == 28) {
+            const double* v = velocities + 3*(size_t) part;
+            for (int c = 0; c < 3; c++) {
+                float r = (float) (v[c]/a.velocityScale);
+                float f = r*norm.vScale;
+                if (!(fabsf(f) <= norm.vBound))
+                    out = true;
+                xi[84 + c] = (_Float16) f;
+                if (ri)
+                    ri[84 + c] = r;
+            }
+        }
+        else {
+            for (int c = 0; c < 3; c++)
+                xi[MLP_IN + 3*(lane - 29) + c] = (_Float16) 0.0f;
+        }
+    }
+    // The 32 lanes of this particle are one half of the wave's ballot.
+    unsigned long long votes = __ballot(out);
+    unsigned mine = (unsigned) (votes >> (threadIdx.x & 32));
+    if (active && lane == 0)
+        ood[i] = mine != 0u ? 1 : 0;
+}
+
+template <typename S>
+__global__ void __launch_bounds__(BLOCK) closureTargets(const S* __restrict__ src, const uint32_t* __restrict__ mask,
+        const double* __restrict__ positions, const int* __restrict__ particles, const double* __restrict__ ucm,
+        ParticleArgs a, const double* __restrict__ centreVelocity, float* __restrict__ targets) {
+    constexpr int LANES = 64, GROUPS = BLOCK/LANES;
+    int lane = threadIdx.x % LANES;
+    int i = blockIdx.x*GROUPS + threadIdx.x/LANES;
+    bool active = i < a.numParticles;
+    int part = active ? particles[i] : 0;
+    double u[3];
+    interpolateGroup<S, LANES>(src, mask, positions + 3*(size_t) part, active, lane, ucm, a, u);
+    if (active && lane == 0)
+        for (int c = 0; c < 3; c++)
+            targets[3*(size_t) i + c] = (float) (u[c] - centreVelocity[3*(size_t) i + c]);
+}
+
+__global__ void __launch_bounds__(BLOCK) closureCouple(const double* __restrict__ velocities,
+        const int* __restrict__ particles, const double* __restrict__ mass, ParticleArgs a,
+        const uint32_t* __restrict__ centreNode, const double* __restrict__ centreVelocity, const float* __restrict__ du,
+        const uint8_t* __restrict__ ood, double* __restrict__ particleForce, unsigned long long* __restrict__ reactionFixed,
+        uint32_t* __restrict__ touched, unsigned long long* __restrict__ counters) {
+    int i = blockIdx.x*BLOCK + threadIdx.x;
+    bool active = i < a.numParticles;
+    bool fallback = active && ood[i] != 0;
+    bool network = active && !fallback;
+    unsigned long long netVotes = __ballot(network), fallVotes = __ballot(fallback);
+    if ((threadIdx.x & 63) == 0) {
+        if (netVotes)
+            atomicAdd(counters, (unsigned long long) __popcll(netVotes));
+        if (fallVotes)
+            atomicAdd(counters + 1, (unsigned long long) __popcll(fallVotes));
+    }
+    if (!network)
+        return;
+    double u[3];
+    for (int c = 0; c < 3; c++)
+        u[c] = centreVelocity[3*(size_t) i + c] + (double) du[3*(size_t) i + c];
+    double f[3];
+    dragForce(velocities + 3*(size_t) particles[i], u, (uint32_t) i, mass[i], a, f);
+    for (int c = 0; c < 3; c++)
+        particleForce[3*(size_t) i + c] = f[c]*a.forceScale;
+    if (a.isStep) {
+        StencilSlot sl;
+        sl.node = centreNode[i];
+        sl.weight = 1.0;
+        sl.inStencil = true;
+        sl.solid = false;
+        double wallAcc[3] = {0, 0, 0};
+        spreadReaction(sl, f, a, reactionFixed, touched, wallAcc);
+    }
+}
+
+template <typename S>
+__global__ void __launch_bounds__(BLOCK) closureFallback(const S* __restrict__ src, const uint32_t* __restrict__ mask,
+        const double* __restrict__ positions, const double* __restrict__ velocities, const int* __restrict__ particles,
+        const double* __restrict__ mass, const double* __restrict__ ucm, ParticleArgs a, const uint8_t* __restrict__ ood,
+        double* __restrict__ particleForce, unsigned long long* __restrict__ reactionFixed, uint32_t* __restrict__ touched,
+        double* __restrict__ partials) {
+    constexpr int LANES = 64, GROUPS = BLOCK/LANES;
+    int lane = threadIdx.x % LANES;
+    double acc[3] = {0, 0, 0};
+    // One particle per group (a whole wave) at a time: the loop bound and the flag are uniform across the wave.
+    for (int i = blockIdx.x*GROUPS + threadIdx.x/LANES; i < a.numParticles; i += gridDim.x*GROUPS) {
+        if (!ood[i])
+            continue;
+        int part = particles[i];
+        double wallAcc[3] = {0, 0, 0};
+        coupleGroup<S, LANES>(src, mask, positions + 3*(size_t) part, velocities + 3*(size_t) part, (uint32_t) i,
+                mass[i], true, lane, ucm, a, particleForce + 3*(size_t) i, reactionFixed, touched, wallAcc);
+        for (int k = 0; k < 3; k++)
+            acc[k] += wallAcc[k];
+    }
+    if (partials != nullptr) {
+        __shared__ double lds[BLOCK][3];
+        for (int k = 0; k < 3; k++)
+            lds[threadIdx.x][k] = acc[k];
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double s[3] = {0, 0, 0};
+            for (int t = 0; t < BLOCK; t++)
+                for (int k = 0; k < 3; k++)
+                    s[k] += lds[t][k];
+            for (int k = 0; k < 3; k++)
+                partials[3*blockIdx.x + k] = s[k];
+        }
+    }
+}
+
+unsigned gridOf(int64_t items, int perBlock) {
+    return (unsigned) ((items + perBlock - 1)/perBlock);
+}
+
+}  // namespace
+
+#define LBM_CLOSURE_STORAGE(storage, CALL) \
+    switch (storage) { \
+    case LBM_STORAGE_FP16: { using S = __half; CALL; break; } \
+    case LBM_STORAGE_BF16: { using S = __hip_bfloat16; CALL; break; } \
+    case LBM_STORAGE_FP64: { using S = double; CALL; break; } \
+    default: { using S = float; CALL; break; } \
+    }
+
+void launchClosureFeatures(int storage, const void* src, const uint32_t* mask, const double* positions,
+        const double* velocities, const int* particles, const double* ucm, const ParticleArgs& a, const ClosureNorm& norm,
+        _Float16* X, float* raw, uint32_t* centreNode, double* centreVelocity, uint8_t* ood, hipStream_t stream) {
+    if (a.numParticles == 0)
+        return;
+    unsigned grid = gridOf(a.numParticles, BLOCK/FEATURE_LANES);
+    LBM_CLOSURE_STORAGE(storage, (closureFeatures<S><<<grid, BLOCK, 0, stream>>>((const S*) src, mask, positions,
+            velocities, particles, ucm, a, norm, X, raw, centreNode, centreVelocity, ood)))
+}
+
+void launchClosureTargets(int storage, const void* src, const uint32_t* mask, const double* positions,
+        const int* particles, const double* ucm, const ParticleArgs& a, const double* centreVelocity, float* targets,
+        hipStream_t stream) {
+    if (a.numParticles == 0)
+        return;
+    unsigned grid = gridOf(a.numParticles, BLOCK/64);
+    LBM_CLOSURE_STORAGE(storage, (closureTargets<S><<<grid, BLOCK, 0, stream>>>((const S*) src, mask, positions,
+            particles, ucm, a, centreVelocity, targets)))
+}
+
+void launchClosureCouple(const double* velocities, const int* particles, const double* mass, const ParticleArgs& a,
+        const uint32_t* centreNode, const double* centreVelocity, const float* du, const uint8_t* ood,
+        double* particleForce, unsigned long long* reactionFixed, uint32_t* touched, unsigned long long* counters,
+        hipStream_t stream) {
+    if (a.numParticles == 0)
+        return;
+    closureCouple<<<gridOf(a.numParticles, BLOCK), BLOCK, 0, stream>>>(velocities, particles, mass, a, centreNode,
+            centreVelocity, du, ood, particleForce, reactionFixed, touched, counters);
+}
+
+int closureFallbackBlocks(int numParticles) {
+    return std::max(1, std::min((int) gridOf(numParticles, BLOCK/64), FALLBACK_MAX_BLOCKS));
+}
+
+void launchClosureFallback(int storage, const void* src, const uint32_t* mask, const double* positions,
+        const double* velocities, const int* particles, const double* mass, const double* ucm, const ParticleArgs& a,
+        const uint8_t* ood, double* particleForce, unsigned long long* reactionFixed, uint32_t* touched, double* partials,
+        hipStream_t stream) {
+    if (a.numParticles == 0)
+        return;
+    unsigned grid = (unsigned) closureFallbackBlocks(a.numParticles);
+    LBM_CLOSURE_STORAGE(storage, (closureFallback<S><<<grid, BLOCK, 0, stream>>>((const S*) src, mask, positions,
+            velocities, particles, mass, ucm, a, ood, particleForce, reactionFixed, touched, partials)))
+}
+
+}  // namespace lbm

@@ -20,7 +20,7 @@
  *
  * Everything runs in order on one HIP stream owned by the context.
  */
-#include "lbm_internal.h"
+#include "lbm_mlp.h"
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -135,6 +135,14 @@ struct lbm_context {
     uint32_t* touched = nullptr;                   // one bit per node: accumulators holding a reaction
     double* stagePos = nullptr;
     double* stageVel = nullptr;
+    // Learned closure (include/lbm_amd_closure.h): the network and its per-particle buffers.
+    lbm_mlp* closure = nullptr;
+    _Float16* closureX = nullptr;             // normalized inputs [n][96]
+    uint32_t* closureNode = nullptr;          // nearest node
+    double* closureU = nullptr;               // u(nearest), lattice units
+    float* closureDu = nullptr;               // network output
+    uint8_t* closureOod = nullptr;            // out-of-distribution flags
+    unsigned long long* closureCounters = nullptr;   // couplings through the network / the fallback
     // Timing of the fused pass with HIP events on the context's stream (diagnostics).
     bool profile = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
@@ -298,6 +306,19 @@ void couple(lbm_context* c, const double* dPos, const double* dVel, int64_t step
     if (a.removal)
         reduceMoments(c);
     double* partials = isStep ? c->wallPartials + 3*(size_t) c->wallCoupleOffset : nullptr;
+    if (c->closure != nullptr) {
+        // Nearest node + network for the particles in distribution, Keys for the others.
+        ClosureNorm norm = mlpClosureNorm(c->closure);
+        launchClosureFeatures(c->p.storage, c->popA, c->mask, dPos, dVel, c->particles, c->ucm, a, norm, c->closureX,
+                nullptr, c->closureNode, c->closureU, c->closureOod, c->stream);
+        launchMlpInfer(mlpDeviceWeights(c->closure), c->closureX, c->numParticles, norm.outScale, c->closureDu,
+                c->stream);
+        launchClosureCouple(dVel, c->particles, c->mass, a, c->closureNode, c->closureU, c->closureDu, c->closureOod,
+                c->particleForce, c->reactionFixed, c->touched, c->closureCounters, c->stream);
+        launchClosureFallback(c->p.storage, c->popA, c->mask, dPos, dVel, c->particles, c->mass, c->ucm, a,
+                c->closureOod, c->particleForce, c->reactionFixed, c->touched, partials, c->stream);
+        return;
+    }
     launchCouple(c->p.storage, c->popA, c->mask, dPos, dVel, c->particles, c->mass, c->ucm, a, c->particleForce,
             c->reactionFixed, c->touched, partials, c->stream);
 }
@@ -410,7 +431,8 @@ void destroy(lbm_context* c) {
     clearEvents(c);
     void* buffers[] = {c->popA, c->popB, c->mask, c->wallNodes, c->wallLinks, c->boundary, c->reducePartials, c->totals,
                        c->ucm, c->wallPartials, c->particles, c->mass, c->particleForce, c->reactionFixed,
-                       c->touched, c->stagePos, c->stageVel};
+                       c->touched, c->stagePos, c->stageVel, c->closureX, c->closureNode, c->closureU, c->closureDu,
+                       c->closureOod, c->closureCounters};
     for (void* b : buffers)
         if (b)
             (void) hipFree(b);
@@ -859,6 +881,86 @@ int lbm_profile_fused(lbm_context* c, int enable, double* totalMs, int* count) {
             *count = (int) c->events.size();
         clearEvents(c);
         c->profile = enable != 0;
+    })
+}
+
+int lbm_closure_pairs(lbm_context* c, const double* positions, const double* velocities, float* features,
+        float* targets) {
+    LBM_CHECK_CTX(c);
+    LBM_TRY({
+        if (c->p.interpolation_stencil != LBM_STENCIL_KEYS)
+            throw Error(LBM_ERROR_INVALID, "lbm_closure_pairs: the context must use the Keys stencil");
+        int n = c->numParticles;
+        if (n == 0)
+            return LBM_OK;
+        size_t bytes = 3*sizeof(double)*(size_t) c->numSystem;
+        HIP_CHECK(hipMemcpyAsync(c->stagePos, positions, bytes, hipMemcpyHostToDevice, c->stream));
+        HIP_CHECK(hipMemcpyAsync(c->stageVel, velocities, bytes, hipMemcpyHostToDevice, c->stream));
+        // The coupling of the next step would see this state: the same removal shift.
+        ParticleArgs a = particleArgs(c, c->stepIndex, false);
+        if (a.removal)
+            reduceMoments(c);
+        ClosureNorm norm = {1.0f, 1.0f, 1.0f, 3.0e38f, 3.0e38f};
+        _Float16* X = deviceAlloc<_Float16>((size_t) n*MLP_IN_PAD);
+        float* raw = deviceAlloc<float>((size_t) n*MLP_IN);
+        uint32_t* node = deviceAlloc<uint32_t>(n);
+        double* u = deviceAlloc<double>(3*(size_t) n);
+        uint8_t* ood = deviceAlloc<uint8_t>(n);
+        float* t = deviceAlloc<float>(3*(size_t) n);
+        launchClosureFeatures(c->p.storage, c->popA, c->mask, c->stagePos, c->stageVel, c->particles, c->ucm, a, norm, X,
+                raw, node, u, ood, c->stream);
+        launchClosureTargets(c->p.storage, c->popA, c->mask, c->stagePos, c->particles, c->ucm, a, u, t, c->stream);
+        hipError_t e = hipGetLastError();
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(features, raw, (size_t) n*MLP_IN*sizeof(float), hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(targets, t, 3*(size_t) n*sizeof(float), hipMemcpyDeviceToHost, c->stream);
+        (void) hipStreamSynchronize(c->stream);
+        void* scratch[] = {X, raw, node, u, ood, t};
+        for (void* b : scratch)
+            (void) hipFree(b);
+        HIP_CHECK(e);
+    })
+}
+
+int lbm_set_closure(lbm_context* c, lbm_mlp* mlp) {
+    LBM_CHECK_CTX(c);
+    LBM_TRY({
+        if (mlp != nullptr) {
+            if (c->p.interpolation_stencil != LBM_STENCIL_KEYS)
+                throw Error(LBM_ERROR_INVALID, "lbm_set_closure: the context must use the Keys stencil (the fallback)");
+            if (c->p.drag_scheme != LBM_DRAG_EXPLICIT)
+                throw Error(LBM_ERROR_UNSUPPORTED, "lbm_set_closure: the closure corrects the explicit drag only");
+            size_t n = (size_t) std::max(c->numParticles, 1);
+            if (c->closureX == nullptr) {
+                c->closureX = deviceAlloc<_Float16>(n*MLP_IN_PAD);
+                c->closureNode = deviceAlloc<uint32_t>(n);
+                c->closureU = deviceAlloc<double>(3*n);
+                c->closureDu = deviceAlloc<float>(3*n);
+                c->closureOod = deviceAlloc<uint8_t>(n);
+                c->closureCounters = deviceAlloc<unsigned long long>(2);
+                HIP_CHECK(hipMemsetAsync(c->closureCounters, 0, 2*sizeof(unsigned long long), c->stream));
+            }
+        }
+        HIP_CHECK(hipStreamSynchronize(c->stream));
+        c->closure = mlp;
+        c->stepForcesCurrent = false;
+    })
+}
+
+int lbm_closure_stats(lbm_context* c, int64_t* network, int64_t* fallback) {
+    LBM_CHECK_CTX(c);
+    LBM_TRY({
+        unsigned long long counts[2] = {0, 0};
+        if (c->closureCounters != nullptr) {
+            HIP_CHECK(hipMemcpyAsync(counts, c->closureCounters, sizeof(counts), hipMemcpyDeviceToHost, c->stream));
+            HIP_CHECK(hipMemsetAsync(c->closureCounters, 0, sizeof(counts), c->stream));
+            HIP_CHECK(hipStreamSynchronize(c->stream));
+        }
+        if (network)
+            *network = (int64_t) counts[0];
+        if (fallback)
+            *fallback = (int64_t) counts[1];
     })
 }
 

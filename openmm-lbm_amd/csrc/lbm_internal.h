@@ -119,6 +119,14 @@ void launchCouple(int storage, const void* src, const uint32_t* mask, const doub
         unsigned long long* reactionFixed, uint32_t* touched, double* partials, hipStream_t stream);
 void launchAddForces(const double* particleForce, const int* particles, int count, double* forces, hipStream_t stream);
 
-__host__ __device__ int stencilWidth(int stencil);
+/* Nodes per axis of an interpolation stencil (LBMStencils.h:42-104). */
+__host__ __device__ inline int stencilWidth(int stencil) {
+    switch (stencil) {
+    case LBM_STENCIL_TRILINEAR: return 2;
+    case LBM_STENCIL_THREE_POINT: return 3;
+    case LBM_STENCIL_KEYS: return 4;
+    default: return 1;
+    }
+}
 
 }  // namespace lbm

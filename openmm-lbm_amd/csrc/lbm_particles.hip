@@ -16,92 +16,9 @@
  *    of the flagged nodes and clears them and the mask (the reference's sumCellReactions and
  *    clearCellReactions, lbmCoupling.cc:251-272, 1852-1877, folded into the lattice pass).
  */
-#include "lbm_internal.h"
+#include "lbm_coupling.h"
 
 namespace lbm {
-
-__host__ __device__ int stencilWidth(int stencil) {
-    switch (stencil) {
-    case LBM_STENCIL_TRILINEAR: return 2;
-    case LBM_STENCIL_THREE_POINT: return 3;
-    case LBM_STENCIL_KEYS: return 4;
-    default: return 1;
-    }
-}
-
-/* Position wrapped into [0, n) in lattice spacings (ReferenceLBMKernels.cpp:978-979), without FMA
-   contraction so that the node of a particle is the same bit for bit as on the CPU. */
-__device__ __forceinline__ double wrapCoordinate(double x, double dx, int n) {
-#pragma clang fp contract(off)
-    double s = x/dx;
-    s -= floor(s/n)*n;
-    return s;
-}
-
-__device__ __forceinline__ int nearestIndex(double s, int n) {
-#pragma clang fp contract(off)
-    return ((int) floor(s + 0.5)) % n;
-}
-
-/* Kernel phi(r) of the stencils (LBMStencils.h:42-62). */
-__device__ __forceinline__ double stencilKernel(int stencil, double r) {
-#pragma clang fp contract(off)
-    double s = fabs(r);
-    switch (stencil) {
-    case LBM_STENCIL_TRILINEAR:
-        return s < 1.0 ? 1.0 - s : 0.0;
-    case LBM_STENCIL_THREE_POINT:
-        if (s <= 0.5)
-            return (1.0 + sqrt(1.0 - 3.0*s*s))/3.0;
-        if (s < 1.5)
-            return (5.0 - 3.0*s - sqrt(fmax(0.0, 1.0 - 3.0*(1.0 - s)*(1.0 - s))))/6.0;
-        return 0.0;
-    case LBM_STENCIL_KEYS:
-        if (s <= 1.0)
-            return 1.0 - 2.5*s*s + 1.5*s*s*s;
-        if (s < 2.0)
-            return 2.0 - 4.0*s + 2.5*s*s - 0.5*s*s*s;
-        return 0.0;
-    default:
-        return (r >= -0.5 && r < 0.5) ? 1.0 : 0.0;
-    }
-}
-
-/* Nodes and weights of the stencil along one axis (LBMStencils.h:73-95): width w nodes from floor(s),
-   round(s)-1 or floor(s)-1, wrapped or clamped at open faces, repeated nodes merged into the first slot. */
-__device__ __forceinline__ void axisWeights(int stencil, int w, double s, int n, bool open, int index[4], double weight[4]) {
-#pragma clang fp contract(off)
-    int first;
-    if (stencil == LBM_STENCIL_THREE_POINT)
-        first = (int) floor(s + 0.5) - 1;
-    else if (stencil == LBM_STENCIL_KEYS)
-        first = (int) floor(s) - 1;
-    else if (stencil == LBM_STENCIL_TRILINEAR)
-        first = (int) floor(s);
-    else
-        first = (int) floor(s + 0.5);
-    for (int a = 0; a < 4; a++) {
-        if (a >= w)
-            break;
-        int i = first + a;
-        weight[a] = stencilKernel(stencil, s - (double) i);
-        index[a] = open ? min(max(i, 0), n - 1) : ((i % n) + n) % n;
-    }
-    for (int a = 1; a < 4; a++) {
-        if (a >= w)
-            break;
-        for (int b = 0; b < a; b++)
-            if (index[b] == index[a]) {
-                weight[b] += weight[a];
-                weight[a] = 0.0;
-                break;
-            }
-    }
-}
-
-__device__ __forceinline__ bool solidAt(const uint32_t* mask, uint32_t node) {
-    return mask != nullptr && isSolid(mask, node);
-}
 
 /* ------------------------------------------------------------------------------------------ */
 __global__ void __launch_bounds__(BLOCK) reflectParticles(const double* __restrict__ positions, double* __restrict__ velocities,
@@ -162,121 +79,6 @@ __global__ void __launch_bounds__(BLOCK) reflectParticles(const double* __restri
 }
 
 /* ------------------------------------------------------------------------------------------ */
-/*
- * The coupling of one particle by a group of LANES lanes (one lane per stencil slot; every lane of the group calls
- * it, inactive ones with active = false).  x, v: the particle's position (nm) and velocity (nm/ps); index: its
- * coupled-particle index (the counter of its random numbers); m: its mass in lattice units.  The force (kJ/mol/nm) is
- * written to forceOut by the first lane; in a lattice step the reaction goes to the accumulators and, on solid nodes,
- * into wallAcc (lattice momentum x momentumScale).
- */
-template <typename S, int LANES>
-__device__ __forceinline__ void coupleGroup(const S* __restrict__ src, const uint32_t* __restrict__ mask,
-        const double* x, const double* vel, uint32_t index, double m, bool active, int lane,
-        const double* __restrict__ ucm, const ParticleArgs& a, double* forceOut,
-        unsigned long long* __restrict__ reactionFixed, uint32_t* __restrict__ touched, double wallAcc[3]) {
-    int w = stencilWidth(a.stencil);
-    int numSlots = w*w*w;
-    int slot = lane;
-    bool inStencil = active && slot < numSlots;
-    int size[3] = {(int) a.g.nx, (int) a.g.ny, (int) a.g.nz};
-    int openAxis[3] = {a.g.openX, a.g.openY, a.g.openZ};
-    double weight = 0.0;
-    uint32_t node = 0;
-    if (inStencil) {
-        int sub[3] = {slot % w, (slot/w) % w, slot/(w*w)};
-        int nodeIdx[3];
-        weight = 1.0;
-        for (int k = 0; k < 3; k++) {
-            double s = wrapCoordinate(x[k], a.dx, size[k]);
-            int ix[4];
-            double wx[4];
-            if (a.stencil == LBM_STENCIL_NEAREST) {
-                ix[0] = nearestIndex(s, size[k]);
-                wx[0] = 1.0;
-            }
-            else
-                axisWeights(a.stencil, w, s, size[k], openAxis[k] != 0, ix, wx);
-            nodeIdx[k] = ix[sub[k]];
-            weight *= wx[sub[k]];
-        }
-        node = (uint32_t) (nodeIdx[0] + (int) a.g.nx*(nodeIdx[1] + (int) a.g.ny*nodeIdx[2]));
-    }
-    // Moments of the stencil node, as computeMoments gives them: solid nodes have rho = 0 and are skipped.
-    double ux = 0, uy = 0, uz = 0;
-    bool solid = inStencil && solidAt(mask, node);
-    if (inStencil && !solid) {
-        uint32_t ni, nj, nk;
-        nodeCoords(a.g, node, ni, nj, nk);
-        Neighbours nb = neighbours(a.g, ni, nj, nk);
-        typename StorageTraits<S>::Real f[NQ];
-        pullNode(src, a.g.n, node, nb, f);
-        double dr = 0, jx = 0, jy = 0, jz = 0;
-        staticFor<NQ>([&](auto I) {
-            constexpr int q = decltype(I)::value;
-            double v = (double) f[q];
-            dr += v;
-            jx = addTerm<CX[q]>(jx, v);
-            jy = addTerm<CY[q]>(jy, v);
-            jz = addTerm<CZ[q]>(jz, v);
-        });
-        double rho = 1.0 + dr;
-        if (a.removal) {
-            jx -= rho*ucm[0];
-            jy -= rho*ucm[1];
-            jz -= rho*ucm[2];
-        }
-        if (rho > 0) {
-            ux = jx*(weight/rho);
-            uy = jy*(weight/rho);
-            uz = jz*(weight/rho);
-        }
-    }
-#pragma unroll
-    for (int off = 1; off < LANES; off <<= 1) {
-        ux += __shfl_xor(ux, off, LANES);
-        uy += __shfl_xor(uy, off, LANES);
-        uz += __shfl_xor(uz, off, LANES);
-    }
-    // Every lane of the group now holds u; each computes the force (identical values).
-    double f[3] = {0, 0, 0};
-    if (active) {
-        double noise[4] = {0, 0, 0, 0};
-        if (a.randomForce)
-            drawNormals<double, 4>(a.seed, TAG_PARTICLE, index, a.step, noise);
-        double u[3] = {ux, uy, uz};
-        double amp = sqrt(2.0*a.gamma*m*a.kT);
-        for (int k = 0; k < 3; k++) {
-            double v = vel[k]/a.velocityScale;
-            f[k] = (v - u[k])*(-a.gamma*m);
-            if (a.randomForce)
-                f[k] += noise[k]*amp;
-        }
-        if (lane == 0)
-            for (int k = 0; k < 3; k++)
-                forceOut[k] = f[k]*a.forceScale;
-    }
-    if (a.isStep && inStencil) {
-        // The reaction -w F: on a fluid node into its fixed-point accumulators (flagged in the touched mask, which
-        // the fused pass reads and clears); on a solid node it leaves the fluid and goes to the wall
-        // (ReferenceLBMKernels.cpp:959-971).
-        bool any = false;
-        for (int k = 0; k < 3; k++) {
-            double r = -weight*f[k];
-            if (solid)
-                wallAcc[k] = r*a.momentumScale;
-            else {
-                long long fixed = llrint(r*FIXED_SCALE);
-                if (fixed != 0) {
-                    atomicAdd(reactionFixed + (size_t) k*a.g.n + node, (unsigned long long) fixed);
-                    any = true;
-                }
-            }
-        }
-        if (any)
-            atomicOr(touched + (node >> 5), 1u << (node & 31));
-    }
-}
-
 template <typename S, int LANES>
 __global__ void __launch_bounds__(BLOCK) coupleParticles(const S* __restrict__ src, const uint32_t* __restrict__ mask,
         const double* __restrict__ positions, const double* __restrict__ velocities, const int* __restrict__ particles,

@@ -1,0 +1,224 @@
+"""The learned coupling closure on the GPU (include/lbm_amd_closure.h), against the closure oracle (oracle/mlp_oracle.py,
+itself pinned by tests/test_closure_oracle.py) and against the library's own stencil coupling.
+
+Tolerances: features and targets are FP32 results of FP64 sums, compared with FP64 at 1e-6 relative; the network's
+outputs and gradients come from FP16 operands with FP32 accumulation and are compared with the oracle's emulation of
+the same FP16 roundings at 5e-3 and 2e-2 relative (norm of the difference over norm of the oracle, per parameter
+block); the couplings that must reduce to the stencil coupling are compared bit for bit.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import oracle_lib
+from lbm_amd import LBMForce
+from lbm_amd.closure import MLPClosure, pairs, stats, OFF_W1, OFF_B1, OFF_W2, OFF_B2, OFF_W3, OFF_B3, NUM_PARAMS
+from lbm_amd.workloads import BOLTZ
+from parity import Setup
+from test_closure_oracle import random_params, random_state
+
+sys.path.insert(0, os.path.join(oracle_lib.ROOT, "oracle"))
+import mlp_oracle as mo  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+BLOCKS = {"W1": (OFF_W1, OFF_B1), "b1": (OFF_B1, OFF_W2), "W2": (OFF_W2, OFF_B2), "b2": (OFF_B2, OFF_W3),
+          "W3": (OFF_W3, OFF_B3), "b3": (OFF_B3, NUM_PARAMS)}
+
+
+def bead_setup(n, state, pos, vel, stencil=LBMForce.Keys, temperature=0.0, fluct=False, seed=99, storage=None):
+    s = Setup((n, n, n), tau=0.7)
+    s.state = state
+    s.add_particles(pos, mass=100.0, velocities=vel)
+
+    def conf(f):
+        f.setFriction(10.0)
+        f.setTemperature(temperature)
+        f.setFluidFluctuations(fluct)
+        f.setRandomNumberSeed(seed)
+        f.setInterpolationStencil(stencil)
+    s.configure.append(conf)
+    return s
+
+
+def beads(rng, n, count):
+    pos = rng.uniform(0, n, size=(count, 3))
+    pos[:4] = np.floor(pos[:4]) + 0.5
+    return pos, rng.normal(scale=0.2, size=(count, 3))
+
+
+@pytest.mark.parametrize("storage", [LBMForce.FP64, LBMForce.FP32])
+def test_features_and_targets_match_the_oracle(storage):
+    n = 12
+    rng = np.random.default_rng(5)
+    pos, vel = beads(rng, n, 60)
+    ctx = bead_setup(n, random_state(n, rng), pos, vel).build(storage=storage)
+    feats, targs = pairs(ctx)
+    state = ctx.getFluidState()
+    ref_f = mo.features(state, (n, n, n), 1.0, 0.01, pos, vel)
+    ref_t = mo.targets(state, (n, n, n), 1.0, pos)
+    print("max |feature - oracle|", np.abs(feats - ref_f).max(), "max |target - oracle|", np.abs(targs - ref_t).max())
+    assert np.abs(feats - ref_f).max() <= 1e-6*np.abs(ref_f).max()
+    assert np.abs(targs - ref_t).max() <= 1e-6*np.abs(ref_f[:, 3:84]).max()
+
+
+def network_with(rng, scale=0.3):
+    net = MLPClosure(seed=3)
+    net.setParameters(random_params(rng, scale).astype(np.float32))
+    norm = dict(u_scale=40.0, v_scale=4.0, out_scale=60.0, u_bound=1e3, v_bound=1e3)
+    net.setNormalization(**norm)
+    return net, norm
+
+
+def raw_batch(rng, count):
+    raw = np.zeros((count, 87), dtype=np.float32)
+    raw[:, 0:3] = rng.uniform(-1, 1, size=(count, 3))
+    raw[:, 3:84] = rng.normal(scale=0.025, size=(count, 81))
+    raw[:, 84:87] = rng.normal(scale=0.25, size=(count, 3))
+    return raw
+
+
+def test_forward_matches_the_oracle_emulation():
+    rng = np.random.default_rng(21)
+    net, norm = network_with(rng)
+    raw = raw_batch(rng, 1000)
+    out = net.forward(raw)
+    params = net.getParameters().astype(np.float64)
+    y, _ = mo.forward(params, mo.normalize(raw.astype(np.float64), norm["u_scale"], norm["v_scale"]))
+    ref = y/norm["out_scale"]
+    err = np.abs(out - ref).max()/np.abs(ref).max()
+    print("forward relative error vs the FP16-emulating oracle:", err)
+    assert err < 5e-3
+
+
+def test_gradients_match_the_oracle():
+    rng = np.random.default_rng(22)
+    net, norm = network_with(rng)
+    raw = raw_batch(rng, 300)            # not a multiple of the 256-row chunk: padding rows must not contribute
+    t_raw = rng.normal(scale=0.02, size=(300, 3)).astype(np.float32)
+    g, loss = net.gradients(raw, t_raw)
+    params = net.getParameters().astype(np.float64)
+    x = mo.normalize(raw.astype(np.float64), norm["u_scale"], norm["v_scale"])
+    t = t_raw.astype(np.float64)*norm["out_scale"]
+    ref = mo.backward(params, x, t, emulate16=True)
+    ref_loss = mo.loss(params, x, t, emulate16=True)
+    assert abs(loss - ref_loss) <= 1e-3*ref_loss
+    for name, (a, b) in BLOCKS.items():
+        rel = np.linalg.norm(g[a:b] - ref[a:b])/max(np.linalg.norm(ref[a:b]), 1e-30)
+        print(f"gradient {name}: relative error {rel:.2e}")
+        assert rel < 2e-2, name
+    # Padding entries stay zero.
+    pad = np.concatenate([np.arange(OFF_W1 + 87*64, OFF_B1),
+                          (OFF_W3 + np.arange(64)[:, None]*16 + np.arange(3, 16)[None, :]).ravel(),
+                          np.arange(OFF_B3 + 3, NUM_PARAMS)])
+    assert np.all(g[pad] == 0)
+
+
+def run_pair(n, rng, steps, net=None, stencil_b=LBMForce.NearestNode):
+    """Two contexts from one setup with fluctuations and particle noise: A couples through the closure (Keys stencil),
+    B with the stencil stencil_b; both stepped in lockstep."""
+    pos, vel = beads(rng, n, 80)
+    state = random_state(n, rng, amplitude=0.01)
+    a = bead_setup(n, state, pos, vel, LBMForce.Keys, temperature=300.0, fluct=True).build()
+    b = bead_setup(n, state, pos, vel, stencil_b, temperature=300.0, fluct=True).build()
+    net.attach(a)
+    for _ in range(steps):
+        a.step(1)
+        b.step(1)
+    return a, b
+
+
+def test_zero_network_is_the_nearest_node_coupling():
+    rng = np.random.default_rng(8)
+    net = MLPClosure(seed=1)
+    net.setParameters(np.zeros(NUM_PARAMS, dtype=np.float32))
+    net.setNormalization(u_scale=1.0, v_scale=1.0, out_scale=1.0, u_bound=1e30, v_bound=1e30)
+    a, b = run_pair(12, rng, 15, net, LBMForce.NearestNode)
+    ra, ua = a.getFluidFields()
+    rb, ub = b.getFluidFields()
+    assert np.array_equal(ra, rb) and np.array_equal(ua, ub)
+    assert np.array_equal(a.positions, b.positions) and np.array_equal(a.velocities, b.velocities)
+    assert stats(a) == (80*15, 0)
+
+
+def test_out_of_distribution_particles_take_the_keys_coupling():
+    rng = np.random.default_rng(9)
+    net = MLPClosure(seed=1)
+    net.setNormalization(u_scale=1.0, v_scale=1.0, out_scale=1.0, u_bound=0.0, v_bound=0.0)
+    a, b = run_pair(12, rng, 15, net, LBMForce.Keys)
+    ra, ua = a.getFluidFields()
+    rb, ub = b.getFluidFields()
+    assert np.array_equal(ra, rb) and np.array_equal(ua, ub)
+    assert np.array_equal(a.positions, b.positions) and np.array_equal(a.velocities, b.velocities)
+    assert stats(a) == (0, 80*15)
+
+
+def test_closure_force_is_nearest_plus_network_correction():
+    """At one state: F_closure - F_nearest = friction m dx/dt Du (in distribution) and F_closure = F_Keys (out of
+    distribution), with Du the network's output for the particle's inputs."""
+    n = 12
+    rng = np.random.default_rng(10)
+    pos, vel = beads(rng, n, 64)
+    state = random_state(n, rng)
+    keys = bead_setup(n, state, pos, vel, LBMForce.Keys).build()
+    near = bead_setup(n, state, pos, vel, LBMForce.NearestNode).build()
+    feats, _ = pairs(keys)
+    net, norm = network_with(rng, scale=0.1)
+    # Put roughly a quarter of the particles out of distribution through the bead-velocity bound.
+    vb = np.quantile(np.abs(feats[:, 84:87]*norm["v_scale"]).max(axis=1), 0.75)
+    net.setNormalization(**dict(norm, v_bound=float(vb)))
+    du = net.forward(feats).astype(np.float64)
+    ood = (np.abs(feats[:, 84:87]*np.float32(norm["v_scale"])) > np.float32(vb)).any(axis=1)
+    net.attach(keys)
+    for c in (keys, near):
+        c.setStepCount(1)
+    f_cl = keys.getForces()
+    f_n = near.getForces()
+    MLPClosure.detach(keys)
+    f_k = keys.getForces()
+    expected = f_n + 10.0*100.0*1.0/0.01*du
+    scale = np.abs(f_n).max()
+    assert 0 < ood.sum() < len(ood)
+    assert np.abs(f_cl[~ood] - expected[~ood]).max() <= 1e-9*scale
+    assert np.array_equal(f_cl[ood], f_k[ood])
+
+
+def test_training_learns_the_keys_correction():
+    """Pairs from fluctuating states (16^3, kT = 2.5 kJ/mol): the GPU-trained network's error on held-out pairs is well
+    below that of the nearest-node coupling alone (whose error is the target itself)."""
+    n = 16
+    rng = np.random.default_rng(12)
+    s = Setup((n, n, n), tau=0.7)
+
+    def conf(f):
+        f.setTemperature(2.5/BOLTZ)
+        f.setFluidFluctuations(True)
+        f.setRandomNumberSeed(77)
+        f.setInterpolationStencil(LBMForce.Keys)
+        f.setFriction(10.0)
+    s.configure.append(conf)
+    count = 3000
+    s.add_particles(rng.uniform(0, n, size=(count, 3)), mass=100.0)
+    ctx = s.build()
+    feats, targs = [], []
+    ctx.step(50)
+    for snap in range(14):
+        ctx.step(10)
+        pos = rng.uniform(0, n, size=(count, 3))
+        vel = rng.normal(scale=np.sqrt(2.5/100.0), size=(count, 3))
+        f, t = pairs(ctx, pos, vel)
+        feats.append(f)
+        targs.append(t)
+    train_f, train_t = np.concatenate(feats[:12]), np.concatenate(targs[:12])
+    test_f, test_t = np.concatenate(feats[12:]), np.concatenate(targs[12:])
+    net = MLPClosure(seed=5)
+    losses = net.train(train_f, train_t, epochs=30, batch=512, lr=3e-3, decay=0.9, seed=2)
+    pred = net.forward(test_f)
+    err_net = np.sqrt(np.mean((pred - test_t)**2))
+    err_nearest = np.sqrt(np.mean(test_t**2))
+    print(f"losses {losses[0]:.4f} -> {losses[-1]:.4f}; held-out rms error: network {err_net:.3e}, "
+          f"nearest only {err_nearest:.3e} ({err_net/err_nearest:.3f})")
+    assert losses[-1] < 0.5*losses[0]
+    assert err_net < 0.6*err_nearest
