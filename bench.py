@@ -1,6 +1,6 @@
 """Benchmark of the gfx950 LBMForce platform on its headline workload.
 
-python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload config2|config2-fluid|periodic|config3|config4]
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload config2|config2-fluid|periodic|config3|config4|config5]
                 [--storage fp32|fp16|bf16] [--no-cpu-baseline]
 
 A step is one coupled integration step: beginStep (reflection), the extra forces, the kernel's lattice step
@@ -216,6 +216,85 @@ def batch_study(steps, warmup, sequential_steps=20):
     }
 
 
+def closure_study(steps, warmup, snapshots=10, interval=20, epochs=40, batch=2048):
+    """config5: the learned closure (nearest node + MLP, Keys fallback) against the Keys stencil on 128^3 with 100,000
+    chained beads.  Generates 10 x 100,000 training pairs from the Keys run itself, trains on the GPU, then times the
+    same coupled steps with Keys and with the closure, and judges the closure's bead forces against Keys at one state
+    (same random forces, so the difference is the interpolation alone)."""
+    import numpy as np
+    from lbm_amd import _lib, workloads
+    from lbm_amd.closure import MLPClosure, pairs, stats
+    w = workloads.config5()
+    print(f"bench: building config5 ({w.grid}, {len(w.masses)} beads)", file=sys.stderr, flush=True)
+    ctx = w.build()
+    dp = w.device_particles(ctx)
+    lib = ctx.lib
+    dp.step(warmup)
+    t0 = time.perf_counter()
+    feats, targs = [], []
+    for _ in range(snapshots):
+        dp.step(interval)
+        pos, vel = dp.pull()
+        f, t = pairs(ctx, pos, vel)
+        feats.append(f)
+        targs.append(t)
+    gen_s = time.perf_counter() - t0
+    X, T = np.concatenate(feats), np.concatenate(targs)
+    net = MLPClosure(seed=1)
+    t0 = time.perf_counter()
+    losses = net.train(X, T, epochs=epochs, batch=batch, lr=2e-3, decay=0.92, seed=1)
+    train_s = time.perf_counter() - t0
+    print(f"bench: {len(X)} pairs in {gen_s:.1f} s, trained {epochs} epochs in {train_s:.1f} s, loss {losses[0]:.4f} "
+          f"-> {losses[-1]:.4f}", file=sys.stderr, flush=True)
+
+    def timed(n):
+        _lib.check(lib, lib.lbm_synchronize(ctx.handle))
+        start = time.perf_counter()
+        dp.step(n)
+        _lib.check(lib, lib.lbm_synchronize(ctx.handle))
+        return (time.perf_counter() - start)/n
+
+    keys_s = timed(steps)
+    net.attach(ctx)
+    dp.step(10)
+    stats(ctx)
+    closure_s = timed(steps)
+    network, fallback = stats(ctx)
+    # Judge at the current state: the same step's forces with the closure and with Keys.
+    pos, vel = dp.pull()
+    f_closure = ctx.getForces()
+    MLPClosure.detach(ctx)
+    f_keys = ctx.getForces()
+    f, t = pairs(ctx, pos, vel)
+    scale = 10.0*100.0*w.dx/w.dt                 # friction x mass x dx/dt: lattice velocity -> drag force
+    slip = (t + f[:, 42:45]) - f[:, 84:87]        # u_Keys - v, lattice units
+    drag = scale*slip
+    df = f_closure - f_keys
+    rms = lambda a: float(np.sqrt(np.mean(np.sum(np.asarray(a, dtype=np.float64)**2, axis=1))))
+    pred = net.forward(f)
+    n = w.num_nodes
+    return {
+        "metric": "MLUPS & coupled steps/s (HBM GB/s %) 256^3+20k beads; FP16/MLP vs plain @err",
+        "value": round(keys_s/closure_s, 3),
+        "unit": "x Keys coupled steps/s (nearest node + MLP closure)",
+        "n_gpus": 1, "steps": steps, "warmup": warmup, "higher_is_better": True, "dtype": "f32 (MLP f16/f32-acc)",
+        "data": "synthetic",
+        "config": w.description,
+        "keys": {"ms_per_step": round(1e3*keys_s, 4), "coupled_steps_per_s": round(1/keys_s, 2),
+                 "mlups": round(n/keys_s/1e6, 1)},
+        "closure": {"ms_per_step": round(1e3*closure_s, 4), "coupled_steps_per_s": round(1/closure_s, 2),
+                    "mlups": round(n/closure_s/1e6, 1), "network_couplings": network, "fallback_couplings": fallback},
+        "force_error_vs_keys": {
+            "rms_rel_total_force": round(rms(df)/rms(f_keys), 5),
+            "rms_rel_drag_force": round(rms(df)/rms(drag), 5),
+            "nearest_only_rms_rel_drag_force": round(rms(scale*t)/rms(drag), 5),
+            "network_rms_rel_target": round(rms(pred - t)/rms(t), 4)},
+        "training": {"pairs": int(len(X)), "epochs": epochs, "batch": batch, "generation_s": round(gen_s, 2),
+                     "train_s": round(train_s, 2), "loss_first": round(float(losses[0]), 5),
+                     "loss_last": round(float(losses[-1]), 5)},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -239,6 +318,9 @@ def main():
         return 0
     if args.workload == "config4":
         print(json.dumps(batch_study(args.steps, args.warmup)), flush=True)
+        return 0
+    if args.workload == "config5":
+        print(json.dumps(closure_study(args.steps, args.warmup)), flush=True)
         return 0
 
     from lbm_amd import _lib

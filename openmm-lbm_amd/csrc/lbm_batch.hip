@@ -68,6 +68,8 @@ struct lbm_batch {
     int threads = FUSED_THREADS;
     int numParticles = 0;
     uint16_t* particleReplica = nullptr;
+    uint16_t* particleTiles = nullptr;      // replica of each work-group of the coupling
+    int numParticleTiles = 0;
     double* pos = nullptr;
     double* vel = nullptr;
     double* force = nullptr;
@@ -102,7 +104,7 @@ void buildTiles(lbm_batch* b, int threads) {
 }
 
 void launchStep(lbm_batch* b, bool timed = false) {
-    launchCoupleBatch(b->stencil, b->dDesc, b->particleReplica, b->numParticles, b->pos, b->vel, b->mass, b->force,
+    launchCoupleBatch(b->stencil, b->dDesc, b->particleTiles, b->numParticleTiles, b->pos, b->vel, b->mass, b->force,
             b->stepCounter, b->stream);
     std::pair<hipEvent_t, hipEvent_t> ev{nullptr, nullptr};
     if (timed) {
@@ -165,7 +167,8 @@ void destroyBatch(lbm_batch* b) {
         (void) hipGraphExecDestroy(b->graph);
     for (void* p : b->buffers)
         (void) hipFree(p);
-    void* others[] = {b->dDesc, b->tileReplica, b->particleReplica, b->pos, b->vel, b->force, b->mass, b->invMass,
+    void* others[] = {b->dDesc, b->tileReplica, b->particleReplica, b->particleTiles, b->pos, b->vel, b->force, b->mass,
+                      b->invMass,
                       b->stepCounter};
     for (void* p : others)
         if (p)
@@ -206,7 +209,7 @@ void createBatch(lbm_batch* b, int R, const lbm_params* params, const int* numPa
     hipCheck(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking), "stream");
     b->desc.resize(R);
     int particleBegin = 0;
-    std::vector<uint16_t> particleReplica;
+    std::vector<uint16_t> particleReplica, particleTiles;
     std::vector<double> latticeMass, invMass;
     for (int r = 0; r < R; r++) {
         const lbm_params& p = params[r];
@@ -243,7 +246,6 @@ void createBatch(lbm_batch* b, int R, const lbm_params* params, const int* numPa
         a.kT = p.kT*p.dt*p.dt/(cellMass*p.dx*p.dx);
         a.velocityScale = p.dx/p.dt;
         a.forceScale = cellMass*p.dx/(p.dt*p.dt);
-        a.momentumScale = cellMass*p.dx/p.dt;
         a.cellMass = cellMass;
         a.removal = 0;
         a.isStep = 1;
@@ -272,10 +274,20 @@ void createBatch(lbm_batch* b, int R, const lbm_params* params, const int* numPa
             latticeMass.push_back(m/cellMass);
             invMass.push_back(1.0/m);
         }
+        // Coupling work-groups of this replica: BLOCK/lanes particles each.
+        int perTile = BLOCK/coupleLanes(b->stencil);
+        int tiles = (numParticles[r] + perTile - 1)/perTile;
+        d.particleTileBegin = (uint32_t) particleTiles.size();
+        particleTiles.insert(particleTiles.end(), tiles, (uint16_t) r);
         particleBegin += numParticles[r];
         b->totalNodes += (int64_t) n;
     }
     b->numParticles = particleBegin;
+    b->numParticleTiles = (int) particleTiles.size();
+    b->particleTiles = alloc<uint16_t>(particleTiles.size(), "particle tiles");
+    if (!particleTiles.empty())
+        hipCheck(hipMemcpyAsync(b->particleTiles, particleTiles.data(), particleTiles.size()*sizeof(uint16_t),
+                hipMemcpyHostToDevice, b->stream), "upload");
     size_t np = (size_t) b->numParticles;
     b->dDesc = alloc<ReplicaDesc>(R, "replica table");
     b->particleReplica = alloc<uint16_t>(np, "particle table");

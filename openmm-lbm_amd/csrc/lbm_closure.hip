@@ -11,8 +11,8 @@
  *                   node, exactly as the nearest-node stencil coupling computes them (ReferenceLBMKernels.cpp:503-553);
  *                   with du = 0 the two are identical bit for bit.
  *  closureFallback  the Keys coupling of the out-of-distribution particles (ReferenceLBMKernels.cpp:643-697): fixed
- *                   grid, each group walks the particles with a fixed stride, so the wall partial sums have a fixed
- *                   order.
+ *                   grid, each wave scans the flags of 64 particles at a time with a fixed stride, so the wall partial
+ *                   sums have a fixed order.
  */
 #include "lbm_coupling.h"
 #include "lbm_mlp.h"
@@ -27,9 +27,9 @@ constexpr int FALLBACK_MAX_BLOCKS = 512;
 template <typename S>
 __global__ void __launch_bounds__(BLOCK) closureFeatures(const S* __restrict__ src, const uint32_t* __restrict__ mask,
         const double* __restrict__ positions, const double* __restrict__ velocities, const int* __restrict__ particles,
-        const double* __restrict__ ucm, ParticleArgs a, ClosureNorm norm, _Float16* __restrict__ X,
-        float* __restrict__ raw, uint32_t* __restrict__ centreNode, double* __restrict__ centreVelocity,
-        uint8_t* __restrict__ ood) {
+        const double* __restrict__ ucm, ParticleArgs a, ClosureNorm norm, const double* __restrict__ field,
+        _Float16* __restrict__ X, float* __restrict__ raw, uint32_t* __restrict__ centreNode,
+        double* __restrict__ centreVelocity, uint8_t* __restrict__ ood) {
     constexpr int GROUPS = BLOCK/FEATURE_LANES;
     int lane = threadIdx.x % FEATURE_LANES;
     int i = blockIdx.x*GROUPS + threadIdx.x/FEATURE_LANES;
@@ -59,6 +59,9 @@ __global__ void __launch_bounds__(BLOCK) closureFeatures(const S* __restrict__ s
             double u[3] = {0, 0, 0};
             if (solidAt(mask, node))
                 out = true;
+            else if (field != nullptr)
+                for (int c = 0; c < 3; c++)
+                    u[c] = field[3*(size_t) node + c];
             else
                 nodeVelocity(src, a.g, node, ucm, a.removal, 1.0, u);
             if (lane == 13) {
@@ -107,6 +110,21 @@ __global__ void __launch_bounds__(BLOCK) closureFeatures(const S* __restrict__ s
     unsigned mine = (unsigned) (votes >> (threadIdx.x & 32));
     if (active && lane == 0)
         ood[i] = mine != 0u ? 1 : 0;
+}
+
+/* u = j/rho of every fluid node (zero at solid nodes), FP64, [node][3]: one coalesced pass over the populations that
+   replaces the 27 scattered pulls per particle when the particles are dense. */
+template <typename S>
+__global__ void __launch_bounds__(BLOCK) closureField(const S* __restrict__ src, const uint32_t* __restrict__ mask,
+        const double* __restrict__ ucm, ParticleArgs a, double* __restrict__ field) {
+    uint32_t node = blockIdx.x*BLOCK + threadIdx.x;
+    if (node >= a.g.n)
+        return;
+    double u[3] = {0, 0, 0};
+    if (!solidAt(mask, node))
+        nodeVelocity(src, a.g, node, ucm, a.removal, 1.0, u);
+    for (int c = 0; c < 3; c++)
+        field[3*(size_t) node + c] = u[c];
 }
 
 template <typename S>
@@ -167,19 +185,26 @@ __global__ void __launch_bounds__(BLOCK) closureFallback(const S* __restrict__ s
         const double* __restrict__ mass, const double* __restrict__ ucm, ParticleArgs a, const uint8_t* __restrict__ ood,
         double* __restrict__ particleForce, unsigned long long* __restrict__ reactionFixed, uint32_t* __restrict__ touched,
         double* __restrict__ partials) {
-    constexpr int LANES = 64, GROUPS = BLOCK/LANES;
+    constexpr int LANES = 64;
     int lane = threadIdx.x % LANES;
+    int wave = blockIdx.x*(BLOCK/LANES) + threadIdx.x/LANES;
+    int waves = gridDim.x*(BLOCK/LANES);
     double acc[3] = {0, 0, 0};
-    // One particle per group (a whole wave) at a time: the loop bound and the flag are uniform across the wave.
-    for (int i = blockIdx.x*GROUPS + threadIdx.x/LANES; i < a.numParticles; i += gridDim.x*GROUPS) {
-        if (!ood[i])
-            continue;
-        int part = particles[i];
-        double wallAcc[3] = {0, 0, 0};
-        coupleGroup<S, LANES>(src, mask, positions + 3*(size_t) part, velocities + 3*(size_t) part, (uint32_t) i,
-                mass[i], true, lane, ucm, a, particleForce + 3*(size_t) i, reactionFixed, touched, wallAcc);
-        for (int k = 0; k < 3; k++)
-            acc[k] += wallAcc[k];
+    // Each wave reads the flags of 64 particles at a time and couples the flagged ones one after another, all 64 lanes
+    // on the 64 slots of the Keys stencil: a fixed assignment and order, so the wall partial sums are reproducible.
+    for (int base = wave*LANES; base < a.numParticles; base += waves*LANES) {
+        int j = base + lane;
+        unsigned long long votes = __ballot(j < a.numParticles && ood[j] != 0);
+        while (votes != 0ull) {
+            int i = base + __ffsll((long long) votes) - 1;
+            votes &= votes - 1ull;
+            int part = particles[i];
+            double wallAcc[3] = {0, 0, 0};
+            coupleGroup<S, LANES>(src, mask, positions + 3*(size_t) part, velocities + 3*(size_t) part, (uint32_t) i,
+                    mass[i], true, lane, ucm, a, particleForce + 3*(size_t) i, reactionFixed, touched, wallAcc);
+            for (int k = 0; k < 3; k++)
+                acc[k] += wallAcc[k];
+        }
     }
     if (partials != nullptr) {
         __shared__ double lds[BLOCK][3];
@@ -213,12 +238,19 @@ unsigned gridOf(int64_t items, int perBlock) {
 
 void launchClosureFeatures(int storage, const void* src, const uint32_t* mask, const double* positions,
         const double* velocities, const int* particles, const double* ucm, const ParticleArgs& a, const ClosureNorm& norm,
-        _Float16* X, float* raw, uint32_t* centreNode, double* centreVelocity, uint8_t* ood, hipStream_t stream) {
+        const double* field, _Float16* X, float* raw, uint32_t* centreNode, double* centreVelocity, uint8_t* ood,
+        hipStream_t stream) {
     if (a.numParticles == 0)
         return;
     unsigned grid = gridOf(a.numParticles, BLOCK/FEATURE_LANES);
     LBM_CLOSURE_STORAGE(storage, (closureFeatures<S><<<grid, BLOCK, 0, stream>>>((const S*) src, mask, positions,
-            velocities, particles, ucm, a, norm, X, raw, centreNode, centreVelocity, ood)))
+            velocities, particles, ucm, a, norm, field, X, raw, centreNode, centreVelocity, ood)))
+}
+
+void launchClosureField(int storage, const void* src, const uint32_t* mask, const double* ucm, const ParticleArgs& a,
+        double* field, hipStream_t stream) {
+    unsigned grid = gridOf(a.g.n, BLOCK);
+    LBM_CLOSURE_STORAGE(storage, (closureField<S><<<grid, BLOCK, 0, stream>>>((const S*) src, mask, ucm, a, field)))
 }
 
 void launchClosureTargets(int storage, const void* src, const uint32_t* mask, const double* positions,
@@ -242,7 +274,7 @@ void launchClosureCouple(const double* velocities, const int* particles, const d
 }
 
 int closureFallbackBlocks(int numParticles) {
-    return std::max(1, std::min((int) gridOf(numParticles, BLOCK/64), FALLBACK_MAX_BLOCKS));
+    return std::max(1, std::min((int) gridOf(numParticles, BLOCK), FALLBACK_MAX_BLOCKS));
 }
 
 void launchClosureFallback(int storage, const void* src, const uint32_t* mask, const double* positions,

@@ -90,7 +90,7 @@ static const double HW[NQ] = {1.0/3.0, 1.0/18.0, 1.0/18.0, 1.0/18.0, 1.0/18.0, 1
 
 struct CheckpointHeader {
     char magic[8];
-    int32_t version, nx, ny, nz, numParticles, pad;
+    int32_t version, nx, ny, nz, numParticles, dragScheme;
 };
 
 }  // namespace lbm
@@ -143,6 +143,10 @@ struct lbm_context {
     float* closureDu = nullptr;               // network output
     uint8_t* closureOod = nullptr;            // out-of-distribution flags
     unsigned long long* closureCounters = nullptr;   // couplings through the network / the fallback
+    double* closureField = nullptr;           // node velocities [node][3], when the particles are dense
+    // Centred drag: its solver state and the other forces of the host path.
+    CentredState* centred = nullptr;
+    double* stageForce = nullptr;
     // Timing of the fused pass with HIP events on the context's stream (diagnostics).
     bool profile = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
@@ -285,7 +289,6 @@ ParticleArgs particleArgs(lbm_context* c, int64_t step, bool isStep) {
     a.kT = c->p.kT*c->p.dt*c->p.dt/(c->cellMass()*c->p.dx*c->p.dx);
     a.velocityScale = c->p.dx/c->p.dt;
     a.forceScale = c->cellMass()*c->p.dx/(c->p.dt*c->p.dt);
-    a.momentumScale = c->cellMass()*c->p.dx/c->p.dt;
     a.cellMass = c->cellMass();
     a.removal = c->removalDue(step) ? 1 : 0;
     a.isStep = isStep ? 1 : 0;
@@ -301,16 +304,24 @@ void reduceMoments(lbm_context* c) {
     launchFinishReduce(c->reducePartials, c->numFluid, c->totals, c->ucm, c->stream);
 }
 
-void couple(lbm_context* c, const double* dPos, const double* dVel, int64_t step, bool isStep) {
+void couple(lbm_context* c, const double* dPos, const double* dVel, const double* dForces, int64_t step, bool isStep) {
     ParticleArgs a = particleArgs(c, step, isStep);
     if (a.removal)
         reduceMoments(c);
-    double* partials = isStep ? c->wallPartials + 3*(size_t) c->wallCoupleOffset : nullptr;
+    double* partials = isStep && c->mask != nullptr ? c->wallPartials + 3*(size_t) c->wallCoupleOffset : nullptr;
+    if (c->centred != nullptr) {
+        // Centred drag: the other forces (dForces, before the coupling adds to them) enter the particle velocities.
+        centredCouple(c->centred, c->p.storage, c->popA, c->mask, dPos, dVel, dForces, c->particles, c->mass, c->ucm, a,
+                c->p.body_acceleration, c->particleForce, c->reactionFixed, c->touched, partials, c->stream);
+        return;
+    }
     if (c->closure != nullptr) {
         // Nearest node + network for the particles in distribution, Keys for the others.
         ClosureNorm norm = mlpClosureNorm(c->closure);
-        launchClosureFeatures(c->p.storage, c->popA, c->mask, dPos, dVel, c->particles, c->ucm, a, norm, c->closureX,
-                nullptr, c->closureNode, c->closureU, c->closureOod, c->stream);
+        if (c->closureField != nullptr)
+            launchClosureField(c->p.storage, c->popA, c->mask, c->ucm, a, c->closureField, c->stream);
+        launchClosureFeatures(c->p.storage, c->popA, c->mask, dPos, dVel, c->particles, c->ucm, a, norm,
+                c->closureField, c->closureX, nullptr, c->closureNode, c->closureU, c->closureOod, c->stream);
         launchMlpInfer(mlpDeviceWeights(c->closure), c->closureX, c->numParticles, norm.outScale, c->closureDu,
                 c->stream);
         launchClosureCouple(dVel, c->particles, c->mass, a, c->closureNode, c->closureU, c->closureDu, c->closureOod,
@@ -323,11 +334,11 @@ void couple(lbm_context* c, const double* dPos, const double* dVel, int64_t step
             c->reactionFixed, c->touched, partials, c->stream);
 }
 
-void advanceFluid(lbm_context* c, const double* dPos, const double* dVel) {
+void advanceFluid(lbm_context* c, const double* dPos, const double* dVel, const double* dForces) {
     StepArgs a = stepArgs(c, c->stepIndex);
     bool particles = c->numParticles > 0;
     if (particles)
-        couple(c, dPos, dVel, c->stepIndex, true);   // runs the removal reduction when due
+        couple(c, dPos, dVel, dForces, c->stepIndex, true);   // runs the removal reduction when due
     else if (a.removal)
         reduceMoments(c);
     std::pair<hipEvent_t, hipEvent_t> ev{nullptr, nullptr};
@@ -404,15 +415,16 @@ void beginStepDevice(lbm_context* c, int64_t step, const double* dPos, double* d
 }
 
 /* Returns true if the forces of the coupled particles in c->particleForce are to be added. */
-void executeDevice(lbm_context* c, int64_t stepCount, bool includeForces, const double* dPos, const double* dVel) {
+void executeDevice(lbm_context* c, int64_t stepCount, bool includeForces, const double* dPos, const double* dVel,
+        const double* dForces) {
     bool repeated = c->stepForcesCurrent && stepCount == c->stepIndex - 1;
     if (c->stepPending) {
         c->stepPending = false;
-        advanceFluid(c, dPos, dVel);
+        advanceFluid(c, dPos, dVel, dForces);
     }
     else if (includeForces && c->numParticles > 0 && stepCount > 0 && !repeated) {
         c->stepForcesCurrent = false;
-        couple(c, dPos, dVel, stepCount, false);
+        couple(c, dPos, dVel, dForces, stepCount, false);
     }
     HIP_CHECK(hipGetLastError());
 }
@@ -429,10 +441,11 @@ void destroy(lbm_context* c) {
     if (c->stream)
         (void) hipStreamSynchronize(c->stream);
     clearEvents(c);
+    centredDestroy(c->centred);
     void* buffers[] = {c->popA, c->popB, c->mask, c->wallNodes, c->wallLinks, c->boundary, c->reducePartials, c->totals,
                        c->ucm, c->wallPartials, c->particles, c->mass, c->particleForce, c->reactionFixed,
                        c->touched, c->stagePos, c->stageVel, c->closureX, c->closureNode, c->closureU, c->closureDu,
-                       c->closureOod, c->closureCounters};
+                       c->closureOod, c->closureCounters, c->closureField, c->stageForce};
     for (void* b : buffers)
         if (b)
             (void) hipFree(b);
@@ -453,8 +466,8 @@ void create(const lbm_params* params, const int* solid, int numSolid, const int*
     c->p = p;
     if (p.nx <= 0 || p.ny <= 0 || p.nz <= 0)
         throw Error(LBM_ERROR_INVALID, "LBMForce: the grid size must be set to positive values with setGridSize()");
-    if (p.drag_scheme != LBM_DRAG_EXPLICIT)
-        throw Error(LBM_ERROR_UNSUPPORTED, "LBMForce: the Centered drag scheme is not implemented by this build");
+    if (p.drag_scheme != LBM_DRAG_EXPLICIT && p.drag_scheme != LBM_DRAG_CENTERED)
+        throw Error(LBM_ERROR_INVALID, "LBMForce: unknown drag scheme");
     if (p.storage < LBM_STORAGE_FP32 || p.storage > LBM_STORAGE_FP64)
         throw Error(LBM_ERROR_INVALID, "LBMForce: unknown storage type");
     if (numParts < 0 || numSys < 0 || (numParts > 0 && (!parts || !masses)))
@@ -524,9 +537,11 @@ void create(const lbm_params* params, const int* solid, int numSolid, const int*
             LBM_STENCIL_THREE_POINT ? 32 : p.interpolation_stencil == LBM_STENCIL_TRILINEAR ? 8 : 1);
     c->wallBounceOffset = 0;
     c->wallBoundaryOffset = c->wallBounceOffset + blocks(c->numWall);
+    // The coupling and the reflection give momentum to walls only at solid nodes: without any, no partials.
+    int wallParts = c->mask != nullptr ? numParts : 0;
     c->wallCoupleOffset = c->wallBoundaryOffset + blocks(c->numBoundary);
-    c->wallReflectOffset = c->wallCoupleOffset + (numParts + coupleGroups - 1)/coupleGroups;
-    c->wallPartialCount = c->wallReflectOffset + blocks(numParts);
+    c->wallReflectOffset = c->wallCoupleOffset + (wallParts + coupleGroups - 1)/coupleGroups;
+    c->wallPartialCount = c->wallReflectOffset + blocks(wallParts);
     c->wallPartials = deviceAlloc<double>(3*(size_t) std::max(c->wallPartialCount, 1));
     HIP_CHECK(hipMemsetAsync(c->wallPartials, 0, 3*sizeof(double)*std::max(c->wallPartialCount, 1), c->stream));
     if (numParts > 0) {
@@ -550,6 +565,15 @@ void create(const lbm_params* params, const int* solid, int numSolid, const int*
         HIP_CHECK(hipMemsetAsync(c->touched, 0, ((n + 31)/32)*sizeof(uint32_t), c->stream));
         c->stagePos = deviceAlloc<double>(3*(size_t) std::max(numSys, 1));
         c->stageVel = deviceAlloc<double>(3*(size_t) std::max(numSys, 1));
+        if (p.drag_scheme == LBM_DRAG_CENTERED) {
+            c->stageForce = deviceAlloc<double>(3*(size_t) std::max(numSys, 1));
+            try {
+                c->centred = centredCreate(numParts, p.interpolation_stencil, (uint32_t) n);
+            }
+            catch (const std::runtime_error& e) {
+                throw Error(LBM_ERROR_RUNTIME, e.what());
+            }
+        }
     }
     // The fluid at equilibrium with the initial velocity; solid nodes hold no fluid.
     launchInitEquilibrium(p.storage, c->popA, c->mask, c->g, p.initial_velocity[0], p.initial_velocity[1],
@@ -678,7 +702,7 @@ int lbm_execute_device(lbm_context* c, int64_t step, int includeForces, const do
         double* forces) {
     LBM_CHECK_CTX(c);
     LBM_TRY({
-        executeDevice(c, step, includeForces != 0, positions, velocities);
+        executeDevice(c, step, includeForces != 0, positions, velocities, forces);
         if (includeForces && forces)
             launchAddForces(c->particleForce, c->particles, c->numParticles, forces, c->stream);
         HIP_CHECK(hipGetLastError());
@@ -709,8 +733,10 @@ int lbm_execute(lbm_context* c, int64_t step, int includeForces, const double* p
             size_t bytes = 3*sizeof(double)*(size_t) c->numSystem;
             HIP_CHECK(hipMemcpyAsync(c->stagePos, positions, bytes, hipMemcpyHostToDevice, c->stream));
             HIP_CHECK(hipMemcpyAsync(c->stageVel, velocities, bytes, hipMemcpyHostToDevice, c->stream));
+            if (c->stageForce != nullptr && forces)
+                HIP_CHECK(hipMemcpyAsync(c->stageForce, forces, bytes, hipMemcpyHostToDevice, c->stream));
         }
-        executeDevice(c, step, includeForces != 0, c->stagePos, c->stageVel);
+        executeDevice(c, step, includeForces != 0, c->stagePos, c->stageVel, forces ? c->stageForce : nullptr);
         if (includeForces && c->numParticles > 0 && forces) {
             HIP_CHECK(hipMemcpyAsync(c->hostForce.data(), c->particleForce, c->hostForce.size()*sizeof(double),
                     hipMemcpyDeviceToHost, c->stream));
@@ -828,7 +854,7 @@ int lbm_create_checkpoint(lbm_context* c, void* buffer, size_t size) {
         h.ny = c->p.ny;
         h.nz = c->p.nz;
         h.numParticles = c->numParticles;
-        h.pad = 0;
+        h.dragScheme = c->p.drag_scheme;
         std::vector<double> s = getState(c);
         double m[3];
         wallMomentum(c, m);
@@ -851,6 +877,8 @@ int lbm_load_checkpoint(lbm_context* c, const void* buffer, size_t size) {
             throw Error(LBM_ERROR_INVALID, "LBMForce: the data is not a checkpoint of this LBMForce platform");
         if (h.nx != c->p.nx || h.ny != c->p.ny || h.nz != c->p.nz || h.numParticles != c->numParticles)
             throw Error(LBM_ERROR_INVALID, "LBMForce: the checkpoint was written for another lattice or System");
+        if (h.dragScheme != c->p.drag_scheme)
+            throw Error(LBM_ERROR_INVALID, "LBMForce: the checkpoint was written with a different drag scheme");
         if (size < sizeof(h) + stateBytes + 3*sizeof(double))
             throw Error(LBM_ERROR_INVALID, "LBMForce: the checkpoint is truncated");
         const char* in = (const char*) buffer;
@@ -907,8 +935,8 @@ int lbm_closure_pairs(lbm_context* c, const double* positions, const double* vel
         double* u = deviceAlloc<double>(3*(size_t) n);
         uint8_t* ood = deviceAlloc<uint8_t>(n);
         float* t = deviceAlloc<float>(3*(size_t) n);
-        launchClosureFeatures(c->p.storage, c->popA, c->mask, c->stagePos, c->stageVel, c->particles, c->ucm, a, norm, X,
-                raw, node, u, ood, c->stream);
+        launchClosureFeatures(c->p.storage, c->popA, c->mask, c->stagePos, c->stageVel, c->particles, c->ucm, a, norm,
+                nullptr, X, raw, node, u, ood, c->stream);
         launchClosureTargets(c->p.storage, c->popA, c->mask, c->stagePos, c->particles, c->ucm, a, u, t, c->stream);
         hipError_t e = hipGetLastError();
         if (e == hipSuccess)
@@ -941,6 +969,9 @@ int lbm_set_closure(lbm_context* c, lbm_mlp* mlp) {
                 c->closureCounters = deviceAlloc<unsigned long long>(2);
                 HIP_CHECK(hipMemsetAsync(c->closureCounters, 0, 2*sizeof(unsigned long long), c->stream));
             }
+            // Dense particles (more neighbourhood nodes than lattice nodes): one velocity pass over the lattice.
+            if (c->closureField == nullptr && 27*(int64_t) c->numParticles >= (int64_t) c->g.n)
+                c->closureField = deviceAlloc<double>(3*(size_t) c->g.n);
         }
         HIP_CHECK(hipStreamSynchronize(c->stream));
         c->closure = mlp;

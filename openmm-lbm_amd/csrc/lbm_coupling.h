@@ -94,9 +94,10 @@ struct StencilSlot {
 /* The fluid velocity of one node as the coupling sees it (computeMoments): u = j/rho of the pulled populations, with
    j -= rho u_cm when the momentum removal is due, times weight (ReferenceLBMKernels.cpp:503-553, 643-697).  Zero where
    rho <= 0. */
+/* rho and j (with j -= rho u_cm when the removal is due) of one node from its pulled populations, FP64. */
 template <typename S>
-__device__ __forceinline__ void nodeVelocity(const S* __restrict__ src, const Geometry& g, uint32_t node,
-        const double* __restrict__ ucm, int removal, double weight, double u[3]) {
+__device__ __forceinline__ double nodeMoments(const S* __restrict__ src, const Geometry& g, uint32_t node,
+        const double* __restrict__ ucm, int removal, double j[3]) {
     uint32_t ni, nj, nk;
     nodeCoords(g, node, ni, nj, nk);
     Neighbours nb = neighbours(g, ni, nj, nk);
@@ -117,11 +118,22 @@ __device__ __forceinline__ void nodeVelocity(const S* __restrict__ src, const Ge
         jy -= rho*ucm[1];
         jz -= rho*ucm[2];
     }
+    j[0] = jx;
+    j[1] = jy;
+    j[2] = jz;
+    return rho;
+}
+
+template <typename S>
+__device__ __forceinline__ void nodeVelocity(const S* __restrict__ src, const Geometry& g, uint32_t node,
+        const double* __restrict__ ucm, int removal, double weight, double u[3]) {
+    double j[3];
+    double rho = nodeMoments(src, g, node, ucm, removal, j);
     u[0] = u[1] = u[2] = 0.0;
     if (rho > 0) {
-        u[0] = jx*(weight/rho);
-        u[1] = jy*(weight/rho);
-        u[2] = jz*(weight/rho);
+        u[0] = j[0]*(weight/rho);
+        u[1] = j[1]*(weight/rho);
+        u[2] = j[2]*(weight/rho);
     }
 }
 
@@ -208,7 +220,7 @@ __device__ __forceinline__ void spreadReaction(const StencilSlot& sl, const doub
     for (int k = 0; k < 3; k++) {
         double r = -sl.weight*f[k];
         if (sl.solid)
-            wallAcc[k] = r*a.momentumScale;
+            wallAcc[k] = r;
         else {
             long long fixed = llrint(r*FIXED_SCALE);
             if (fixed != 0) {
@@ -226,7 +238,7 @@ __device__ __forceinline__ void spreadReaction(const StencilSlot& sl, const doub
  * it, inactive ones with active = false).  x, v: the particle's position (nm) and velocity (nm/ps); index: its
  * coupled-particle index (the counter of its random numbers); m: its mass in lattice units.  The force (kJ/mol/nm) is
  * written to forceOut by the first lane; in a lattice step the reaction goes to the accumulators and, on solid nodes,
- * into wallAcc (lattice momentum x momentumScale).
+ * into wallAcc (lattice momentum; the host converts the wall partial sums with m_c dx/dt).
  */
 template <typename S, int LANES>
 __device__ __forceinline__ void coupleGroup(const S* __restrict__ src, const uint32_t* __restrict__ mask,

@@ -59,7 +59,6 @@ struct ParticleArgs {
     double kT;              // lattice units
     double velocityScale;   // dx/dt
     double forceScale;      // m_c dx/dt^2
-    double momentumScale;   // m_c dx/dt (reactions at solid nodes -> wall momentum)
     double cellMass;
     int removal;
     int isStep;
@@ -84,14 +83,16 @@ struct ReplicaDesc {
     double dt;                   // ps
     uint32_t tileBegin;          // first work-group of this replica in the fused grid
     int particleBegin;           // first particle of this replica in the batch arrays
+    uint32_t particleTileBegin;  // first work-group of this replica in the coupling grid
 };
 
 /* Host-side launchers (defined in lbm_fluid.hip / lbm_particles.hip). */
 void launchFusedBatch(int threads, bool fluct, const ReplicaDesc* table, const uint16_t* tileReplica, uint32_t numTiles,
         const int64_t* stepCounter, hipStream_t stream);
-void launchCoupleBatch(int stencil, const ReplicaDesc* table, const uint16_t* particleReplica, int numParticles,
+void launchCoupleBatch(int stencil, const ReplicaDesc* table, const uint16_t* particleTiles, int numParticleTiles,
         const double* positions, const double* velocities, const double* mass, double* forces,
         const int64_t* stepCounter, hipStream_t stream);
+int coupleLanes(int stencil);
 void launchVerletBatch(const ReplicaDesc* table, const uint16_t* particleReplica, int numParticles, double* positions,
         double* velocities, const double* forces, const double* invMass, int64_t* stepCounter, hipStream_t stream);
 void launchFused(int storage, bool solids, bool reaction, bool fluct, const void* src, void* dst, const uint32_t* mask,
@@ -118,6 +119,17 @@ void launchCouple(int storage, const void* src, const uint32_t* mask, const doub
         const int* particles, const double* mass, const double* ucm, const ParticleArgs& a, double* particleForce,
         unsigned long long* reactionFixed, uint32_t* touched, double* partials, hipStream_t stream);
 void launchAddForces(const double* particleForce, const int* particles, int count, double* forces, hipStream_t stream);
+
+/* The centred drag (lbm_centred.hip): its buffers for numParticles coupled particles, and one coupling.  otherForces
+   (kJ/mol/nm, all System particles, may be null) are the forces on the particles before the coupling; g is the body
+   acceleration in lattice units.  Throws std::runtime_error. */
+struct CentredState;
+CentredState* centredCreate(int numParticles, int stencil, uint32_t numNodes);
+void centredDestroy(CentredState* state);
+void centredCouple(CentredState* state, int storage, const void* src, const uint32_t* mask, const double* positions,
+        const double* velocities, const double* otherForces, const int* particles, const double* mass,
+        const double* ucm, const ParticleArgs& a, const double g[3], double* particleForce,
+        unsigned long long* reactionFixed, uint32_t* touched, double* partials, hipStream_t stream);
 
 /* Nodes per axis of an interpolation stencil (LBMStencils.h:42-104). */
 __host__ __device__ inline int stencilWidth(int stencil) {

@@ -59,7 +59,13 @@ void launchMlpInfer(const MlpDevice* weights, const _Float16* X, int n, float ou
  */
 void launchClosureFeatures(int storage, const void* src, const uint32_t* mask, const double* positions,
         const double* velocities, const int* particles, const double* ucm, const ParticleArgs& a, const ClosureNorm& norm,
-        _Float16* X, float* raw, uint32_t* centreNode, double* centreVelocity, uint8_t* ood, hipStream_t stream);
+        const double* field, _Float16* X, float* raw, uint32_t* centreNode, double* centreVelocity, uint8_t* ood,
+        hipStream_t stream);
+/* The node velocities u = j/rho of the whole lattice, FP64 [node][3] (zero at solid nodes), for
+   launchClosureFeatures(field = ...) when 27 x particles exceed the nodes: one coalesced pass instead of 27 scattered
+   pulls per particle.  The values are those the per-particle path forms, bit for bit. */
+void launchClosureField(int storage, const void* src, const uint32_t* mask, const double* ucm, const ParticleArgs& a,
+        double* field, hipStream_t stream);
 void launchClosureTargets(int storage, const void* src, const uint32_t* mask, const double* positions,
         const int* particles, const double* ucm, const ParticleArgs& a, const double* centreVelocity, float* targets,
         hipStream_t stream);

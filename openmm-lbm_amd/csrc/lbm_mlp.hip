@@ -412,8 +412,8 @@ void launchMlpInfer(const MlpDevice* weights, const _Float16* X, int n, float ou
     if (n <= 0)
         return;
     int tiles = (n + 15)/16;
-    // Persistent waves: at most four per SIMD of the 256 compute units, one weight load each.
-    int waves = std::min(tiles, 4*1024);
+    // Persistent waves, one per SIMD of the 256 compute units: each loads the weights once.
+    int waves = std::min(tiles, 1024);
     mlpInfer<<<blocksFor(waves, WAVES), 64*WAVES, 0, stream>>>(weights, X, n, 1.0f/outScale, du);
 }
 

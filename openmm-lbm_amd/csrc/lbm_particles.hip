@@ -109,19 +109,20 @@ __global__ void __launch_bounds__(BLOCK) coupleParticles(const S* __restrict__ s
     }
 }
 
-/* Batched replicas: the coupling of every particle of every replica in one launch (particles of one replica are
-   contiguous; particleReplica gives the replica of each). */
+/* Batched replicas: the coupling of every particle of every replica in one launch.  Each work-group couples up to
+   BLOCK/LANES particles of one replica (particleTiles[blockIdx.x] names it), so the replica's descriptor is uniform
+   across the work-group and read with scalar loads. */
 template <int LANES>
 __global__ void __launch_bounds__(BLOCK) coupleBatch(const ReplicaDesc* __restrict__ table,
-        const uint16_t* __restrict__ particleReplica, int numParticles, const double* __restrict__ positions,
+        const uint16_t* __restrict__ particleTiles, const double* __restrict__ positions,
         const double* __restrict__ velocities, const double* __restrict__ mass, double* __restrict__ forces,
         const int64_t* __restrict__ stepCounter) {
     constexpr int GROUPS = BLOCK/LANES;
     int lane = threadIdx.x % LANES;
-    int p = blockIdx.x*GROUPS + threadIdx.x/LANES;
-    bool active = p < numParticles;
-    int pp = active ? p : 0;
-    const ReplicaDesc* d = table + particleReplica[pp];
+    const ReplicaDesc* d = table + particleTiles[blockIdx.x];
+    int local = (int) (blockIdx.x - d->particleTileBegin)*GROUPS + threadIdx.x/LANES;
+    bool active = local < d->part.numParticles;
+    int pp = d->particleBegin + (active ? local : 0);
     ParticleArgs a = d->part;
     a.step = *stepCounter;
     a.isStep = 1;
@@ -241,28 +242,24 @@ void launchCouple(int storage, const void* src, const uint32_t* mask, const doub
     }
 }
 
-void launchCoupleBatch(int stencil, const ReplicaDesc* table, const uint16_t* particleReplica, int numParticles,
+void launchCoupleBatch(int stencil, const ReplicaDesc* table, const uint16_t* particleTiles, int numParticleTiles,
         const double* positions, const double* velocities, const double* mass, double* forces,
         const int64_t* stepCounter, hipStream_t stream) {
-    if (numParticles == 0)
+    if (numParticleTiles == 0)
         return;
-    unsigned grid = (unsigned) coupleBlocks(stencil, numParticles);
+    unsigned grid = (unsigned) numParticleTiles;
     switch (coupleLanes(stencil)) {
     case 8:
-        coupleBatch<8><<<grid, BLOCK, 0, stream>>>(table, particleReplica, numParticles, positions, velocities, mass,
-                forces, stepCounter);
+        coupleBatch<8><<<grid, BLOCK, 0, stream>>>(table, particleTiles, positions, velocities, mass, forces, stepCounter);
         break;
     case 32:
-        coupleBatch<32><<<grid, BLOCK, 0, stream>>>(table, particleReplica, numParticles, positions, velocities, mass,
-                forces, stepCounter);
+        coupleBatch<32><<<grid, BLOCK, 0, stream>>>(table, particleTiles, positions, velocities, mass, forces, stepCounter);
         break;
     case 64:
-        coupleBatch<64><<<grid, BLOCK, 0, stream>>>(table, particleReplica, numParticles, positions, velocities, mass,
-                forces, stepCounter);
+        coupleBatch<64><<<grid, BLOCK, 0, stream>>>(table, particleTiles, positions, velocities, mass, forces, stepCounter);
         break;
     default:
-        coupleBatch<1><<<grid, BLOCK, 0, stream>>>(table, particleReplica, numParticles, positions, velocities, mass,
-                forces, stepCounter);
+        coupleBatch<1><<<grid, BLOCK, 0, stream>>>(table, particleTiles, positions, velocities, mass, forces, stepCounter);
         break;
     }
 }

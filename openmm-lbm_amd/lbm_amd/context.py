@@ -150,15 +150,21 @@ class Context:
         return mass.value, p
 
     def createCheckpoint(self):
+        """LBMForce::createCheckpoint: LBMForceImpl's header (LBMForceImpl.cpp:516-529), then the kernel's data."""
+        from .serialization import checkpoint_header
         size = ctypes.c_size_t()
         self._check(self.lib.lbm_checkpoint_size(self.handle, ctypes.byref(size)))
         buf = ctypes.create_string_buffer(size.value)
         self._check(self.lib.lbm_create_checkpoint(self.handle, buf, size.value))
-        return buf.raw
+        return checkpoint_header(self.force, len(self.coupled)) + buf.raw
 
     def loadCheckpoint(self, data):
-        buf = ctypes.create_string_buffer(bytes(data), len(data))
-        self._check(self.lib.lbm_load_checkpoint(self.handle, buf, len(data)))
+        from .serialization import read_checkpoint_header
+        data = bytes(data)
+        offset = read_checkpoint_header(data, self.force, len(self.coupled))
+        body = data[offset:]
+        buf = ctypes.create_string_buffer(body, len(body))
+        self._check(self.lib.lbm_load_checkpoint(self.handle, buf, len(body)))
 
     def updateParametersInContext(self):
         """LBMForceImpl::updateParametersInContext (LBMForceImpl.cpp:388-416)."""

@@ -28,6 +28,12 @@ class LBMForce:
         _lib.load_library().lbm_force_defaults(ctypes.byref(self._s))
         self._particles = []
         self._solid = []
+        self._group = 0
+        self._name = "LBMForce"
+        self._decomposition = [1, 1, 1]
+        self._copiesCheck = True
+        self._densityHalo = False
+        self._velocityHalo = False
 
     # Grid
     def getGridSize(self):
@@ -193,9 +199,53 @@ class LBMForce:
     def usesPeriodicBoundaryConditions(self):
         return True
 
+    # OpenMM Force settings (Force.h) and the MPI settings of the reference (LBMForce.h), kept so that a force
+    # serializes and deserializes like the reference's; this platform runs one domain, so any decomposition other
+    # than one domain is refused when a Context is created.
+    def getForceGroup(self):
+        return self._group
+
+    def setForceGroup(self, group):
+        if not 0 <= int(group) <= 31:
+            raise ValueError("Force group must be between 0 and 31")
+        self._group = int(group)
+
+    def getName(self):
+        return self._name
+
+    def setName(self, name):
+        self._name = str(name)
+
+    def getDomainDecomposition(self):
+        return tuple(self._decomposition)
+
+    def setDomainDecomposition(self, px, py, pz):
+        self._decomposition = [int(px), int(py), int(pz)]
+
+    def getParticleCopiesCheck(self):
+        return self._copiesCheck
+
+    def setParticleCopiesCheck(self, check):
+        self._copiesCheck = bool(check)
+
+    def getDensityHaloExchange(self):
+        return self._densityHalo
+
+    def setDensityHaloExchange(self, exchange):
+        self._densityHalo = bool(exchange)
+
+    def getVelocityHaloExchange(self):
+        return self._velocityHalo
+
+    def setVelocityHaloExchange(self, exchange):
+        self._velocityHalo = bool(exchange)
+
     def computeLatticeParameters(self, box, stepSize, masses):
         """LBMForceImpl::computeLatticeParameters (LBMForceImpl.cpp:44-214): the lattice parameters of this force
         for a rectangular box (three lengths, nm), an integrator step (ps) and the masses of all the particles."""
+        if any(p not in (0, 1) for p in self._decomposition):     # 0: chosen automatically, one domain here
+            raise _lib.LBMError(_lib.LBM_ERROR_UNSUPPORTED, "LBMForce: this platform runs the whole lattice in one "
+                                                            "domain on one GPU (setDomainDecomposition(1, 1, 1))")
         lib = _lib.load_library()
         out = _lib.LBMParams()
         boxArr = (ctypes.c_double * 3)(*[float(b) for b in box])

@@ -3,6 +3,8 @@
  config1: 32^3 periodic D3Q19, dx = 1 nm, dt = 0.01 ps, rho0 = 602.214 Da/nm^3, tau = 0.8; shear wave
           u_x = 0.01 sin(2 pi z/32); Guo body force 1e-5 (lattice) along x; 64 beads of 100 Da at seeded
           uniform positions, v = 0, gamma = 10/ps, trilinear; T = 0 (no noise); removal off; 1000 steps.
+ config4: 64 periodic replicas of uneven size, relaxation time, temperature, seed and bead count (batched).
+ config5: 128^3 periodic, 2,000 harmonic chains of 50 beads, fluctuating; the learned closure against Keys.
  config2: 256^3, solid planes z = 0 and z = 255 with bounce-back (walls at z = 0.5 and 254.5 nm), periodic in
           x and y; Guo body force 1e-6 (lattice) along x; tau = 0.6; dx = 1 nm, dt = 0.01 ps, rho0 = 602.214;
           20,000 beads of 100 Da, gamma = 10/ps, uniform in 1.5 <= z <= 253.5 nm, Keys stencil, repulsive walls
@@ -50,6 +52,7 @@ class Workload:
         self.configure = []
         self.state = None
         self.wall = None          # (eps, sigma, zlo, zhi)
+        self.bonds = None         # (pairs, k, r0)
         self.steps = 0
         self.description = {}
 
@@ -88,6 +91,8 @@ class Workload:
         dp = DeviceParticles(ctx)
         if self.wall is not None:
             dp.setWallPotential(*self.wall)
+        if self.bonds is not None:
+            dp.setBonds(*self.bonds)
         return dp
 
 
@@ -182,3 +187,41 @@ def config4(num_replicas=64, steps=1000, seed=4242, edges=(32, 48, 64, 96), max_
                    "replicas": num_replicas, "total_nodes": nodes, "total_beads": beads, "steps": steps,
                    "dx_nm": dx, "dt_ps": dt}
     return replicas, rows, description
+
+
+def config5(n=128, num_chains=2000, chain_length=50, seed=5150, temperature=300.0):
+    """128^3 periodic, tau = 0.7, dx = 1 nm, dt = 0.01 ps, rho0 = 602.214 Da/nm^3; 2,000 chains of 50 beads (100 Da,
+    gamma = 10/ps) bonded by harmonic springs k = 1000 kJ/mol/nm^2, r0 = 0.38 nm, started as seeded random walks of
+    step r0 from uniform random origins; fluctuating fluid and particle noise at 300 K; Keys stencil (the judge and
+    fallback of the closure); removal off."""
+    w = Workload("config5", (n, n, n), 0.7)
+    rng = np.random.default_rng(seed)
+    count = num_chains*chain_length
+    pos = np.zeros((count, 3))
+    bonds = []
+    r0 = 0.38
+    for c in range(num_chains):
+        start = c*chain_length
+        pos[start] = rng.uniform(0, n*w.dx, size=3)
+        steps = rng.normal(size=(chain_length - 1, 3))
+        steps *= r0/np.linalg.norm(steps, axis=1)[:, None]
+        pos[start + 1:start + chain_length] = pos[start] + np.cumsum(steps, axis=0)
+        bonds.extend((start + i, start + i + 1) for i in range(chain_length - 1))
+    w.positions = pos % (n*w.dx)
+    w.masses = [100.0]*count
+    w.velocities = np.zeros((count, 3))
+    w.bonds = (np.array(bonds, dtype=np.int32), 1000.0, r0)
+
+    def conf(f):
+        f.setFriction(10.0)
+        f.setTemperature(temperature)
+        f.setFluidFluctuations(True)
+        f.setRandomNumberSeed(seed)
+        f.setInterpolationStencil(LBMForce.Keys)
+    w.configure.append(conf)
+    w.steps = 1000
+    w.description = {"workload": f"config5 {n}^3 periodic, tau=0.7, {num_chains} harmonic chains x {chain_length} beads "
+                                 f"(k=1000, r0=0.38), fluctuating kT={BOLTZ*temperature:.3f}, Keys judge",
+                     "lattice": [n, n, n], "beads": count, "chains": num_chains, "chain_length": chain_length,
+                     "dx_nm": w.dx, "dt_ps": w.dt, "tau": 0.7}
+    return w

@@ -337,8 +337,6 @@ public:
         p = params;
         numNodes = p.nx*p.ny*p.nz;
         numSystemParticles = numSys;
-        if (p.drag_scheme != LBM_DRAG_EXPLICIT)
-            throw OracleError(LBM_ERROR_UNSUPPORTED, "LBMForce: the Centered drag scheme is not implemented by this build");
         seed = (uint32_t) p.random_number_seed;
         if (seed == 0)
             seed = (uint32_t) std::chrono::high_resolution_clock::now().time_since_epoch().count() | 1u;
@@ -500,23 +498,23 @@ public:
         bool repeated = stepForcesCurrent && stepCount == stepIndex - 1;
         if (stepPending) {
             stepPending = false;
-            advanceFluid(positions, velocities);
+            advanceFluid(positions, velocities, forces);
         }
         else if (includeForces && !particles.empty() && stepCount > 0 && !repeated)
-            nextStepForces(stepCount, positions, velocities);
+            nextStepForces(stepCount, positions, velocities, forces);
         if (includeForces)
             for (size_t i = 0; i < particles.size(); i++)
                 for (int k = 0; k < 3; k++)
                     forces[3*(size_t) particles[i] + k] += particleForces[i][k];
     }
 
-    void advanceFluid(const double* positions, const double* velocities) {
+    void advanceFluid(const double* positions, const double* velocities, const double* forces) {
         // ReferenceLBMKernels.cpp:367-389.
         computeMoments();
         if (p.momentum_removal_frequency > 0 && stepIndex % p.momentum_removal_frequency == 0)
             removeMomentum();
         if (!particles.empty())
-            couple(positions, velocities, true, stepIndex);
+            couple(positions, velocities, forces, true, stepIndex);
         collideAndStream();
         if (!wallNodes.empty()) {
             bounceBack();
@@ -537,13 +535,13 @@ public:
         }
     }
 
-    void nextStepForces(int64_t stepCount, const double* positions, const double* velocities) {
+    void nextStepForces(int64_t stepCount, const double* positions, const double* velocities, const double* forces) {
         // ReferenceLBMKernels.cpp:391-401.
         stepForcesCurrent = false;
         computeMoments();
         if (p.momentum_removal_frequency > 0 && stepCount % p.momentum_removal_frequency == 0)
             removeMomentum();
-        couple(positions, velocities, false, stepCount);
+        couple(positions, velocities, forces, false, stepCount);
     }
 
     void computeMoments() {
@@ -656,7 +654,19 @@ public:
         return g;
     }
 
-    void couple(const double* positions, const double* velocities, bool isStep, int64_t noiseStep) {
+    void couple(const double* positions, const double* velocities, const double* forces, bool isStep,
+            int64_t noiseStep) {
+        if (p.drag_scheme == LBM_DRAG_CENTERED) {
+            if (isStep)
+                std::fill(reaction.begin(), reaction.end(), 0.0);
+            if (p.interpolation_stencil == LBM_STENCIL_NEAREST)
+                coupleCentered(positions, velocities, forces, isStep, noiseStep);
+            else
+                coupleCenteredStencil(positions, velocities, forces, isStep, noiseStep);
+            if (isStep)
+                applyReaction();
+            return;
+        }
         // Explicit drag (ReferenceLBMKernels.cpp:503-553 nearest, 643-697 stencils).  The random numbers of a step
         // are keyed by the step index, so every evaluation of the same step sees the same ones.
         double gamma = p.friction*p.dt;
@@ -701,14 +711,211 @@ public:
                     for (int k = 0; k < 3; k++)
                         reaction[3*(size_t) nodes[n] + k] -= weights[n]*f[k];
         }
-        if (isStep) {
-            // applyReaction (ReferenceLBMKernels.cpp:959-971).
-            for (size_t k = 0; k < reaction.size(); k++)
-                force[k] += reaction[k];
-            double scale = cellMass()*p.dx/p.dt;
-            for (int node : solidNodes)
+        if (isStep)
+            applyReaction();
+    }
+
+    void applyReaction() {
+        // ReferenceLBMKernels.cpp:959-971.
+        for (size_t k = 0; k < reaction.size(); k++)
+            force[k] += reaction[k];
+        double scale = cellMass()*p.dx/p.dt;
+        for (int node : solidNodes)
+            for (int k = 0; k < 3; k++)
+                wallMomentum[k] += reaction[3*(size_t) node + k]*scale;
+    }
+
+    /* The particle's velocity at the time of the force before its drag, v~ = v(t - h) + h Fc/m (lattice units), and its
+       random force, as both centred schemes use them. */
+    void centredKnowns(const double* velocities, const double* forces, int64_t noiseStep, std::vector<Vec3d>& known,
+            std::vector<Vec3d>& random) const {
+        double gamma = p.friction*p.dt;
+        double kT = p.kT*p.dt*p.dt/(cellMass()*p.dx*p.dx);
+        double velocityScale = p.dx/p.dt;
+        double forceScale = cellMass()*p.dx/(p.dt*p.dt);
+        double h = 0.5;
+        bool randomForce = kT > 0 && gamma > 0;
+        size_t n = particles.size();
+        known.assign(n, Vec3d());
+        random.assign(n, Vec3d());
+        for (size_t i = 0; i < n; i++) {
+            int part = particles[i];
+            double m = particleMass[i];
+            for (int k = 0; k < 3; k++) {
+                double fc = forces ? forces[3*(size_t) part + k] : 0.0;
+                known[i][k] = velocities[3*(size_t) part + k]*(1.0/velocityScale) + fc*(h/(m*forceScale));
+            }
+            if (randomForce) {
+                double noise[3];
+                drawNormals(seed, TAG_PARTICLE, (uint32_t) i, noiseStep, 3, noise);
                 for (int k = 0; k < 3; k++)
-                    wallMomentum[k] += reaction[3*(size_t) node + k]*scale;
+                    random[i][k] = noise[k]*std::sqrt(2.0*gamma*m*kT);
+            }
+        }
+    }
+
+    void coupleCentered(const double* positions, const double* velocities, const double* forces, bool isStep,
+            int64_t noiseStep) {
+        // Centred drag at the nearest node (ReferenceLBMKernels.cpp:555-641): with a = gamma h, h = 1/2, for the
+        // particles of one node (sums M, P~ = sum m v~, R in particle order) and u~ = (j + h Fbody)/rho,
+        //   S = [-gamma (P~ - M u~) + R]/(1 + a + a M/rho),  u_c = u~ - h S/rho,
+        //   F_k = [-gamma m_k (v~_k - u_c) + R_k]/(1 + a),
+        // the node receiving -S; at a solid node u_c = 0 and S = [-gamma P~ + R]/(1 + a).
+        double gamma = p.friction*p.dt, h = 0.5, a = gamma*h;
+        double forceScale = cellMass()*p.dx/(p.dt*p.dt);
+        std::vector<Vec3d> known, random;
+        centredKnowns(velocities, forces, noiseStep, known, random);
+        size_t n = particles.size();
+        std::vector<long long> keys(n);
+        for (size_t i = 0; i < n; i++)
+            keys[i] = (long long) nearestNode(positions + 3*(size_t) particles[i])*(long long) n + (long long) i;
+        std::sort(keys.begin(), keys.end());
+        for (size_t first = 0; first < n; ) {
+            int node = (int) (keys[first]/(long long) n);
+            size_t end = first;
+            double mass = 0;
+            Vec3d pm, rsum;
+            for (; end < n && keys[end]/(long long) n == node; end++) {
+                size_t i = (size_t) (keys[end] % (long long) n);
+                mass += particleMass[i];
+                for (int k = 0; k < 3; k++) {
+                    pm[k] += known[i][k]*particleMass[i];
+                    rsum[k] += random[i][k];
+                }
+            }
+            Vec3d u, sum;
+            if (rho[node] > 0) {
+                for (int k = 0; k < 3; k++) {
+                    double knownFluid = (mom[3*(size_t) node + k] + h*force[3*(size_t) node + k])*(1.0/rho[node]);
+                    sum[k] = ((pm[k] - knownFluid*mass)*(-gamma) + rsum[k])*(1.0/(1.0 + a + a*mass/rho[node]));
+                    u[k] = knownFluid - sum[k]*(h/rho[node]);
+                }
+            }
+            else
+                for (int k = 0; k < 3; k++)
+                    sum[k] = ((pm[k] - u[k]*mass)*(-gamma) + rsum[k])*(1.0/(1.0 + a));
+            for (size_t e = first; e < end; e++) {
+                size_t i = (size_t) (keys[e] % (long long) n);
+                for (int k = 0; k < 3; k++)
+                    particleForces[i][k] = ((known[i][k] - u[k])*(-gamma*particleMass[i]) + random[i][k])*(1.0/(1.0 + a))*
+                            forceScale;
+            }
+            if (isStep)
+                for (int k = 0; k < 3; k++)
+                    reaction[3*(size_t) node + k] = -sum[k];
+            first = end;
+        }
+    }
+
+    void coupleCenteredStencil(const double* positions, const double* velocities, const double* forces, bool isStep,
+            int64_t noiseStep) {
+        // Centred drag with a stencil (ReferenceLBMKernels.cpp:726-901): (1 + a)/m_k F_k + a sum_l K_kl F_l = b_k/m_k,
+        // K_kl = sum_j xi_jk xi_jl/rho_j, b_k = -gamma m_k (v~_k - sum_j xi_jk u~_j) + R_k, solved for each component by
+        // conjugate gradients preconditioned with the diagonal, from the diagonal solution, until the residual is
+        // 1e-13 of the right-hand side; the products spread on the nodes and interpolate back.
+        double gamma = p.friction*p.dt, h = 0.5, a = gamma*h;
+        double forceScale = cellMass()*p.dx/(p.dt*p.dt);
+        std::vector<Vec3d> known, random;
+        centredKnowns(velocities, forces, noiseStep, known, random);
+        size_t n = particles.size();
+        std::vector<std::vector<int>> nodes(n);
+        std::vector<std::vector<double>> weights(n);
+        std::vector<Vec3d> rhs(n);
+        std::vector<double> diagonal(n, 0.0);
+        for (size_t i = 0; i < n; i++) {
+            stencilNodes(positions + 3*(size_t) particles[i], nodes[i], weights[i]);
+            double m = particleMass[i];
+            Vec3d u;
+            double self = 0;
+            for (size_t e = 0; e < nodes[i].size(); e++) {
+                int node = nodes[i][e];
+                if (rho[node] > 0) {
+                    for (int k = 0; k < 3; k++)
+                        u[k] += (mom[3*(size_t) node + k] + h*force[3*(size_t) node + k])*(weights[i][e]/rho[node]);
+                    self += weights[i][e]*weights[i][e]/rho[node];
+                }
+            }
+            for (int k = 0; k < 3; k++)
+                rhs[i][k] = ((known[i][k] - u[k])*(-gamma*m) + random[i][k])*(1.0/m);
+            diagonal[i] = (1.0 + a)/m + a*self;
+        }
+        std::vector<double> spread(3*(size_t) numNodes, 0.0);
+        auto multiply = [&](const std::vector<Vec3d>& x, std::vector<Vec3d>& result) {
+            for (size_t i = 0; i < n; i++)
+                for (size_t e = 0; e < nodes[i].size(); e++)
+                    for (int k = 0; k < 3; k++)
+                        spread[3*(size_t) nodes[i][e] + k] += weights[i][e]*x[i][k];
+            for (size_t i = 0; i < n; i++) {
+                Vec3d sum;
+                for (size_t e = 0; e < nodes[i].size(); e++) {
+                    int node = nodes[i][e];
+                    if (rho[node] > 0)
+                        for (int k = 0; k < 3; k++)
+                            sum[k] += spread[3*(size_t) node + k]*(weights[i][e]/rho[node]);
+                }
+                for (int k = 0; k < 3; k++)
+                    result[i][k] = x[i][k]*((1.0 + a)/particleMass[i]) + sum[k]*a;
+            }
+            for (size_t i = 0; i < n; i++)
+                for (size_t e = 0; e < nodes[i].size(); e++)
+                    for (int k = 0; k < 3; k++)
+                        spread[3*(size_t) nodes[i][e] + k] = 0.0;
+        };
+        std::vector<Vec3d> F(n), r(n), z(n), d(n), Ad(n);
+        Vec3d rz, norm;
+        for (size_t i = 0; i < n; i++)
+            for (int k = 0; k < 3; k++)
+                F[i][k] = rhs[i][k]*(1.0/diagonal[i]);
+        multiply(F, Ad);
+        for (size_t i = 0; i < n; i++)
+            for (int k = 0; k < 3; k++) {
+                r[i][k] = rhs[i][k] - Ad[i][k];
+                z[i][k] = r[i][k]*(1.0/diagonal[i]);
+                d[i][k] = z[i][k];
+                rz[k] += r[i][k]*z[i][k];
+                norm[k] += rhs[i][k]*rhs[i][k];
+            }
+        const double tolerance = 1e-13;
+        for (int iteration = 0; ; iteration++) {
+            Vec3d residual;
+            for (size_t i = 0; i < n; i++)
+                for (int k = 0; k < 3; k++)
+                    residual[k] += r[i][k]*r[i][k];
+            bool converged = true;
+            for (int k = 0; k < 3; k++)
+                if (residual[k] > tolerance*tolerance*norm[k])
+                    converged = false;
+            if (converged)
+                break;
+            if (iteration == 1000)
+                throw OracleError(LBM_ERROR_RUNTIME, "LBMForce: the centred drag with the interpolation stencil did not "
+                        "converge in 1000 iterations of the conjugate gradients");
+            multiply(d, Ad);
+            Vec3d dAd, rzNew, alpha;
+            for (size_t i = 0; i < n; i++)
+                for (int k = 0; k < 3; k++)
+                    dAd[k] += d[i][k]*Ad[i][k];
+            for (int k = 0; k < 3; k++)
+                alpha[k] = dAd[k] > 0 ? rz[k]/dAd[k] : 0.0;
+            for (size_t i = 0; i < n; i++)
+                for (int k = 0; k < 3; k++) {
+                    F[i][k] += alpha[k]*d[i][k];
+                    r[i][k] -= alpha[k]*Ad[i][k];
+                    z[i][k] = r[i][k]*(1.0/diagonal[i]);
+                    rzNew[k] += r[i][k]*z[i][k];
+                }
+            for (size_t i = 0; i < n; i++)
+                for (int k = 0; k < 3; k++)
+                    d[i][k] = z[i][k] + (rz[k] > 0 ? rzNew[k]/rz[k] : 0.0)*d[i][k];
+            rz = rzNew;
+        }
+        for (size_t i = 0; i < n; i++) {
+            for (int k = 0; k < 3; k++)
+                particleForces[i][k] = F[i][k]*forceScale;
+            if (isStep)
+                for (size_t e = 0; e < nodes[i].size(); e++)
+                    for (int k = 0; k < 3; k++)
+                        reaction[3*(size_t) nodes[i][e] + k] -= weights[i][e]*F[i][k];
         }
     }
 
@@ -932,7 +1139,7 @@ public:
 
 struct CheckpointHeader {
     char magic[8];
-    int32_t version, nx, ny, nz, numParticles, pad;
+    int32_t version, nx, ny, nz, numParticles, dragScheme;
 };
 
 int fail(const OracleError& e) {
@@ -1054,7 +1261,7 @@ int lbm_create_checkpoint(lbm_context* ctx, void* buffer, size_t size) {
             throw OracleError(LBM_ERROR_INVALID, "lbm_create_checkpoint: buffer too small");
         CheckpointHeader h;
         std::memcpy(h.magic, "LBMAMDC1", 8);
-        h.version = 1; h.nx = k.p.nx; h.ny = k.p.ny; h.nz = k.p.nz; h.numParticles = (int) k.particles.size(); h.pad = 0;
+        h.version = 1; h.nx = k.p.nx; h.ny = k.p.ny; h.nz = k.p.nz; h.numParticles = (int) k.particles.size(); h.dragScheme = k.p.drag_scheme;
         char* out = (char*) buffer;
         std::memcpy(out, &h, sizeof(h));
         std::memcpy(out + sizeof(h), k.pop.data(), k.pop.size()*sizeof(double));
@@ -1074,6 +1281,8 @@ int lbm_load_checkpoint(lbm_context* ctx, const void* buffer, size_t size) {
             throw OracleError(LBM_ERROR_INVALID, "LBMForce: the data is not a checkpoint of this LBMForce platform");
         if (h.nx != k.p.nx || h.ny != k.p.ny || h.nz != k.p.nz || h.numParticles != (int) k.particles.size())
             throw OracleError(LBM_ERROR_INVALID, "LBMForce: the checkpoint was written for another lattice or System");
+        if (h.dragScheme != k.p.drag_scheme)
+            throw OracleError(LBM_ERROR_INVALID, "LBMForce: the checkpoint was written with a different drag scheme");
         if (size < need)
             throw OracleError(LBM_ERROR_INVALID, "LBMForce: the checkpoint is truncated");
         const char* in = (const char*) buffer;

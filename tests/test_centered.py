@@ -1,0 +1,227 @@
+"""The centred drag (LBMForce::Centered, ReferenceLBMKernels.cpp:555-641 nearest node, 726-901 stencils), on the FP64
+oracle and on the gfx950 library, against the closed forms of the reference's tests (TestLBMCentered.h) and against a
+direct solution of the linear system:
+
+* first step: a particle in a fluid at rest has v1 = v0 (1 - a + b)/(1 + a + b), a = gamma dt/2, b = a m/m_c, and the
+  force of a between-step evaluation is the next step's (TestLBMCentered.h:70-98);
+* several particles at one node, other forces, uniform flow and body force: the velocities after one step equal the
+  Gaussian-elimination solution of (1 + a) F_k + (a m_k/m_c) sum_l F_l = -gamma m_k (v~_k - u~) per component, and the
+  fluid gains g - sum F (TestLBMCentered.h:100-180);
+* a particle at a solid node: v1 = v0 (1 - a)/(1 + a), the wall receives the opposite (TestLBMCentered.h:182-204);
+* gamma dt = 3, four particles at one node: stable, total momentum conserved (TestLBMCentered.h:206-233);
+* stencils: the velocities after one step equal the dense solution of (1 + a)/m_k F_k + a sum_l K_kl F_l = b_k/m_k,
+  K_kl = sum_j xi_jk xi_jl/rho_j, for particles whose stencils overlap, and an isolated particle's closed form.
+The GPU is also compared with the oracle over many steps with fluctuations.
+"""
+import numpy as np
+import pytest
+
+import oracle_lib
+from lbm_amd import LBMError, LBMForce
+from parity import Setup, lockstep, rel_err
+
+BACKENDS = [pytest.param("oracle", id="oracle"), pytest.param("gpu", id="gpu", marks=pytest.mark.gpu)]
+DX, DT, RHO = 0.5, 0.01, 602.2
+CELL = RHO*DX**3
+
+
+def library(backend):
+    return oracle_lib.load_oracle() if backend == "oracle" else None
+
+
+def tolerance(backend, fp64, fp32):
+    return fp64 if backend == "oracle" else fp32
+
+
+def centred_setup(masses, positions, velocities, friction, stencil=LBMForce.NearestNode, u0=(0, 0, 0), g=(0, 0, 0),
+                  external=None, solid=None, n=8, temperature=0.0, fluct=False, seed=1):
+    s = Setup((n, n, n), dx=DX, dt=DT, tau=0.8, density=RHO)
+    for m, x, v in zip(masses, positions, velocities):
+        s.add_particles([x], mass=m, velocities=[v])
+    if external is not None:
+        ext = np.asarray(external, dtype=np.float64)
+        s.extra.append(lambda pos: ext.copy())
+
+    def conf(f):
+        f.setFriction(friction)
+        f.setTemperature(temperature)
+        f.setFluidFluctuations(fluct)
+        f.setRandomNumberSeed(seed)
+        f.setDragScheme(LBMForce.Centered)
+        f.setInterpolationStencil(stencil)
+        f.setInitialFluidVelocity(tuple(np.asarray(u0, dtype=np.float64)))
+        f.setBodyAcceleration(tuple(np.asarray(g, dtype=np.float64)))
+        if solid is not None:
+            f.setSolidNodes(solid)
+    s.configure.append(conf)
+    return s
+
+
+def fluid_momentum(ctx):
+    mass, j = ctx.getFluidTotals()
+    return j*(CELL*DX/DT)
+
+
+@pytest.mark.parametrize("backend", BACKENDS)
+def test_first_step(backend):
+    friction = 50.0
+    v0 = np.array([0.3, -0.2, 0.1])
+    ctx = centred_setup([100.0], [[1.1, 1.3, 1.7]], [v0], friction).build(library(backend))
+    assert np.all(ctx.getForces() == 0)
+    ctx.step(1)
+    v1 = ctx.velocities[0].copy()
+    f_next = ctx.getForces()[0]
+    a = 0.5*friction*DT
+    b = a*100.0/CELL
+    assert np.abs(v1 - v0*(1 - a + b)/(1 + a + b)).max() <= tolerance(backend, 1e-14, 5e-14)
+    ctx.step(1)
+    v2 = ctx.velocities[0]
+    assert np.abs((v2 - v1)*(100.0/DT) - f_next).max() <= 1e-12*np.abs(f_next).max()
+
+
+def solve_node(masses, known, known_fluid, friction):
+    """Direct solution, per component, of (1 + a) F_k + (a m_k/m_c) sum_l F_l = -gamma m_k (v~_k - u~)."""
+    a = 0.5*friction*DT
+    m = np.asarray(masses)
+    A = (1 + a)*np.eye(len(m)) + (a*m/CELL)[:, None]*np.ones((1, len(m)))
+    rhs = -friction*m[:, None]*(np.asarray(known) - np.asarray(known_fluid)[None, :])
+    return np.linalg.solve(A, rhs)
+
+
+@pytest.mark.parametrize("backend", BACKENDS)
+def test_shared_node(backend):
+    masses = [80.0, 120.0, 150.0, 100.0]
+    external = np.array([[100, -50, 30], [-20, 40, 0], [0, 0, -80], [60, 10, -10]], dtype=np.float64)
+    u0 = np.array([0.4, -0.3, 0.2])
+    g = np.array([20.0, 10.0, -30.0])
+    pos = [[1.02, 0.98, 1.05], [0.95, 1.1, 0.97], [1.1, 1.04, 0.92], [3.1, 2.6, 0.4]]
+    v0 = np.array([[0.5, -0.2, 0.3], [-0.4, 0.1, 0.2], [0.2, 0.6, -0.1], [-0.3, 0.2, 0.4]])
+    friction = 30.0
+    ctx = centred_setup(masses, pos, v0, friction, u0=u0, g=g, external=external).build(library(backend))
+    p0 = fluid_momentum(ctx)
+    ctx.step(1)
+    v1 = ctx.velocities
+    known = v0 + external*(0.5*DT/np.array(masses)[:, None])
+    known_fluid = u0 + g*(0.5*DT)
+    drag = np.concatenate([solve_node(masses[:3], known[:3], known_fluid, friction),
+                           solve_node(masses[3:], known[3:], known_fluid, friction)])
+    expected = v0 + (external + drag)*(DT/np.array(masses)[:, None])
+    # FP32 storage rounds the fluid's u~ by ~1e-7 of itself (the reference allows 2e-6 on single-precision platforms).
+    assert np.abs(v1 - expected).max() <= tolerance(backend, 1e-12, 1e-8)*np.abs(expected).max()
+    p1 = fluid_momentum(ctx)
+    exp_p = p0 + g*(512*CELL*DT) - drag.sum(axis=0)*DT
+    assert np.abs(p1 - exp_p).max() <= tolerance(backend, 1e-11, 1e-6)*np.abs(exp_p).max()
+
+
+@pytest.mark.parametrize("backend", BACKENDS)
+def test_particle_at_a_solid_node(backend):
+    friction = 40.0
+    n = 8
+    plane = [i + n*n*k for k in range(n) for i in range(n)]          # the plane j = 0
+    v0 = np.array([0.3, 5.0, 0.1])
+    ctx = centred_setup([100.0], [[1.1, 0.20, 1.7]], [v0], friction, solid=plane).build(library(backend))
+    ctx.step(1)
+    v1 = ctx.velocities[0]
+    a = 0.5*friction*DT
+    assert np.abs(v1 - v0*(1 - a)/(1 + a)).max() <= tolerance(backend, 1e-14, 5e-14)
+    assert np.abs(ctx.getWallForce() - (v0 - v1)*(100.0/DT)).max() <= 1e-12*np.abs(v0*100.0/DT).max()
+
+
+@pytest.mark.parametrize("backend", BACKENDS)
+def test_large_friction_is_stable(backend):
+    friction = 300.0
+    pos = [[1.02, 0.98, 1.05], [0.95, 1.1, 0.97], [1.1, 1.04, 0.92], [0.98, 1.03, 1.01]]
+    v0 = np.array([[1.0, -0.5, 0.3], [-0.8, 0.4, 0.6], [0.2, 1.0, -0.9], [0.5, -0.7, 0.1]])
+    ctx = centred_setup([100.0]*4, pos, v0, friction).build(library(backend))
+    scale = 100.0*np.linalg.norm(v0, axis=1).sum()
+
+    def total():
+        # Particles at the half step of the leapfrog plus the fluid.
+        return 100.0*ctx.velocities.sum(axis=0) + fluid_momentum(ctx)
+    p0 = total()
+    ctx.step(200)
+    assert np.all(np.linalg.norm(ctx.velocities, axis=1) < 0.1)
+    assert np.linalg.norm(total() - p0)/scale <= tolerance(backend, 1e-11, 1e-6)
+
+
+def dense_stencil_solution(backend, stencil, positions, v0, masses, friction, u0, g):
+    """The one-step velocities from the dense linear system of the centred drag with a stencil, built from the
+    oracle library's own stencil weights (oracle_axis_weights) and the fluid at rest with uniform u0 and g."""
+    lib = oracle_lib.load_oracle()
+    import ctypes
+    n = 8
+    a = 0.5*friction*DT
+    num = len(masses)
+    xi = np.zeros((num, n**3))
+    for i, x in enumerate(positions):
+        w_axes = []
+        for k in range(3):
+            s = x[k]/DX
+            s -= np.floor(s/n)*n
+            idx = (ctypes.c_int*4)()
+            wt = (ctypes.c_double*4)()
+            lib.oracle_axis_weights(stencil, s, n, 0, idx, wt)
+            w_axes.append((list(idx), list(wt)))
+        width = {LBMForce.Trilinear: 2, LBMForce.ThreePoint: 3, LBMForce.Keys: 4}[stencil]
+        for c in range(width):
+            for b in range(width):
+                for aa in range(width):
+                    node = w_axes[0][0][aa] + n*(w_axes[1][0][b] + n*w_axes[2][0][c])
+                    xi[i, node] += w_axes[0][1][aa]*w_axes[1][1][b]*w_axes[2][1][c]
+    m = np.asarray(masses)/CELL
+    K = xi @ xi.T                                           # rho = 1 everywhere
+    A = np.diag((1 + a)/m) + a*K
+    u_known = (np.asarray(u0) + np.asarray(g)*0.5*DT)*(DT/DX)
+    v_known = np.asarray(v0)*(DT/DX)
+    gamma = friction*DT
+    b = -gamma*(v_known - xi.sum(axis=1)[:, None]*u_known[None, :])
+    F = np.linalg.solve(A, b)                               # lattice units, per component
+    force = F*(CELL*DX/DT**2)
+    return v0 + force*(DT/np.asarray(masses)[:, None])
+
+
+@pytest.mark.parametrize("backend", BACKENDS)
+@pytest.mark.parametrize("stencil", [LBMForce.Trilinear, LBMForce.ThreePoint, LBMForce.Keys])
+def test_stencil_against_dense_solution(backend, stencil):
+    rng = np.random.default_rng(int(stencil))
+    masses = [100.0, 60.0, 150.0, 90.0, 120.0]
+    # Three particles with overlapping stencils, two apart.
+    pos = np.array([[1.1, 1.2, 1.3], [1.4, 1.0, 1.35], [0.9, 1.5, 1.1], [3.1, 2.7, 3.3], [2.2, 3.6, 0.6]])
+    v0 = rng.normal(scale=0.3, size=(5, 3))
+    friction = 80.0
+    u0 = np.array([0.2, -0.1, 0.15])
+    g = np.array([5.0, -3.0, 2.0])
+    ctx = centred_setup(masses, pos, v0, friction, stencil=stencil, u0=u0, g=g).build(library(backend))
+    ctx.step(1)
+    expected = dense_stencil_solution(backend, stencil, pos, v0, masses, friction, u0, g)
+    err = np.abs(ctx.velocities - expected).max()/np.abs(expected).max()
+    print("relative error against the dense solution", err)
+    assert err <= tolerance(backend, 1e-12, 1e-8)
+
+
+@pytest.mark.parametrize("stencil", [LBMForce.NearestNode, LBMForce.Keys])
+def test_checkpoint_refused_by_the_other_drag(stencil):
+    oracle = oracle_lib.load_oracle()
+    s = centred_setup([100.0], [[1.1, 1.3, 1.7]], [[0.1, 0.0, 0.0]], 10.0, stencil=stencil)
+    data = s.build(oracle).createCheckpoint()
+    explicit = Setup((8, 8, 8), dx=DX, dt=DT, tau=0.8, density=RHO)
+    explicit.add_particles([[1.1, 1.3, 1.7]], mass=100.0)
+    explicit.configure.append(lambda f: (f.setFriction(10.0), f.setInterpolationStencil(stencil)))
+    with pytest.raises(LBMError, match="drag"):
+        explicit.build(oracle).loadCheckpoint(data)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("stencil", [LBMForce.NearestNode, LBMForce.Keys])
+def test_centred_parity_with_fluctuations(stencil):
+    """The GPU against the oracle, 200 steps, dense particles sharing nodes and stencils, fluctuating fluid and particle
+    noise on the same Philox streams: fields, forces and positions to 1e-5 (the FP32 criterion)."""
+    rng = np.random.default_rng(31)
+    n = 10
+    pos = rng.uniform(0, n*DX, size=(150, 3))
+    v0 = rng.normal(scale=0.3, size=(150, 3))
+    s = centred_setup([100.0]*150, pos, v0, 20.0, stencil=stencil, n=n, temperature=300.0, fluct=True, seed=9)
+    worst, _, _ = lockstep(s, 200, check_every=10)
+    print("worst relative errors", worst)
+    for key in ("density", "velocity", "force"):
+        assert worst[key] < 1e-4, (key, worst)

@@ -116,10 +116,11 @@ def test_gradients_match_the_oracle():
     assert np.all(g[pad] == 0)
 
 
-def run_pair(n, rng, steps, net=None, stencil_b=LBMForce.NearestNode):
+def run_pair(n, rng, steps, net=None, stencil_b=LBMForce.NearestNode, count=80):
     """Two contexts from one setup with fluctuations and particle noise: A couples through the closure (Keys stencil),
-    B with the stencil stencil_b; both stepped in lockstep."""
-    pos, vel = beads(rng, n, 80)
+    B with the stencil stencil_b; both stepped in lockstep.  With 27 count >= n^3 the closure reads the node velocities
+    from a lattice-wide pass, below it pulls them per particle."""
+    pos, vel = beads(rng, n, count)
     state = random_state(n, rng, amplitude=0.01)
     a = bead_setup(n, state, pos, vel, LBMForce.Keys, temperature=300.0, fluct=True).build()
     b = bead_setup(n, state, pos, vel, stencil_b, temperature=300.0, fluct=True).build()
@@ -130,17 +131,18 @@ def run_pair(n, rng, steps, net=None, stencil_b=LBMForce.NearestNode):
     return a, b
 
 
-def test_zero_network_is_the_nearest_node_coupling():
+@pytest.mark.parametrize("count", [80, 20])
+def test_zero_network_is_the_nearest_node_coupling(count):
     rng = np.random.default_rng(8)
     net = MLPClosure(seed=1)
     net.setParameters(np.zeros(NUM_PARAMS, dtype=np.float32))
     net.setNormalization(u_scale=1.0, v_scale=1.0, out_scale=1.0, u_bound=1e30, v_bound=1e30)
-    a, b = run_pair(12, rng, 15, net, LBMForce.NearestNode)
+    a, b = run_pair(12, rng, 15, net, LBMForce.NearestNode, count=count)
     ra, ua = a.getFluidFields()
     rb, ub = b.getFluidFields()
     assert np.array_equal(ra, rb) and np.array_equal(ua, ub)
     assert np.array_equal(a.positions, b.positions) and np.array_equal(a.velocities, b.velocities)
-    assert stats(a) == (80*15, 0)
+    assert stats(a) == (count*15, 0)
 
 
 def test_out_of_distribution_particles_take_the_keys_coupling():

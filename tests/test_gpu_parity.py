@@ -181,6 +181,34 @@ def test_fluctuating_walls_and_particles_parity():
     assert worst["velocity"] < 1e-4 and worst["density"] < 1e-5 and worst["force"] < 1e-4, worst
 
 
+@pytest.mark.parametrize("stencil", [LBMForce.Trilinear, LBMForce.Keys])
+def test_wall_force_with_particles_at_solid_nodes(stencil):
+    """Particles next to bounce-back walls, whose stencils reach solid nodes: the reaction on the solid slots leaves
+    the fluid and goes to the wall (ReferenceLBMKernels.cpp:959-971), so the wall force of every step includes it."""
+    nx, ny, nz = 8, 8, 10
+    s = Setup((nx, ny, nz), dx=1.0, dt=0.01, tau=0.8)
+    solid = [i + nx*(j + ny*k) for k in (0, nz - 1) for j in range(ny) for i in range(nx)]
+    rng = np.random.default_rng(17)
+    pos = rng.uniform([0, 0, 0.6], [8, 8, 2.0], size=(12, 3))
+    pos[6:, 2] = rng.uniform(7.0, 8.4, size=6)
+    s.add_particles(pos, mass=100.0, velocities=rng.normal(scale=0.5, size=(12, 3)))
+
+    def conf(f):
+        f.setSolidNodes(solid)
+        f.setWallScheme(LBMForce.BounceBack)
+        f.setFriction(10.0)
+        f.setTemperature(0.0)
+        f.setInterpolationStencil(stencil)
+        f.setBodyAcceleration((0.5, 0, 0))
+    s.configure.append(conf)
+
+    def wall(step, ref, gpu):
+        w0, w1 = ref.getWallForce(), gpu.getWallForce()
+        assert rel_err(w1, w0) < 1e-5, (step, w0, w1)
+    worst, ref, gpu = lockstep(s, 20, check_every=1, callback=wall)
+    assert worst["velocity"] < TOL32 and worst["force"] < TOL32, worst
+
+
 def test_fp64_storage_parity():
     s = config1_setup(n=12, num_beads=10, stencil=LBMForce.Keys)
     worst, ref, gpu = lockstep(s, 30, check_every=1, storage=LBMForce.FP64)
