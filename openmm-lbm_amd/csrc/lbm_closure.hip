@@ -35,10 +35,23 @@ __global__ void __launch_bounds__(BLOCK) closureFeatures(const S* __restrict__ s
     int i = blockIdx.x*GROUPS + threadIdx.x/FEATURE_LANES;
     bool active = i < a.numParticles;
     bool out = false;
+    int part = active ? particles[i] : 0;
+    const double* x = positions + 3*(size_t) part;
+    int size[3] = {(int) a.g.nx, (int) a.g.ny, (int) a.g.nz};
+    // The wrapped coordinate and the nearest node (FP64, as the coupling computes them) by the group's first lane,
+    // shared with the others.
+    double sw[3] = {0, 0, 0};
+    int near[3] = {0, 0, 0};
+    if (active && lane == 0)
+        for (int k = 0; k < 3; k++) {
+            sw[k] = wrapCoordinate(x[k], a.dx, size[k]);
+            near[k] = nearestIndex(sw[k], size[k]);
+        }
+    for (int k = 0; k < 3; k++) {
+        sw[k] = __shfl(sw[k], 0, FEATURE_LANES);
+        near[k] = __shfl(near[k], 0, FEATURE_LANES);
+    }
     if (active) {
-        int part = particles[i];
-        const double* x = positions + 3*(size_t) part;
-        int size[3] = {(int) a.g.nx, (int) a.g.ny, (int) a.g.nz};
         int openAxis[3] = {a.g.openX, a.g.openY, a.g.openZ};
         _Float16* xi = X + (size_t) i*MLP_IN_PAD;
         float* ri = raw ? raw + (size_t) i*MLP_IN : nullptr;
@@ -46,8 +59,7 @@ __global__ void __launch_bounds__(BLOCK) closureFeatures(const S* __restrict__ s
             int d[3] = {lane % 3 - 1, (lane/3) % 3 - 1, lane/9 - 1};
             int idx[3];
             for (int k = 0; k < 3; k++) {
-                double s = wrapCoordinate(x[k], a.dx, size[k]);
-                int t = nearestIndex(s, size[k]) + d[k];
+                int t = near[k] + d[k];
                 if (t < 0 || t >= size[k]) {
                     if (openAxis[k])
                         out = true;
@@ -81,7 +93,7 @@ __global__ void __launch_bounds__(BLOCK) closureFeatures(const S* __restrict__ s
         }
         else if (lane == 27) {
             for (int c = 0; c < 3; c++) {
-                double s = wrapCoordinate(x[c], a.dx, size[c]);
+                double s = sw[c];
                 float r = (float) (2.0*(s - floor(s + 0.5)));
                 xi[c] = (_Float16) r;
                 if (ri)
