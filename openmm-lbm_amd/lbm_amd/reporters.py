@@ -1,21 +1,28 @@
-"""Reporters of a run, for this package's Context: the counterparts of the reference's LBMTemperatureReporter and
-LBMVTKReporter (python/openmmlbm.i:619-860), which serve openmm.app.Simulation.
+"""Run reporters for this package's Context, counterparts of the reference's LBMTemperatureReporter and
+LBMVTKReporter (python/openmmlbm.i:619-872), which report from an openmm.app.Simulation.
 
-A reporter has describeNextReport(context) -> steps until its next report, and report(context); run(context, steps,
-reporters) steps a Context and calls them when due.
+A reporter offers describeNextReport(context), the number of steps until its next report, and report(context);
+run(context, steps, reporters) advances the Context and calls each reporter when its interval divides the step count
+(the part Simulation.step plays for the reference's reporters).
 
-* LBMTemperatureReporter: the temperature of the coupled particles, from the velocity that has the right temperature
-  for the drag scheme: the full-step velocity v + dt F/(2m) with the explicit drag, the state's velocity with the
-  centred drag; three degrees of freedom per coupled particle; lines "step time temperature" after a header written
-  only at the start of a file.
-* LBMVTKReporter: VTK XML files (ParaView, VisIt) in OpenMM units: <prefix>_density_<step>.vti and
-  <prefix>_velocity_<step>.vti (ImageData, point (i, j, k) at (i, j, k) dx, with a "solid" array when the lattice has
-  solid nodes), <prefix>_particles_<step>.vtp (PolyData: positions wrapped into the box unless wrap=False,
-  "velocity", "mass", "index", "coupled"), and <prefix>.pvd listing them with their times.  Arrays are appended in
-  binary (raw, little endian, each after its UInt64 byte count), single precision unless double=True.
+Temperature files hold the header '#"Step","Time (ps)","Temperature (K)"' (only when the file starts empty) and one
+line "step time temperature" per report.  The temperature counts three degrees of freedom per coupled particle and
+uses the velocity whose temperature is right for the drag scheme: with the explicit drag the full-step velocity
+v + dt F/(2m), with the centred drag the velocity of the state.
+
+VTK files (ParaView, VisIt) are written in OpenMM units, each array stored after the XML in an appended raw section
+(little endian, every array preceded by its UInt64 byte count), FP32 unless double=True:
+  <prefix>_density_<step>.vti, <prefix>_velocity_<step>.vti  ImageData, node (i, j, k) at (i, j, k) dx, plus an
+                                                              array "solid" when the lattice has solid nodes;
+  <prefix>_particles_<step>.vtp                               PolyData: positions (wrapped into the periodic box
+                                                              unless wrap=False), "velocity", "mass", "index",
+                                                              "coupled", one vertex per particle;
+  <prefix>.pvd                                                the collection of every file written, with its time.
+Each dataset carries a field-data string array "units" naming the unit of each of its arrays.
 """
 import os
 import struct
+import xml.etree.ElementTree as ET
 
 import numpy as np
 
@@ -24,175 +31,202 @@ from .workloads import BOLTZ
 
 
 def run(context, steps, reporters):
-    """Advance context by steps, calling each reporter whenever its interval divides the step count."""
-    done = 0
-    while done < steps:
-        todo = min([r.describeNextReport(context) for r in reporters] + [steps - done])
-        context.step(todo)
-        done += todo
-        for r in reporters:
-            if r.describeNextReport(context) == r.interval:
-                r.report(context)
+    """Advance context by steps, calling each reporter at every step count that its interval divides."""
+    target = context.getStepCount() + steps
+    while context.getStepCount() < target:
+        stride = min([r.describeNextReport(context) for r in reporters] + [target - context.getStepCount()])
+        context.step(stride)
+        due = [r for r in reporters if context.getStepCount() % r.interval == 0]
+        for r in due:
+            r.report(context)
+
+
+def _steps_to_next(interval, context):
+    return interval - context.getStepCount() % interval
 
 
 class LBMTemperatureReporter:
+    """file: a path (created or truncated) or an open text stream, such as sys.stdout or a file opened for appending
+    (a restart continues it without a second header)."""
+
     def __init__(self, file, reportInterval, force):
         self.interval = int(reportInterval)
         self.force = force
-        self._opened = isinstance(file, str)
-        self._out = open(file, "a" if self._opened and os.path.exists(file) else "w") if self._opened else file
-        self._particles = None
+        self._owns = isinstance(file, str)
+        self._stream = open(file, "w") if self._owns else file
+        self._coupled = None
 
     def __del__(self):
-        if getattr(self, "_opened", False):
-            self._out.close()
+        if getattr(self, "_owns", False) and not self._stream.closed:
+            self._stream.close()
 
     def describeNextReport(self, context):
-        return self.interval - context.getStepCount() % self.interval
+        return _steps_to_next(self.interval, context)
+
+    def _starts_empty(self):
+        try:
+            return self._stream.tell() == 0
+        except (AttributeError, OSError, ValueError):
+            return True          # a stream without a position
 
     def report(self, context):
-        if self._particles is None:
-            self._particles = [self.force.getParticle(i) for i in range(self.force.getNumParticles())]
-            self._masses = np.array([context.system.masses[i] for i in self._particles])
-            try:
-                new = self._out.tell() == 0
-            except (AttributeError, OSError, ValueError):
-                new = True
-            if new:
-                print('#"Step","Time (ps)","Temperature (K)"', file=self._out)
-        v = np.asarray(context.velocities)[self._particles]
+        if self._coupled is None:
+            self._coupled = np.array([self.force.getParticle(i) for i in range(self.force.getNumParticles())],
+                                     dtype=np.int64)
+            self._mass = np.asarray(context.masses, dtype=np.float64)[self._coupled]
+            if self._starts_empty():
+                self._stream.write('#"Step","Time (ps)","Temperature (K)"\n')
+        dt = context.integrator.stepSize
+        v = np.asarray(context.velocities, dtype=np.float64)[self._coupled]
         if self.force.getDragScheme() != LBMForce.Centered:
-            f = np.asarray(context.getForces())[self._particles]
-            v = v + f*(0.5*context.integrator.stepSize/self._masses[:, None])
-        energy = 0.5*np.sum(self._masses[:, None]*v*v)
-        temperature = 2*energy/(3*len(self._particles)*BOLTZ)
+            # Leapfrog velocities sit half a step before the forces of this step.
+            v = v + np.asarray(context.getForces())[self._coupled]*(dt/(2.0*self._mass))[:, None]
+        twice_kinetic = float(np.einsum("i,ij,ij->", self._mass, v, v))
+        temperature = twice_kinetic/(3.0*len(self._coupled)*BOLTZ)
         step = context.getStepCount()
-        print("%d %.4f %.3f" % (step, step*context.integrator.stepSize, temperature), file=self._out)
-        self._out.flush()
+        self._stream.write("%d %.4f %.3f\n" % (step, step*dt, temperature))
+        self._stream.flush()
 
 
-def _units_field(lines):
-    """Field data with the string array "units" (one "array: unit" line each), ASCII: character codes, each string
-    ended by 0."""
-    codes = " ".join(" ".join(str(ord(c)) for c in line) + " 0" for line in lines)
-    return ('    <FieldData>\n'
-            f'      <DataArray type="String" Name="units" NumberOfTuples="{len(lines)}" format="ascii">\n'
-            f'        {codes}\n'
-            '      </DataArray>\n'
-            '    </FieldData>\n')
+class _AppendedDataset:
+    """A VTK XML file under construction: an element tree whose DataArray elements point into one appended raw
+    section, filled in the order the arrays are added."""
 
+    _TYPES = {np.dtype(np.float32): "Float32", np.dtype(np.float64): "Float64", np.dtype(np.uint8): "UInt8",
+              np.dtype(np.int32): "Int32", np.dtype(np.int64): "Int64"}
 
-def _write_vtk(path, kind, opening, closing, arrays):
-    """A VTK XML file whose data arrays are appended in binary; opening holds {offset<i>} placeholders."""
-    blocks = [np.ascontiguousarray(a).astype(a.dtype.newbyteorder("<"), copy=False).tobytes() for a in arrays]
-    offsets, pos = {}, 0
-    for i, b in enumerate(blocks):
-        offsets[f"offset{i}"] = pos
-        pos += 8 + len(b)
-    head = ('<?xml version="1.0"?>\n'
-            f'<VTKFile type="{kind}" version="1.0" byte_order="LittleEndian" header_type="UInt64">\n' +
-            opening.format(**offsets) + closing + '  <AppendedData encoding="raw">\n_')
-    with open(path, "wb") as out:
-        out.write(head.encode("ascii"))
-        for b in blocks:
-            out.write(struct.pack("<Q", len(b)))
-            out.write(b)
-        out.write(b"\n  </AppendedData>\n</VTKFile>\n")
+    def __init__(self, kind, units):
+        self.root = ET.Element("VTKFile", type=kind, version="1.0", byte_order="LittleEndian", header_type="UInt64")
+        self.data = ET.SubElement(self.root, kind)
+        self._units = units
+        self._payload = []
+        self._offset = 0
+
+    def array(self, parent, values, name=None, components=1):
+        values = np.ascontiguousarray(values)
+        attrs = {"type": self._TYPES[values.dtype]}
+        if name is not None:
+            attrs["Name"] = name
+        if components > 1:
+            attrs["NumberOfComponents"] = str(components)
+        attrs.update(format="appended", offset=str(self._offset))
+        ET.SubElement(parent, "DataArray", attrs)
+        raw = values.astype(values.dtype.newbyteorder("<"), copy=False).tobytes()
+        self._payload.append(struct.pack("<Q", len(raw)) + raw)
+        self._offset += len(self._payload[-1])
+
+    def write(self, path):
+        # String arrays in ASCII are the character codes of each string followed by 0.
+        units = ET.Element("FieldData")
+        codes = []
+        for line in self._units:
+            codes.extend(str(b) for b in line.encode("ascii"))
+            codes.append("0")
+        strings = ET.SubElement(units, "DataArray", type="String", Name="units",
+                                NumberOfTuples=str(len(self._units)), format="ascii")
+        strings.text = " ".join(codes)
+        self.data.insert(0, units)
+        ET.indent(self.root, space="  ")
+        xml = ET.tostring(self.root, encoding="unicode")
+        body, tail = xml.rsplit("</VTKFile>", 1)
+        with open(path, "wb") as out:
+            out.write(b'<?xml version="1.0"?>\n')
+            out.write(body.encode("ascii"))
+            out.write(b'  <AppendedData encoding="raw">\n_')
+            for block in self._payload:
+                out.write(block)
+            out.write(b"\n  </AppendedData>\n</VTKFile>\n")
 
 
 class LBMVTKReporter:
+    """Arguments as the reference's: fluid/density/velocity/particles choose the files, double the precision, wrap
+    the folding of positions into the box, append keeps the datasets already listed in <prefix>.pvd (a restart)."""
+
     def __init__(self, prefix, reportInterval, force, fluid=True, particles=True, double=False, wrap=True, append=False,
                  density=True, velocity=True):
         self.prefix = prefix
         self.interval = int(reportInterval)
         self.force = force
-        self._density = fluid and density
-        self._velocity = fluid and velocity
+        self._fields = [name for name, on in (("density", fluid and density), ("velocity", fluid and velocity)) if on]
         self._particles = particles
-        self._double = double
+        self._real = np.float64 if double else np.float32
         self._wrap = wrap
-        self._datasets = []
+        self._collection = []
         if append and os.path.exists(prefix + ".pvd"):
-            with open(prefix + ".pvd") as pvd:
-                self._datasets = [line for line in pvd if "<DataSet " in line]
+            listed = ET.parse(prefix + ".pvd").getroot().find("Collection")
+            self._collection = [dict(d.attrib) for d in listed.iter("DataSet")]
 
     def describeNextReport(self, context):
-        return self.interval - context.getStepCount() % self.interval
+        return _steps_to_next(self.interval, context)
+
+    def _fluid_file(self, context, field, values, solid):
+        nx, ny, nz = self.force.getGridSize()
+        dx = context.getLatticeParameters()[0]
+        extent = f"0 {nx - 1} 0 {ny - 1} 0 {nz - 1}"
+        units = ["Origin, Spacing: nm", "density: Da/nm^3" if field == "density" else "velocity: nm/ps"]
+        if solid is not None:
+            units.append("solid: 1 for a solid node, 0 for a fluid node")
+        ds = _AppendedDataset("ImageData", units)
+        ds.data.set("WholeExtent", extent)
+        ds.data.set("Origin", "0 0 0")
+        ds.data.set("Spacing", f"{dx!r} {dx!r} {dx!r}")
+        piece = ET.SubElement(ds.data, "Piece", Extent=extent)
+        point = ET.SubElement(piece, "PointData", {"Scalars" if field == "density" else "Vectors": field})
+        ds.array(point, np.asarray(values, dtype=self._real).reshape(-1), field, 1 if field == "density" else 3)
+        if solid is not None:
+            ds.array(point, solid, "solid")
+        return ds
+
+    def _particle_file(self, context):
+        n = context.system.getNumParticles()
+        pos = np.asarray(context.positions, dtype=np.float64)
+        if self._wrap:
+            box = np.asarray(context.system.box, dtype=np.float64)
+            pos = pos - box*np.floor(pos/box)
+        coupled = np.zeros(n, dtype=np.uint8)
+        coupled[[self.force.getParticle(i) for i in range(self.force.getNumParticles())]] = 1
+        ds = _AppendedDataset("PolyData", ["Points: nm", "velocity: nm/ps, at the half step of the leapfrog",
+                                           "mass: Da", "index: index of the particle in the System",
+                                           "coupled: 1 for a particle coupled to the fluid"])
+        piece = ET.SubElement(ds.data, "Piece", NumberOfPoints=str(n), NumberOfVerts=str(n), NumberOfLines="0",
+                              NumberOfStrips="0", NumberOfPolys="0")
+        ds.array(ET.SubElement(piece, "Points"), pos.astype(self._real).reshape(-1), components=3)
+        point = ET.SubElement(piece, "PointData", Vectors="velocity")
+        ds.array(point, np.asarray(context.velocities, dtype=self._real).reshape(-1), "velocity", 3)
+        ds.array(point, np.asarray(context.system.masses, dtype=self._real), "mass")
+        ds.array(point, np.arange(n, dtype=np.int32), "index")
+        ds.array(point, coupled, "coupled")
+        verts = ET.SubElement(piece, "Verts")
+        ds.array(verts, np.arange(n, dtype=np.int64), "connectivity")
+        ds.array(verts, np.arange(1, n + 1, dtype=np.int64), "offsets")
+        return ds
 
     def report(self, context):
-        real = np.float64 if self._double else np.float32
-        tname = "Float64" if self._double else "Float32"
         step = context.getStepCount()
-        time = step*context.integrator.stepSize
-        files = []
-        nx, ny, nz = self.force.getGridSize()
-        solid = self.force.getSolidNodes()
-        dx = context.getLatticeParameters()[0]
-        if self._density or self._velocity:
+        written = []
+        if self._fields:
             rho, u = context.getFluidFields()
-        mask = None
-        if solid:
-            mask = np.zeros(nx*ny*nz, dtype=np.uint8)
-            mask[np.asarray(solid)] = 1
-        for field, comps, unit, values in (("density", 1, "Da/nm^3", rho if self._density else None),
-                                           ("velocity", 3, "nm/ps", u if self._velocity else None)):
-            if values is None:
-                continue
-            extent = f"0 {nx - 1} 0 {ny - 1} 0 {nz - 1}"
-            units = ["Origin, Spacing: nm", f"{field}: {unit}"]
-            arrays = [np.asarray(values, dtype=real).reshape(-1)]
-            if mask is not None:
-                units.append("solid: 1 for a solid node, 0 for a fluid node")
-                arrays.append(mask)
-            kind = "Scalars" if comps == 1 else "Vectors"
-            ncomp = ' NumberOfComponents="3"' if comps == 3 else ""
-            opening = (f'  <ImageData WholeExtent="{extent}" Origin="0 0 0" Spacing="{dx!r} {dx!r} {dx!r}">\n' +
-                       _units_field(units) +
-                       f'    <Piece Extent="{extent}">\n'
-                       f'      <PointData {kind}="{field}">\n'
-                       f'        <DataArray type="{tname}" Name="{field}"{ncomp} format="appended" offset="{{offset0}}"/>\n')
-            if mask is not None:
-                opening += '        <DataArray type="UInt8" Name="solid" format="appended" offset="{offset1}"/>\n'
-            closing = "      </PointData>\n    </Piece>\n  </ImageData>\n"
-            path = f"{self.prefix}_{field}_{step:010d}.vti"
-            _write_vtk(path, "ImageData", opening, closing, arrays)
-            files.append(path)
+            nodes = self.force.getSolidNodes()
+            solid = None
+            if len(nodes):
+                nx, ny, nz = self.force.getGridSize()
+                solid = np.zeros(nx*ny*nz, dtype=np.uint8)
+                solid[np.asarray(nodes, dtype=np.int64)] = 1
+            for field in self._fields:
+                path = f"{self.prefix}_{field}_{step:010d}.vti"
+                self._fluid_file(context, field, rho if field == "density" else u, solid).write(path)
+                written.append(path)
         if self._particles:
-            n = context.system.getNumParticles()
-            pos = np.asarray(context.positions, dtype=np.float64)
-            if self._wrap:
-                box = np.asarray(context.system.box, dtype=np.float64)
-                pos = pos - np.floor(pos/box)*box
-            coupled = np.zeros(n, dtype=np.uint8)
-            coupled[[self.force.getParticle(i) for i in range(self.force.getNumParticles())]] = 1
-            arrays = [pos.astype(real).reshape(-1), np.asarray(context.velocities, dtype=real).reshape(-1),
-                      np.asarray(context.system.masses, dtype=real), np.arange(n, dtype=np.int32), coupled,
-                      np.arange(n, dtype=np.int64), np.arange(1, n + 1, dtype=np.int64)]
-            units = ["Points: nm", "velocity: nm/ps, at the half step of the leapfrog", "mass: Da",
-                     "index: index of the particle in the System", "coupled: 1 for a particle coupled to the fluid"]
-            opening = ("  <PolyData>\n" + _units_field(units) +
-                       f'    <Piece NumberOfPoints="{n}" NumberOfVerts="{n}" NumberOfLines="0" NumberOfStrips="0" '
-                       'NumberOfPolys="0">\n'
-                       '      <Points>\n'
-                       f'        <DataArray type="{tname}" NumberOfComponents="3" format="appended" offset="{{offset0}}"/>\n'
-                       '      </Points>\n'
-                       '      <PointData Vectors="velocity">\n'
-                       f'        <DataArray type="{tname}" Name="velocity" NumberOfComponents="3" format="appended" '
-                       'offset="{offset1}"/>\n'
-                       f'        <DataArray type="{tname}" Name="mass" format="appended" offset="{{offset2}}"/>\n'
-                       '        <DataArray type="Int32" Name="index" format="appended" offset="{offset3}"/>\n'
-                       '        <DataArray type="UInt8" Name="coupled" format="appended" offset="{offset4}"/>\n'
-                       '      </PointData>\n'
-                       '      <Verts>\n'
-                       '        <DataArray type="Int64" Name="connectivity" format="appended" offset="{offset5}"/>\n'
-                       '        <DataArray type="Int64" Name="offsets" format="appended" offset="{offset6}"/>\n'
-                       '      </Verts>\n')
-            closing = "    </Piece>\n  </PolyData>\n"
             path = f"{self.prefix}_particles_{step:010d}.vtp"
-            _write_vtk(path, "PolyData", opening, closing, arrays)
-            files.append(path)
-        for path in files:
-            self._datasets.append(f'    <DataSet timestep="{time!r}" part="0" file="{os.path.basename(path)}"/>\n')
-        with open(self.prefix + ".pvd", "w") as pvd:
-            pvd.write('<?xml version="1.0"?>\n<VTKFile type="Collection" version="1.0" byte_order="LittleEndian">\n'
-                      "  <Collection>\n" + "".join(self._datasets) + "  </Collection>\n</VTKFile>\n")
+            self._particle_file(context).write(path)
+            written.append(path)
+        time = step*context.integrator.stepSize
+        for part, path in enumerate(written):
+            self._collection.append({"timestep": repr(time), "part": str(part), "file": os.path.basename(path)})
+        pvd = ET.Element("VTKFile", type="Collection", version="0.1", byte_order="LittleEndian")
+        listed = ET.SubElement(pvd, "Collection")
+        for entry in self._collection:
+            ET.SubElement(listed, "DataSet", entry)
+        ET.indent(pvd, space="  ")
+        with open(self.prefix + ".pvd", "w") as out:
+            out.write('<?xml version="1.0"?>\n' + ET.tostring(pvd, encoding="unicode") + "\n")

@@ -1,7 +1,7 @@
 """The run reporters (lbm_amd/reporters.py), on the CPU oracle: the temperature file (python/openmmlbm.i:619-680) and
 the VTK files (:682-860), read back and compared with the Context's own fields and particles."""
-import re
 import struct
+import xml.etree.ElementTree as ET
 
 import numpy as np
 
@@ -30,17 +30,27 @@ def small_context(drag=LBMForce.Explicit):
 
 
 def read_vtk(path):
+    """The XML part of a VTK file (parsed) and its appended arrays by name ("Points" for the unnamed one)."""
     data = open(path, "rb").read()
-    head_end = data.index(b'<AppendedData encoding="raw">\n_') + len(b'<AppendedData encoding="raw">\n_')
-    head = data[:head_end].decode()
+    mark = b'<AppendedData encoding="raw">\n_'
+    start = data.index(mark)
+    root = ET.fromstring(data[:start].decode("ascii") + "</VTKFile>")
+    base = start + len(mark)
     arrays = {}
-    for m in re.finditer(r'<DataArray type="(\w+)"(?: Name="(\w+)")?(?: NumberOfComponents="3")? format="appended" '
-                         r'offset="(\d+)"/>', head):
-        kind, name, off = m.group(1), m.group(2) or "Points", int(m.group(3))
-        size = struct.unpack_from("<Q", data, head_end + off)[0]
-        dtype = {"Float32": "<f4", "Float64": "<f8", "UInt8": "u1", "Int32": "<i4", "Int64": "<i8"}[kind]
-        arrays[name] = np.frombuffer(data, dtype=dtype, count=size//np.dtype(dtype).itemsize, offset=head_end + off + 8)
-    return head, arrays
+    for el in root.iter("DataArray"):
+        if el.get("format") != "appended":
+            continue
+        dtype = np.dtype({"Float32": "<f4", "Float64": "<f8", "UInt8": "u1", "Int32": "<i4", "Int64": "<i8"}[el.get("type")])
+        off = base + int(el.get("offset"))
+        size = struct.unpack_from("<Q", data, off)[0]
+        arrays[el.get("Name", "Points")] = np.frombuffer(data, dtype=dtype, count=size//dtype.itemsize, offset=off + 8)
+    assert data.endswith(b"\n  </AppendedData>\n</VTKFile>\n")
+    return root, arrays
+
+
+def units(root):
+    codes = [int(c) for c in root.find(".//FieldData/DataArray").text.split()]
+    return bytes(codes).decode("ascii").split("\0")[:-1]
 
 
 def test_vtk_files(tmp_path):
@@ -48,22 +58,32 @@ def test_vtk_files(tmp_path):
     prefix = str(tmp_path/"run")
     rep = LBMVTKReporter(prefix, 4, ctx.force, double=True)
     run(ctx, 8, [rep])
-    head, a = read_vtk(f"{prefix}_velocity_{8:010d}.vti")
+    root, a = read_vtk(f"{prefix}_velocity_{8:010d}.vti")
     rho, u = ctx.getFluidFields()
-    assert 'Spacing="1.0 1.0 1.0"' in head and 'WholeExtent="0 5 0 5 0 5"' in head
+    image = root.find("ImageData")
+    assert image.get("Spacing") == "1.0 1.0 1.0" and image.get("WholeExtent") == "0 5 0 5 0 5"
+    assert units(root) == ["Origin, Spacing: nm", "velocity: nm/ps", "solid: 1 for a solid node, 0 for a fluid node"]
     assert np.array_equal(a["velocity"], u.reshape(-1))
     assert a["solid"].sum() == 6 and a["solid"][:6].all()
     _, d = read_vtk(f"{prefix}_density_{8:010d}.vti")
     assert np.array_equal(d["density"], rho)
-    _, p = read_vtk(f"{prefix}_particles_{8:010d}.vtp")
+    root, p = read_vtk(f"{prefix}_particles_{8:010d}.vtp")
+    assert root.find("PolyData/Piece").get("NumberOfVerts") == "5"
     pos = p["Points"].reshape(-1, 3)
     box = np.array(s.grid, dtype=float)*s.dx
     assert np.allclose(pos, ctx.positions - np.floor(ctx.positions/box)*box)
     assert np.all((pos >= 0) & (pos < box))
     assert np.array_equal(p["velocity"].reshape(-1, 3), ctx.velocities)
     assert list(p["coupled"]) == [1]*5 and list(p["index"]) == list(range(5))
-    pvd = open(prefix + ".pvd").read()
-    assert pvd.count("<DataSet ") == 6 and 'timestep="0.04"' in pvd and 'timestep="0.08"' in pvd
+    assert list(p["connectivity"]) == list(range(5)) and list(p["offsets"]) == list(range(1, 6))
+    sets = ET.parse(prefix + ".pvd").getroot().findall("Collection/DataSet")
+    assert [(d.get("timestep"), d.get("part")) for d in sets] == [(t, str(k)) for t in ("0.04", "0.08") for k in range(3)]
+    # A restart with append=True keeps the listed datasets; single precision by default.
+    more = LBMVTKReporter(prefix, 4, ctx.force, append=True, particles=False)
+    run(ctx, 4, [more])
+    _, f = read_vtk(f"{prefix}_density_{12:010d}.vti")
+    assert f["density"].dtype == np.float32
+    assert len(ET.parse(prefix + ".pvd").getroot().findall("Collection/DataSet")) == 8
 
 
 def test_temperature_file(tmp_path):
@@ -81,3 +101,13 @@ def test_temperature_file(tmp_path):
             v = v + ctx.getForces()*(0.5*0.01/100.0)
         expected = 100.0*np.sum(v*v)/(3*5*BOLTZ)
         assert abs(float(temperature) - expected) <= 1e-3
+
+
+def test_temperature_continues_an_appended_file(tmp_path):
+    ctx, _ = small_context()
+    path = tmp_path/"t.txt"
+    run(ctx, 4, [LBMTemperatureReporter(str(path), 2, ctx.force)])
+    with open(path, "a") as stream:
+        run(ctx, 4, [LBMTemperatureReporter(stream, 2, ctx.force)])
+    lines = path.read_text().splitlines()
+    assert [line.split()[0] for line in lines] == ['#"Step","Time', "2", "4", "6", "8"]
