@@ -486,6 +486,9 @@ void create(const lbm_params* params, const int* solid, int numSolid, const int*
     c->g.openY = c->openAxis(1);
     c->g.openZ = c->openAxis(2);
     c->elem = storageSize(p.storage);
+    if (n*c->elem > (1ull << 32))
+        throw Error(LBM_ERROR_INVALID, "LBMForce: the lattice is too large for one GPU: a population plane would exceed "
+                "4 GiB (at most 2^30 nodes with FP32 storage, 2^29 with FP64)");
     c->seed = (uint32_t) p.random_number_seed;
     if (c->seed == 0)
         c->seed = (uint32_t) std::chrono::high_resolution_clock::now().time_since_epoch().count() | 1u;

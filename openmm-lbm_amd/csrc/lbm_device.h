@@ -100,25 +100,41 @@ struct Moments {
     R dr, jx, jy, jz, sxx, syy, szz, sxy, sxz, syz;
 };
 
+/* The directions come in opposite pairs (2p+1, 2p+2).  With s_p = f_q + f_opp and d_p = f_q - f_opp, the odd
+   moments take d_p times c_q and the even ones s_p times c_q c_q: 51 additions instead of 90. */
+constexpr bool oppositePairs() {
+    for (int p = 0; p < 9; p++)
+        if (OPP[2*p + 1] != 2*p + 2 || CX[2*p + 1] != -CX[2*p + 2] || CY[2*p + 1] != -CY[2*p + 2] ||
+            CZ[2*p + 1] != -CZ[2*p + 2])
+            return false;
+    return true;
+}
+static_assert(oppositePairs(), "the velocity set is not ordered in opposite pairs");
+
 template <typename R>
 __device__ __forceinline__ Moments<R> nodeMoments(const R f[NQ]) {
+    R s[9], d[9];
+#pragma unroll
+    for (int p = 0; p < 9; p++) {
+        s[p] = f[2*p + 1] + f[2*p + 2];
+        d[p] = f[2*p + 1] - f[2*p + 2];
+    }
+    // Pairs: 0 x, 1 y, 2 z, 3 (1,1,0), 4 (1,-1,0), 5 (0,1,1), 6 (0,1,-1), 7 (1,0,1), 8 (-1,0,1).
+    static_assert(CX[7] == 1 && CY[7] == 1 && CX[9] == 1 && CY[9] == -1 && CY[11] == 1 && CZ[11] == 1 &&
+                  CY[13] == 1 && CZ[13] == -1 && CX[15] == 1 && CZ[15] == 1 && CX[17] == -1 && CZ[17] == 1,
+                  "pair table of nodeMoments");
     Moments<R> m;
-    m.dr = f[0];
-    m.jx = m.jy = m.jz = m.sxx = m.syy = m.szz = m.sxy = m.sxz = m.syz = R(0);
-    staticFor<NQ - 1>([&](auto I) {
-        constexpr int q = decltype(I)::value + 1;
-        R v = f[q];
-        m.dr += v;
-        m.jx = addTerm<CX[q]>(m.jx, v);
-        m.jy = addTerm<CY[q]>(m.jy, v);
-        m.jz = addTerm<CZ[q]>(m.jz, v);
-        m.sxx = addTerm<CX[q]*CX[q]>(m.sxx, v);
-        m.syy = addTerm<CY[q]*CY[q]>(m.syy, v);
-        m.szz = addTerm<CZ[q]*CZ[q]>(m.szz, v);
-        m.sxy = addTerm<CX[q]*CY[q]>(m.sxy, v);
-        m.sxz = addTerm<CX[q]*CZ[q]>(m.sxz, v);
-        m.syz = addTerm<CY[q]*CZ[q]>(m.syz, v);
-    });
+    m.dr = f[0] + (((s[0] + s[1]) + (s[2] + s[3])) + ((s[4] + s[5]) + (s[6] + s[7])) + s[8]);
+    m.jx = d[0] + ((d[3] + d[4]) + (d[7] - d[8]));
+    m.jy = d[1] + ((d[3] - d[4]) + (d[5] + d[6]));
+    m.jz = d[2] + ((d[5] - d[6]) + (d[7] + d[8]));
+    R xy = s[3] + s[4], yz = s[5] + s[6], xz = s[7] + s[8];   // the diagonal pairs of each coordinate plane
+    m.sxx = s[0] + (xy + xz);
+    m.syy = s[1] + (xy + yz);
+    m.szz = s[2] + (yz + xz);
+    m.sxy = s[3] - s[4];
+    m.syz = s[5] - s[6];
+    m.sxz = s[7] - s[8];
     return m;
 }
 
@@ -199,6 +215,56 @@ __device__ __forceinline__ void ghostNoise(const R r[16], R amp, R g[NQ]) {
                 sum += R(coef)*r[k - 4];
         });
         g[q] = amp*sum;
+    });
+}
+
+/* Ghost modes 10-15 are odd in c and 16-18 even (D3Q19.h:145-169), which ghostNoiseWeighted relies on. */
+constexpr bool ghostParity() {
+    for (int q = 1; q < NQ; q += 2)
+        for (int k = 10; k < NQ; k++)
+            if (basisMode(k, q + 1) != (k < 16 ? -1.0 : 1.0)*basisMode(k, q))
+                return false;
+    for (int k = 10; k < 16; k++)
+        if (basisMode(k, 0) != 0.0)
+            return false;
+    return true;
+}
+static_assert(ghostParity(), "parity of the ghost modes");
+
+/* c*x for a compile-time coefficient c, added to acc; a zero coefficient adds nothing. */
+template <typename R>
+__device__ __forceinline__ R addScaled(R acc, double c, R x) {
+    return c == 0.0 ? acc : acc + R(c)*x;
+}
+
+/*
+ * w_q g_q for the ghost part g_q of ghostNoise (the same normals r[6..14] and amplitude), formed per opposite pair:
+ * with the odd modes' sum O_q and the even modes' sum E_q at c_q, w g_q = E_q + O_q and w g_opp = E_q - O_q.
+ */
+template <typename R>
+__device__ __forceinline__ void ghostNoiseWeighted(const R r[16], R amp, R out[NQ]) {
+    R a[9];
+#pragma unroll
+    for (int k = 0; k < 9; k++)
+        a[k] = amp*r[k + 6];
+    out[0] = R(0);
+    staticFor<3>([&](auto K) {
+        constexpr int k = decltype(K)::value + 16;
+        out[0] = addScaled(out[0], WQ[0]*basisMode(k, 0)*INV_SQRT_NORM[k], a[k - 10]);
+    });
+    staticFor<9>([&](auto P) {
+        constexpr int q = 2*decltype(P)::value + 1;
+        R odd = R(0), even = R(0);
+        staticFor<9>([&](auto K) {
+            constexpr int k = decltype(K)::value + 10;
+            constexpr double c = WQ[q]*basisMode(k, q)*INV_SQRT_NORM[k];
+            if constexpr (k < 16)
+                odd = addScaled(odd, c, a[k - 10]);
+            else
+                even = addScaled(even, c, a[k - 10]);
+        });
+        out[q] = even + odd;
+        out[q + 1] = even - odd;
     });
 }
 
@@ -485,12 +551,24 @@ __device__ __forceinline__ void pullNode(const S* __restrict__ src, uint32_t n, 
     });
 }
 
+/* The element at a 32-bit byte offset from a population plane: the plane's base stays in scalar registers and the
+   access takes the offset from one vector register.  Planes are at most 4 GiB (checked at context creation). */
+template <typename T>
+__device__ __forceinline__ T* atByte(T* plane, uint32_t offset) {
+    using Byte = typename std::conditional<std::is_const<T>::value, const char, char>::type;
+    return reinterpret_cast<T*>(reinterpret_cast<Byte*>(plane) + offset);
+}
+
+/* The fused pass's pull: streaming loads at byte offsets node*sizeof(S) + pullOffset*sizeof(S) in each plane. */
 template <typename S>
 __device__ __forceinline__ void pullNodeStream(const S* __restrict__ src, uint32_t n, uint32_t node, const Neighbours& nb,
         typename StorageTraits<S>::Real f[NQ]) {
+    constexpr int B = (int) sizeof(S);
+    const Neighbours nbBytes = {nb.xm*B, nb.xp*B, nb.ym*B, nb.yp*B, nb.zm*B, nb.zp*B};
+    const uint32_t at = node*(uint32_t) B;
     staticFor<NQ>([&](auto I) {
         constexpr int q = decltype(I)::value;
-        f[q] = loadPopStream(src + (size_t) q*n + (uint32_t) ((int) node + pullOffset<q>(nb)));
+        f[q] = loadPopStream(atByte(src + (size_t) q*n, at + (uint32_t) pullOffset<q>(nbBytes)));
     });
 }
 

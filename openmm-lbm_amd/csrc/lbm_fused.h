@@ -68,38 +68,36 @@ __device__ __forceinline__ void fusedNode(const S* __restrict__ src, S* __restri
         }
     }
     C stress[6] = {C(0), C(0), C(0), C(0), C(0), C(0)};
-    C ghost[NQ];
     C omega = C(a.omega);
+    // The random part is formed in the storage's load type R (FP32, or FP64 with FP64 storage): its precision is
+    // that of a random number.  Its stress modes join P; its ghost modes, weighted, are added after the collision.
+    R noise[NQ];
     if constexpr (FLUCT) {
-        // The random part in FP32 (FP64 with FP64 storage): its precision is that of a random number.
-        using N = typename std::conditional<std::is_same<S, double>::value, double, float>::type;
-        N r[16], s[6], g[NQ];
-        drawNormals<N, 16>(a.seed, TAG_FLUID, node, a.step, r);
-        N mu = N(a.mu), rhoN = N(rho), om = N(a.omega);
-        N stressAmp, ghostAmp;
-        if constexpr (std::is_same<N, float>::value) {
-            stressAmp = __builtin_amdgcn_sqrtf(mu*rhoN*om*(2.0f - om));
-            ghostAmp = __builtin_amdgcn_sqrtf(mu*rhoN);
+        R r[16], s[6];
+        drawNormals<R, 16>(a.seed, TAG_FLUID, node, a.step, r);
+        R mu = R(a.mu), rhoR = R(rho), om = R(a.omega);
+        R stressAmp, ghostAmp;
+        if constexpr (std::is_same<R, float>::value) {
+            stressAmp = __builtin_amdgcn_sqrtf(mu*rhoR*om*(2.0f - om));
+            ghostAmp = __builtin_amdgcn_sqrtf(mu*rhoR);
         }
         else {
-            stressAmp = sqrt(mu*rhoN*om*(N(2) - om));
-            ghostAmp = sqrt(mu*rhoN);
+            stressAmp = sqrt(mu*rhoR*om*(R(2) - om));
+            ghostAmp = sqrt(mu*rhoR);
         }
         stressNoise(r, stressAmp, s);
-        ghostNoise(r, ghostAmp, g);
+        ghostNoiseWeighted(r, ghostAmp, noise);
 #pragma unroll
         for (int c = 0; c < 6; c++)
             stress[c] = C(s[c]);
-#pragma unroll
-        for (int q = 0; q < NQ; q++)
-            ghost[q] = C(g[q]);
     }
     C out[NQ];
     // The exact weights matter where the storage keeps FP32 or FP64 precision; FP16/BF16 rounding is far coarser.
-    collide<C, FLUCT, !isReduced<S>()>(m.dr, jx, jy, jz, pi, fx, fy, fz, omega, stress, ghost, out);
+    collide<C, false, !isReduced<S>()>(m.dr, jx, jy, jz, pi, fx, fy, fz, omega, stress, nullptr, out);
     uint32_t rnd = 0;
     if constexpr (isReduced<S>())
         rnd = mixBits(node ^ a.roundSeed);
+    const uint32_t at = node*(uint32_t) sizeof(S);
 #pragma unroll
     for (int q = 0; q < NQ; q++) {
         uint32_t r = 0;
@@ -107,7 +105,10 @@ __device__ __forceinline__ void fusedNode(const S* __restrict__ src, S* __restri
             r = (rnd ^ (uint32_t) (q*0x9E3779B9u))*0x85EBCA6Bu;
             r ^= r >> 13;
         }
-        storePopStream(dst + (size_t) q*a.g.n + node, (R) out[q], r);
+        R v = (R) out[q];
+        if constexpr (FLUCT)
+            v += noise[q];
+        storePopStream(atByte(dst + (size_t) q*a.g.n, at), v, r);
     }
 }
 
