@@ -79,6 +79,49 @@ __device__ __forceinline__ void axisWeights(int stencil, int w, double s, int n,
     }
 }
 
+/* The stencil that a group of LANES lanes couples with (one lane per slot): 1 nearest, 8 trilinear, 32 three-point (27
+   slots), 64 Keys; the launchers pick LANES from the stencil. */
+template <int LANES>
+constexpr int groupStencil() {
+    return LANES == 64 ? LBM_STENCIL_KEYS : LANES == 32 ? LBM_STENCIL_THREE_POINT :
+           LANES == 8 ? LBM_STENCIL_TRILINEAR : LBM_STENCIL_NEAREST;
+}
+
+/*
+ * Node and weight of slot `sub` of the stencil along one axis: axisWeights for one slot, one kernel evaluation.  The
+ * slots are w consecutive indices from `first`, within [-1, n + 1]; they fall on distinct nodes unless the lattice is
+ * narrower than the stencil (periodic) or the stencil reaches past an open face (clamped): there the full rule, with
+ * repeated nodes merged into the first slot, applies.
+ */
+template <int STENCIL>
+__device__ __forceinline__ void axisSlot(double s, int n, bool open, int sub, int& index, double& weight) {
+#pragma clang fp contract(off)
+    if constexpr (STENCIL == LBM_STENCIL_NEAREST) {
+        (void) open;
+        (void) sub;
+        index = nearestIndex(s, n);
+        weight = 1.0;
+    }
+    else {
+        constexpr int w = STENCIL == LBM_STENCIL_KEYS ? 4 : STENCIL == LBM_STENCIL_THREE_POINT ? 3 : 2;
+        int first = STENCIL == LBM_STENCIL_THREE_POINT ? (int) floor(s + 0.5) - 1 :
+                    STENCIL == LBM_STENCIL_KEYS ? (int) floor(s) - 1 : (int) floor(s);
+        bool merged = open ? (first < 0 || first + w > n) : n < w;
+        if (!merged) {
+            int i = first + sub;
+            weight = stencilKernel(STENCIL, s - (double) i);
+            index = open ? i : (i < 0 ? i + n : (i >= n ? i - n : i));
+        }
+        else {
+            int ix[4];
+            double wx[4];
+            axisWeights(STENCIL, w, s, n, open, ix, wx);
+            index = ix[sub];
+            weight = wx[sub];
+        }
+    }
+}
+
 __device__ __forceinline__ bool solidAt(const uint32_t* mask, uint32_t node) {
     return mask != nullptr && isSolid(mask, node);
 }
@@ -146,11 +189,12 @@ __device__ __forceinline__ void nodeVelocity(const S* __restrict__ src, const Ge
 template <typename S, int LANES>
 __device__ __forceinline__ StencilSlot interpolateGroup(const S* __restrict__ src, const uint32_t* __restrict__ mask,
         const double* x, bool active, int lane, const double* __restrict__ ucm, const ParticleArgs& a, double u[3]) {
-    int w = stencilWidth(a.stencil);
-    int numSlots = w*w*w;
+    constexpr int STENCIL = groupStencil<LANES>();
+    constexpr int w = STENCIL == LBM_STENCIL_KEYS ? 4 : STENCIL == LBM_STENCIL_THREE_POINT ? 3 :
+                      STENCIL == LBM_STENCIL_TRILINEAR ? 2 : 1;
     int slot = lane;
     StencilSlot sl;
-    sl.inStencil = active && slot < numSlots;
+    sl.inStencil = active && slot < w*w*w;
     sl.weight = 0.0;
     sl.node = 0;
     int size[3] = {(int) a.g.nx, (int) a.g.ny, (int) a.g.nz};
@@ -160,17 +204,9 @@ __device__ __forceinline__ StencilSlot interpolateGroup(const S* __restrict__ sr
         int nodeIdx[3];
         sl.weight = 1.0;
         for (int k = 0; k < 3; k++) {
-            double s = wrapCoordinate(x[k], a.dx, size[k]);
-            int ix[4];
-            double wx[4];
-            if (a.stencil == LBM_STENCIL_NEAREST) {
-                ix[0] = nearestIndex(s, size[k]);
-                wx[0] = 1.0;
-            }
-            else
-                axisWeights(a.stencil, w, s, size[k], openAxis[k] != 0, ix, wx);
-            nodeIdx[k] = ix[sub[k]];
-            sl.weight *= wx[sub[k]];
+            double wk;
+            axisSlot<STENCIL>(wrapCoordinate(x[k], a.dx, size[k]), size[k], openAxis[k] != 0, sub[k], nodeIdx[k], wk);
+            sl.weight *= wk;
         }
         sl.node = (uint32_t) (nodeIdx[0] + (int) a.g.nx*(nodeIdx[1] + (int) a.g.ny*nodeIdx[2]));
     }
@@ -197,11 +233,21 @@ __device__ __forceinline__ StencilSlot interpolateGroup(const S* __restrict__ sr
 
 /* The explicit Euler-Maruyama drag in lattice units, F = -gamma m (v - u) + sqrt(2 gamma m kT) xi, with the particle's
    random numbers of the step (keyed by index and step). */
+__device__ __forceinline__ void particleNoise(uint32_t index, const ParticleArgs& a, double noise[4]) {
+    drawNormals<double, 4>(a.seed, TAG_PARTICLE, index, a.step, noise);
+}
+
+/* preset: the particle's normals when already drawn (particleNoise), else nullptr. */
 __device__ __forceinline__ void dragForce(const double* vel, const double u[3], uint32_t index, double m,
-        const ParticleArgs& a, double f[3]) {
+        const ParticleArgs& a, double f[3], const double* preset = nullptr) {
     double noise[4] = {0, 0, 0, 0};
-    if (a.randomForce)
-        drawNormals<double, 4>(a.seed, TAG_PARTICLE, index, a.step, noise);
+    if (a.randomForce) {
+        if (preset != nullptr)
+            for (int k = 0; k < 3; k++)
+                noise[k] = preset[k];
+        else
+            particleNoise(index, a, noise);
+    }
     double amp = sqrt(2.0*a.gamma*m*a.kT);
     for (int k = 0; k < 3; k++) {
         double v = vel[k]/a.velocityScale;
@@ -233,10 +279,29 @@ __device__ __forceinline__ void spreadReaction(const StencilSlot& sl, const doub
         atomicOr(touched + (sl.node >> 5), 1u << (sl.node & 31));
 }
 
+/* The second half of coupleGroup, after the interpolation: the drag force (written by the group's first lane) and,
+   in a lattice step, the spread of the reaction. */
+__device__ __forceinline__ void finishGroup(const StencilSlot& sl, const double u[3], const double* vel, uint32_t index,
+        double m, bool active, int lane, const ParticleArgs& a, double* forceOut,
+        unsigned long long* __restrict__ reactionFixed, uint32_t* __restrict__ touched, double wallAcc[3],
+        const double* noise) {
+    // Every lane of the group holds u; each computes the force (identical values).
+    double f[3] = {0, 0, 0};
+    if (active) {
+        dragForce(vel, u, index, m, a, f, noise);
+        if (lane == 0)
+            for (int k = 0; k < 3; k++)
+                forceOut[k] = f[k]*a.forceScale;
+    }
+    if (a.isStep && sl.inStencil)
+        spreadReaction(sl, f, a, reactionFixed, touched, wallAcc);
+}
+
 /*
  * The coupling of one particle by a group of LANES lanes (one lane per stencil slot; every lane of the group calls
  * it, inactive ones with active = false).  x, v: the particle's position (nm) and velocity (nm/ps); index: its
- * coupled-particle index (the counter of its random numbers); m: its mass in lattice units.  The force (kJ/mol/nm) is
+ * coupled-particle index (the counter of its random numbers); m: its mass in lattice units; noise: its normals of the
+ * step when drawn beforehand (else nullptr: drawn here).  The force (kJ/mol/nm) is
  * written to forceOut by the first lane; in a lattice step the reaction goes to the accumulators and, on solid nodes,
  * into wallAcc (lattice momentum; the host converts the wall partial sums with m_c dx/dt).
  */
@@ -244,19 +309,11 @@ template <typename S, int LANES>
 __device__ __forceinline__ void coupleGroup(const S* __restrict__ src, const uint32_t* __restrict__ mask,
         const double* x, const double* vel, uint32_t index, double m, bool active, int lane,
         const double* __restrict__ ucm, const ParticleArgs& a, double* forceOut,
-        unsigned long long* __restrict__ reactionFixed, uint32_t* __restrict__ touched, double wallAcc[3]) {
+        unsigned long long* __restrict__ reactionFixed, uint32_t* __restrict__ touched, double wallAcc[3],
+        const double* noise = nullptr) {
     double u[3];
     StencilSlot sl = interpolateGroup<S, LANES>(src, mask, x, active, lane, ucm, a, u);
-    // Every lane of the group holds u; each computes the force (identical values).
-    double f[3] = {0, 0, 0};
-    if (active) {
-        dragForce(vel, u, index, m, a, f);
-        if (lane == 0)
-            for (int k = 0; k < 3; k++)
-                forceOut[k] = f[k]*a.forceScale;
-    }
-    if (a.isStep && sl.inStencil)
-        spreadReaction(sl, f, a, reactionFixed, touched, wallAcc);
+    finishGroup(sl, u, vel, index, m, active, lane, a, forceOut, reactionFixed, touched, wallAcc, noise);
 }
 
 }  // namespace lbm

@@ -541,22 +541,22 @@ __device__ __forceinline__ int pushOffset(const Neighbours& nb) {
     return pullOffset<OPP[Q]>(nb);
 }
 
-/* Pull the 19 pre-collision deviations of a node: f_q(x) = post-collision f_q stored at x - c_q. */
-template <typename S>
-__device__ __forceinline__ void pullNode(const S* __restrict__ src, uint32_t n, uint32_t node, const Neighbours& nb,
-        typename StorageTraits<S>::Real f[NQ]) {
-    staticFor<NQ>([&](auto I) {
-        constexpr int q = decltype(I)::value;
-        f[q] = loadPop(src + (size_t) q*n + (uint32_t) ((int) node + pullOffset<q>(nb)));
-    });
-}
-
 /* The element at a 32-bit byte offset from a population plane: the plane's base stays in scalar registers and the
    access takes the offset from one vector register.  Planes are at most 4 GiB (checked at context creation). */
 template <typename T>
 __device__ __forceinline__ T* atByte(T* plane, uint32_t offset) {
     using Byte = typename std::conditional<std::is_const<T>::value, const char, char>::type;
     return reinterpret_cast<T*>(reinterpret_cast<Byte*>(plane) + offset);
+}
+
+/* Pull the 19 pre-collision deviations of a node: f_q(x) = post-collision f_q stored at x - c_q. */
+template <typename S>
+__device__ __forceinline__ void pullNode(const S* __restrict__ src, uint32_t n, uint32_t node, const Neighbours& nb,
+        typename StorageTraits<S>::Real f[NQ]) {
+    staticFor<NQ>([&](auto I) {
+        constexpr int q = decltype(I)::value;
+        f[q] = loadPop(atByte(src + (size_t) q*n, (uint32_t) ((int) node + pullOffset<q>(nb))*(uint32_t) sizeof(S)));
+    });
 }
 
 /* The fused pass's pull: streaming loads at byte offsets node*sizeof(S) + pullOffset*sizeof(S) in each plane. */

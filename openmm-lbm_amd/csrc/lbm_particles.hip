@@ -90,22 +90,46 @@ __global__ void __launch_bounds__(BLOCK) coupleParticles(const S* __restrict__ s
     bool active = i < a.numParticles;
     int part = active ? particles[i] : 0;
     double m = active ? mass[i] : 0.0;
-    double wallAcc[3] = {0, 0, 0};
-    coupleGroup<S, LANES>(src, mask, positions + 3*(size_t) part, velocities + 3*(size_t) part, (uint32_t) i, m, active,
-            lane, ucm, a, particleForce + 3*(size_t) (active ? i : 0), reactionFixed, touched, wallAcc);
-    if (partials != nullptr) {
-        __shared__ double lds[BLOCK][3];
-        for (int k = 0; k < 3; k++)
-            lds[threadIdx.x][k] = wallAcc[k];
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            double s[3] = {0, 0, 0};
-            for (int t = 0; t < BLOCK; t++)
-                for (int k = 0; k < 3; k++)
-                    s[k] += lds[t][k];
-            for (int k = 0; k < 3; k++)
-                partials[3*blockIdx.x + k] = s[k];
+    double u[3];
+    StencilSlot sl = interpolateGroup<S, LANES>(src, mask, positions + 3*(size_t) part, active, lane, ucm, a, u);
+    // The normals of the work-group's particles, drawn once by its first GROUPS lanes instead of by every group: a
+    // wavefront pays for the FP64 Box-Muller once however many of its lanes need it.  Drawn after the interpolation,
+    // so that the first wavefront's loads are in flight meanwhile.
+    __shared__ double noise[LANES > 1 ? GROUPS : 1][4];
+    const double* myNoise = nullptr;
+    if (LANES > 1 && a.randomForce) {
+        if (threadIdx.x < GROUPS) {
+            double r[4] = {0, 0, 0, 0};
+            if (blockIdx.x*GROUPS + threadIdx.x < (unsigned) a.numParticles)
+                particleNoise(blockIdx.x*GROUPS + threadIdx.x, a, r);
+            for (int k = 0; k < 4; k++)
+                noise[threadIdx.x][k] = r[k];
         }
+        __syncthreads();
+        myNoise = noise[threadIdx.x/LANES];
+    }
+    double wallAcc[3] = {0, 0, 0};
+    finishGroup(sl, u, velocities + 3*(size_t) part, (uint32_t) i, m, active, lane, a,
+            particleForce + 3*(size_t) (active ? i : 0), reactionFixed, touched, wallAcc, myNoise);
+    if (partials != nullptr) {
+        // Fixed-order sums: within each wavefront by the butterfly, then the wavefronts in order.
+        __shared__ double waveAcc[BLOCK/64][3];
+        for (int k = 0; k < 3; k++) {
+            double v = wallAcc[k];
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1)
+                v += __shfl_xor(v, off, 64);
+            if ((threadIdx.x & 63) == 0)
+                waveAcc[threadIdx.x >> 6][k] = v;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0)
+            for (int k = 0; k < 3; k++) {
+                double s = 0.0;
+                for (int wv = 0; wv < BLOCK/64; wv++)
+                    s += waveAcc[wv][k];
+                partials[3*blockIdx.x + k] = s;
+            }
     }
 }
 
