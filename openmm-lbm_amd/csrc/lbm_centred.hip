@@ -65,14 +65,6 @@ __device__ __forceinline__ void particleKnowns(const double* vel, const double* 
     }
 }
 
-__device__ __forceinline__ uint32_t nearestNodeOf(const double* x, const ParticleArgs& a) {
-    int size[3] = {(int) a.g.nx, (int) a.g.ny, (int) a.g.nz};
-    int idx[3];
-    for (int k = 0; k < 3; k++)
-        idx[k] = nearestIndex(wrapCoordinate(x[k], a.dx, size[k]), size[k]);
-    return (uint32_t) (idx[0] + (int) a.g.nx*(idx[1] + (int) a.g.ny*idx[2]));
-}
-
 __device__ __forceinline__ void addReaction(uint32_t node, bool solid, const double r[3], const ParticleArgs& a,
         unsigned long long* reactionFixed, uint32_t* touched, double wallAcc[3]) {
     if (solid) {

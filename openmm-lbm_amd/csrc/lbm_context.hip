@@ -147,6 +147,8 @@ struct lbm_context {
     // Centred drag: its solver state and the other forces of the host path.
     CentredState* centred = nullptr;
     double* stageForce = nullptr;
+    // The processing order of the stencil coupling (explicit drag).
+    CoupleOrder* order = nullptr;
     // Timing of the fused pass with HIP events on the context's stream (diagnostics).
     bool profile = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
@@ -330,7 +332,15 @@ void couple(lbm_context* c, const double* dPos, const double* dVel, const double
                 c->closureOod, c->particleForce, c->reactionFixed, c->touched, partials, c->stream);
         return;
     }
-    launchCouple(c->p.storage, c->popA, c->mask, dPos, dVel, c->particles, c->mass, c->ucm, a, c->particleForce,
+    const int* order = nullptr;
+    try {
+        if (c->order != nullptr)
+            order = coupleOrderAt(c->order, step, dPos, c->particles, a, c->stream);
+    }
+    catch (const std::runtime_error& e) {
+        throw Error(LBM_ERROR_RUNTIME, e.what());
+    }
+    launchCouple(c->p.storage, c->popA, c->mask, dPos, dVel, c->particles, order, c->mass, c->ucm, a, c->particleForce,
             c->reactionFixed, c->touched, partials, c->stream);
 }
 
@@ -442,6 +452,7 @@ void destroy(lbm_context* c) {
         (void) hipStreamSynchronize(c->stream);
     clearEvents(c);
     centredDestroy(c->centred);
+    coupleOrderDestroy(c->order);
     void* buffers[] = {c->popA, c->popB, c->mask, c->wallNodes, c->wallLinks, c->boundary, c->reducePartials, c->totals,
                        c->ucm, c->wallPartials, c->particles, c->mass, c->particleForce, c->reactionFixed,
                        c->touched, c->stagePos, c->stageVel, c->closureX, c->closureNode, c->closureU, c->closureDu,
@@ -568,14 +579,16 @@ void create(const lbm_params* params, const int* solid, int numSolid, const int*
         HIP_CHECK(hipMemsetAsync(c->touched, 0, ((n + 31)/32)*sizeof(uint32_t), c->stream));
         c->stagePos = deviceAlloc<double>(3*(size_t) std::max(numSys, 1));
         c->stageVel = deviceAlloc<double>(3*(size_t) std::max(numSys, 1));
-        if (p.drag_scheme == LBM_DRAG_CENTERED) {
+        if (p.drag_scheme == LBM_DRAG_CENTERED)
             c->stageForce = deviceAlloc<double>(3*(size_t) std::max(numSys, 1));
-            try {
+        try {
+            if (p.drag_scheme == LBM_DRAG_CENTERED)
                 c->centred = centredCreate(numParts, p.interpolation_stencil, (uint32_t) n);
-            }
-            catch (const std::runtime_error& e) {
-                throw Error(LBM_ERROR_RUNTIME, e.what());
-            }
+            else
+                c->order = coupleOrderCreate(numParts, (uint32_t) n);
+        }
+        catch (const std::runtime_error& e) {
+            throw Error(LBM_ERROR_RUNTIME, e.what());
         }
     }
     // The fluid at equilibrium with the initial velocity; solid nodes hold no fluid.

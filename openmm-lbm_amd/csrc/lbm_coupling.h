@@ -23,6 +23,15 @@ __device__ __forceinline__ int nearestIndex(double s, int n) {
     return ((int) floor(s + 0.5)) % n;
 }
 
+/* The node nearest to a particle at x (nm). */
+__device__ __forceinline__ uint32_t nearestNodeOf(const double* x, const ParticleArgs& a) {
+    int size[3] = {(int) a.g.nx, (int) a.g.ny, (int) a.g.nz};
+    int idx[3];
+    for (int k = 0; k < 3; k++)
+        idx[k] = nearestIndex(wrapCoordinate(x[k], a.dx, size[k]), size[k]);
+    return (uint32_t) (idx[0] + (int) a.g.nx*(idx[1] + (int) a.g.ny*idx[2]));
+}
+
 /* Kernel phi(r) of the stencils (LBMStencils.h:42-62). */
 __device__ __forceinline__ double stencilKernel(int stencil, double r) {
 #pragma clang fp contract(off)

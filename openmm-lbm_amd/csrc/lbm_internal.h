@@ -115,9 +115,25 @@ void launchReduceWall(const double* partials, int count, double* out, hipStream_
 
 void launchReflect(const double* positions, double* velocities, const int* particles, const double* mass,
         const uint32_t* mask, const ParticleArgs& a, double* partials, hipStream_t stream);
+/* order: the coupled particles in the order to process them (a permutation of 0..numParticles-1), or nullptr. */
 void launchCouple(int storage, const void* src, const uint32_t* mask, const double* positions, const double* velocities,
-        const int* particles, const double* mass, const double* ucm, const ParticleArgs& a, double* particleForce,
-        unsigned long long* reactionFixed, uint32_t* touched, double* partials, hipStream_t stream);
+        const int* particles, const int* order, const double* mass, const double* ucm, const ParticleArgs& a,
+        double* particleForce, unsigned long long* reactionFixed, uint32_t* touched, double* partials,
+        hipStream_t stream);
+
+/*
+ * The coupled particles sorted by nearest node (z slowest): processed in this order, the particles of one work-group
+ * and of neighbouring work-groups read populations from nearby rows of the lattice.  The order changes no result:
+ * each particle's coupling is independent of the others and the reactions add as integers.  It is refreshed every
+ * COUPLE_ORDER_STEPS steps (particles move a small fraction of a node per step).  Throws std::runtime_error.
+ */
+constexpr int COUPLE_ORDER_STEPS = 128;
+struct CoupleOrder;
+CoupleOrder* coupleOrderCreate(int numParticles, uint32_t numNodes);
+void coupleOrderDestroy(CoupleOrder* order);
+/* The order for a coupling at `step`, sorted anew from the positions when stale. */
+const int* coupleOrderAt(CoupleOrder* order, int64_t step, const double* positions, const int* particles,
+        const ParticleArgs& a, hipStream_t stream);
 void launchAddForces(const double* particleForce, const int* particles, int count, double* forces, hipStream_t stream);
 
 /* The centred drag (lbm_centred.hip): its buffers for numParticles coupled particles, and one coupling.  otherForces

@@ -18,6 +18,9 @@
  */
 #include "lbm_coupling.h"
 
+#include <hipcub/hipcub.hpp>
+#include <stdexcept>
+
 namespace lbm {
 
 /* ------------------------------------------------------------------------------------------ */
@@ -82,12 +85,14 @@ __global__ void __launch_bounds__(BLOCK) reflectParticles(const double* __restri
 template <typename S, int LANES>
 __global__ void __launch_bounds__(BLOCK) coupleParticles(const S* __restrict__ src, const uint32_t* __restrict__ mask,
         const double* __restrict__ positions, const double* __restrict__ velocities, const int* __restrict__ particles,
-        const double* __restrict__ mass, const double* __restrict__ ucm, ParticleArgs a, double* __restrict__ particleForce,
-        unsigned long long* __restrict__ reactionFixed, uint32_t* __restrict__ touched, double* __restrict__ partials) {
+        const int* __restrict__ order, const double* __restrict__ mass, const double* __restrict__ ucm, ParticleArgs a,
+        double* __restrict__ particleForce, unsigned long long* __restrict__ reactionFixed, uint32_t* __restrict__ touched,
+        double* __restrict__ partials) {
     constexpr int GROUPS = BLOCK/LANES;
     int lane = threadIdx.x % LANES;
-    int i = blockIdx.x*GROUPS + threadIdx.x/LANES;
-    bool active = i < a.numParticles;
+    int rank = blockIdx.x*GROUPS + threadIdx.x/LANES;    // position in the processing order
+    bool active = rank < a.numParticles;
+    int i = active && order != nullptr ? order[rank] : rank;
     int part = active ? particles[i] : 0;
     double m = active ? mass[i] : 0.0;
     double u[3];
@@ -100,8 +105,9 @@ __global__ void __launch_bounds__(BLOCK) coupleParticles(const S* __restrict__ s
     if (LANES > 1 && a.randomForce) {
         if (threadIdx.x < GROUPS) {
             double r[4] = {0, 0, 0, 0};
-            if (blockIdx.x*GROUPS + threadIdx.x < (unsigned) a.numParticles)
-                particleNoise(blockIdx.x*GROUPS + threadIdx.x, a, r);
+            int k = blockIdx.x*GROUPS + threadIdx.x;
+            if (k < a.numParticles)
+                particleNoise(order != nullptr ? order[k] : k, a, r);
             for (int k = 0; k < 4; k++)
                 noise[threadIdx.x][k] = r[k];
         }
@@ -217,50 +223,52 @@ int coupleBlocks(int stencil, int numParticles) {
 
 template <typename S>
 static void launchCoupleT(const void* src, const uint32_t* mask, const double* positions, const double* velocities,
-        const int* particles, const double* mass, const double* ucm, const ParticleArgs& a, double* particleForce,
-        unsigned long long* reactionFixed, uint32_t* touched, double* partials, hipStream_t stream) {
+        const int* particles, const int* order, const double* mass, const double* ucm, const ParticleArgs& a,
+        double* particleForce, unsigned long long* reactionFixed, uint32_t* touched, double* partials,
+        hipStream_t stream) {
     unsigned grid = (unsigned) coupleBlocks(a.stencil, a.numParticles);
     const S* s = (const S*) src;
     switch (coupleLanes(a.stencil)) {
     case 8:
-        coupleParticles<S, 8><<<grid, BLOCK, 0, stream>>>(s, mask, positions, velocities, particles, mass, ucm, a,
+        coupleParticles<S, 8><<<grid, BLOCK, 0, stream>>>(s, mask, positions, velocities, particles, order, mass, ucm, a,
                 particleForce, reactionFixed, touched, partials);
         break;
     case 32:
-        coupleParticles<S, 32><<<grid, BLOCK, 0, stream>>>(s, mask, positions, velocities, particles, mass, ucm, a,
+        coupleParticles<S, 32><<<grid, BLOCK, 0, stream>>>(s, mask, positions, velocities, particles, order, mass, ucm, a,
                 particleForce, reactionFixed, touched, partials);
         break;
     case 64:
-        coupleParticles<S, 64><<<grid, BLOCK, 0, stream>>>(s, mask, positions, velocities, particles, mass, ucm, a,
+        coupleParticles<S, 64><<<grid, BLOCK, 0, stream>>>(s, mask, positions, velocities, particles, order, mass, ucm, a,
                 particleForce, reactionFixed, touched, partials);
         break;
     default:
-        coupleParticles<S, 1><<<grid, BLOCK, 0, stream>>>(s, mask, positions, velocities, particles, mass, ucm, a,
+        coupleParticles<S, 1><<<grid, BLOCK, 0, stream>>>(s, mask, positions, velocities, particles, order, mass, ucm, a,
                 particleForce, reactionFixed, touched, partials);
         break;
     }
 }
 
 void launchCouple(int storage, const void* src, const uint32_t* mask, const double* positions, const double* velocities,
-        const int* particles, const double* mass, const double* ucm, const ParticleArgs& a, double* particleForce,
-        unsigned long long* reactionFixed, uint32_t* touched, double* partials, hipStream_t stream) {
+        const int* particles, const int* order, const double* mass, const double* ucm, const ParticleArgs& a,
+        double* particleForce, unsigned long long* reactionFixed, uint32_t* touched, double* partials,
+        hipStream_t stream) {
     if (a.numParticles == 0)
         return;
     switch (storage) {
     case LBM_STORAGE_FP16:
-        launchCoupleT<__half>(src, mask, positions, velocities, particles, mass, ucm, a, particleForce, reactionFixed,
+        launchCoupleT<__half>(src, mask, positions, velocities, particles, order, mass, ucm, a, particleForce, reactionFixed,
                 touched, partials, stream);
         break;
     case LBM_STORAGE_BF16:
-        launchCoupleT<__hip_bfloat16>(src, mask, positions, velocities, particles, mass, ucm, a, particleForce,
+        launchCoupleT<__hip_bfloat16>(src, mask, positions, velocities, particles, order, mass, ucm, a, particleForce,
                 reactionFixed, touched, partials, stream);
         break;
     case LBM_STORAGE_FP64:
-        launchCoupleT<double>(src, mask, positions, velocities, particles, mass, ucm, a, particleForce, reactionFixed,
+        launchCoupleT<double>(src, mask, positions, velocities, particles, order, mass, ucm, a, particleForce, reactionFixed,
                 touched, partials, stream);
         break;
     default:
-        launchCoupleT<float>(src, mask, positions, velocities, particles, mass, ucm, a, particleForce, reactionFixed,
+        launchCoupleT<float>(src, mask, positions, velocities, particles, order, mass, ucm, a, particleForce, reactionFixed,
                 touched, partials, stream);
         break;
     }
@@ -300,6 +308,82 @@ void launchAddForces(const double* particleForce, const int* particles, int coun
     if (count == 0)
         return;
     addForces<<<gridFor(count), BLOCK, 0, stream>>>(particleForce, particles, count, forces);
+}
+
+
+/* ------------------------------------------------------------------------------------------ */
+/* Processing order of the coupling.                                                           */
+/* ------------------------------------------------------------------------------------------ */
+struct CoupleOrder {
+    int count = 0;
+    int sortBits = 1;
+    uint32_t* keys = nullptr;
+    uint32_t* sortedKeys = nullptr;
+    int* values = nullptr;
+    int* order = nullptr;
+    void* temp = nullptr;
+    size_t tempBytes = 0;
+    int64_t sortedAt = 0;
+    bool valid = false;
+};
+
+__global__ void __launch_bounds__(BLOCK) orderKeys(const double* __restrict__ positions, const int* __restrict__ particles,
+        ParticleArgs a, uint32_t* __restrict__ keys, int* __restrict__ values) {
+    int i = blockIdx.x*BLOCK + threadIdx.x;
+    if (i >= a.numParticles)
+        return;
+    keys[i] = nearestNodeOf(positions + 3*(size_t) particles[i], a);
+    values[i] = i;
+}
+
+static void orderCheck(hipError_t e, const char* what) {
+    if (e != hipSuccess)
+        throw std::runtime_error(std::string("HIP error in the coupling order (") + what + "): " + hipGetErrorString(e));
+}
+
+void coupleOrderDestroy(CoupleOrder* o) {
+    if (o == nullptr)
+        return;
+    void* buffers[] = {o->keys, o->sortedKeys, o->values, o->order, o->temp};
+    for (void* b : buffers)
+        if (b)
+            (void) hipFree(b);
+    delete o;
+}
+
+CoupleOrder* coupleOrderCreate(int numParticles, uint32_t numNodes) {
+    CoupleOrder* o = new CoupleOrder;
+    try {
+        o->count = numParticles;
+        while (o->sortBits < 32 && (1ull << o->sortBits) < numNodes)
+            o->sortBits++;
+        orderCheck(hipMalloc((void**) &o->keys, sizeof(uint32_t)*numParticles), "allocation");
+        orderCheck(hipMalloc((void**) &o->sortedKeys, sizeof(uint32_t)*numParticles), "allocation");
+        orderCheck(hipMalloc((void**) &o->values, sizeof(int)*numParticles), "allocation");
+        orderCheck(hipMalloc((void**) &o->order, sizeof(int)*numParticles), "allocation");
+        orderCheck(hipcub::DeviceRadixSort::SortPairs(nullptr, o->tempBytes, o->keys, o->sortedKeys, o->values, o->order,
+                numParticles, 0, o->sortBits, (hipStream_t) 0), "sort size");
+        orderCheck(hipMalloc(&o->temp, std::max<size_t>(o->tempBytes, 1)), "allocation");
+    }
+    catch (...) {
+        coupleOrderDestroy(o);
+        throw;
+    }
+    return o;
+}
+
+const int* coupleOrderAt(CoupleOrder* o, int64_t step, const double* positions, const int* particles,
+        const ParticleArgs& a, hipStream_t stream) {
+    if (o->valid && step >= o->sortedAt && step - o->sortedAt < COUPLE_ORDER_STEPS)
+        return o->order;
+    orderKeys<<<gridFor(o->count), BLOCK, 0, stream>>>(positions, particles, a, o->keys, o->values);
+    size_t bytes = o->tempBytes;
+    orderCheck(hipcub::DeviceRadixSort::SortPairs(o->temp, bytes, o->keys, o->sortedKeys, o->values, o->order, o->count,
+            0, o->sortBits, stream), "sort");
+    orderCheck(hipGetLastError(), "launch");
+    o->sortedAt = step;
+    o->valid = true;
+    return o->order;
 }
 
 }  // namespace lbm
