@@ -279,13 +279,15 @@ __device__ __forceinline__ void spreadReaction(const StencilSlot& sl, const doub
         else {
             long long fixed = llrint(r*FIXED_SCALE);
             if (fixed != 0) {
-                atomicAdd(reactionFixed + (size_t) k*a.g.n + sl.node, (unsigned long long) fixed);
+                __hip_atomic_fetch_add(asGlobal(reactionFixed + (size_t) k*a.g.n + sl.node), (unsigned long long) fixed,
+                        __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 any = true;
             }
         }
     }
     if (any)
-        atomicOr(touched + (sl.node >> 5), 1u << (sl.node & 31));
+        __hip_atomic_fetch_or(asGlobal(touched + (sl.node >> 5)), 1u << (sl.node & 31), __ATOMIC_RELAXED,
+                __HIP_MEMORY_SCOPE_AGENT);
 }
 
 /* The second half of coupleGroup, after the interpolation: the drag force (written by the group's first lane) and,

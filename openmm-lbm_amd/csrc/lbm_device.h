@@ -394,14 +394,22 @@ template <> struct StorageTraits<double> { using Real = double; using Compute = 
 template <> struct StorageTraits<__half> { using Real = float; using Compute = float; };
 template <> struct StorageTraits<__hip_bfloat16> { using Real = float; using Compute = float; };
 
+/* p as a pointer to global memory: every population, accumulator and mask lives there.  Accessed through it, a
+   pointer that was read from memory (a table of replicas) still gives global loads and stores (with a scalar base
+   where uniform) instead of generic flat ones. */
+template <typename T>
+__device__ __forceinline__ __attribute__((address_space(1))) T* asGlobal(T* p) {
+    return (__attribute__((address_space(1))) T*) p;
+}
+
 template <typename S>
 __device__ __forceinline__ typename StorageTraits<S>::Real loadPop(const S* p) {
     if constexpr (std::is_same<S, __half>::value)
-        return __half2float(*p);
+        return (float) __builtin_bit_cast(_Float16, *asGlobal(reinterpret_cast<const uint16_t*>(p)));
     else if constexpr (std::is_same<S, __hip_bfloat16>::value)
-        return __uint_as_float(((uint32_t) __builtin_bit_cast(uint16_t, *p)) << 16);
+        return __uint_as_float(((uint32_t) *asGlobal(reinterpret_cast<const uint16_t*>(p))) << 16);
     else
-        return *p;
+        return *asGlobal(p);
 }
 
 /* Integer hash for the stochastic rounding of reduced-precision stores (a murmur3-style finalizer). */
@@ -445,15 +453,15 @@ constexpr bool isReduced() {
 template <typename S>
 __device__ __forceinline__ typename StorageTraits<S>::Real loadPopStream(const S* p) {
     if constexpr (std::is_same<S, __half>::value) {
-        uint16_t bits = __builtin_nontemporal_load(reinterpret_cast<const uint16_t*>(p));
+        uint16_t bits = __builtin_nontemporal_load(asGlobal(reinterpret_cast<const uint16_t*>(p)));
         return (float) __builtin_bit_cast(_Float16, bits);
     }
     else if constexpr (std::is_same<S, __hip_bfloat16>::value) {
-        uint16_t bits = __builtin_nontemporal_load(reinterpret_cast<const uint16_t*>(p));
+        uint16_t bits = __builtin_nontemporal_load(asGlobal(reinterpret_cast<const uint16_t*>(p)));
         return __uint_as_float(((uint32_t) bits) << 16);
     }
     else
-        return __builtin_nontemporal_load(p);
+        return __builtin_nontemporal_load(asGlobal(p));
 }
 
 template <typename S>
@@ -462,17 +470,17 @@ __device__ __forceinline__ void storePopStream(S* p, typename StorageTraits<S>::
         typedef _Float16 half2v __attribute__((ext_vector_type(2)));
         half2v h = {0, 0};
         h = __builtin_amdgcn_cvt_sr_f16_f32(h, v, rnd, false);
-        __builtin_nontemporal_store(__builtin_bit_cast(uint16_t, h[0]), reinterpret_cast<uint16_t*>(p));
+        __builtin_nontemporal_store(__builtin_bit_cast(uint16_t, h[0]), asGlobal(reinterpret_cast<uint16_t*>(p)));
     }
     else if constexpr (std::is_same<S, __hip_bfloat16>::value) {
         typedef __bf16 bf2v __attribute__((ext_vector_type(2)));
         bf2v h = {0, 0};
         h = __builtin_amdgcn_cvt_sr_bf16_f32(h, v, rnd, false);
-        __builtin_nontemporal_store(__builtin_bit_cast(uint16_t, h[0]), reinterpret_cast<uint16_t*>(p));
+        __builtin_nontemporal_store(__builtin_bit_cast(uint16_t, h[0]), asGlobal(reinterpret_cast<uint16_t*>(p)));
     }
     else {
         (void) rnd;
-        __builtin_nontemporal_store(v, p);
+        __builtin_nontemporal_store(v, asGlobal(p));
     }
 }
 

@@ -25,9 +25,9 @@ __device__ __forceinline__ void fusedNode(const S* __restrict__ src, S* __restri
     if constexpr (REACTION) {
         // The word of the touched mask covers 32 nodes of this wave: every lane reads it, then its first lane
         // (always inside the lattice, since words are 32-aligned) clears it for the next step.
-        touchedWord = touched[node >> 5];
+        touchedWord = *asGlobal(touched + (node >> 5));
         if ((node & 31) == 0 && touchedWord != 0)
-            touched[node >> 5] = 0;
+            *asGlobal(touched + (node >> 5)) = 0;
     }
     if constexpr (SOLIDS)
         if (isSolid(mask, node))
@@ -58,7 +58,7 @@ __device__ __forceinline__ void fusedNode(const S* __restrict__ src, S* __restri
             C r[3];
 #pragma unroll
             for (int c = 0; c < 3; c++) {
-                unsigned long long* p = reactionFixed + (size_t) c*a.g.n + node;
+                auto p = asGlobal(reactionFixed + (size_t) c*a.g.n + node);
                 r[c] = C((double) (long long) *p*(1.0/FIXED_SCALE));
                 *p = 0ull;
             }

@@ -157,10 +157,26 @@ __global__ void __launch_bounds__(BLOCK) coupleBatch(const ReplicaDesc* __restri
     a.step = *stepCounter;
     a.isStep = 1;
     const float* src = (a.step & 1) ? d->popB : d->popA;
+    double u[3];
+    StencilSlot sl = interpolateGroup<float, LANES>(src, nullptr, positions + 3*(size_t) pp, active, lane, nullptr, a, u);
+    // The work-group's normals, drawn once by its first GROUPS lanes (as in coupleParticles).
+    __shared__ double noise[LANES > 1 ? GROUPS : 1][4];
+    const double* myNoise = nullptr;
+    if (LANES > 1 && a.randomForce) {
+        if (threadIdx.x < GROUPS) {
+            double r[4] = {0, 0, 0, 0};
+            int k = (int) (blockIdx.x - d->particleTileBegin)*GROUPS + threadIdx.x;
+            if (k < a.numParticles)
+                particleNoise((uint32_t) k, a, r);
+            for (int c = 0; c < 4; c++)
+                noise[threadIdx.x][c] = r[c];
+        }
+        __syncthreads();
+        myNoise = noise[threadIdx.x/LANES];
+    }
     double wallAcc[3] = {0, 0, 0};
-    coupleGroup<float, LANES>(src, nullptr, positions + 3*(size_t) pp, velocities + 3*(size_t) pp,
-            (uint32_t) (pp - d->particleBegin), active ? mass[pp] : 0.0, active, lane, nullptr, a, forces + 3*(size_t) pp,
-            d->reactionFixed, d->touched, wallAcc);
+    finishGroup(sl, u, velocities + 3*(size_t) pp, (uint32_t) (pp - d->particleBegin), active ? mass[pp] : 0.0, active,
+            lane, a, forces + 3*(size_t) pp, d->reactionFixed, d->touched, wallAcc, myNoise);
 }
 
 /* Leapfrog Verlet of the batch (the coupling force is the only force).  The step counter is advanced by advanceStep,
