@@ -85,20 +85,8 @@ __device__ __forceinline__ void addReaction(uint32_t node, bool solid, const dou
 }
 
 __device__ __forceinline__ void blockPartials(const double acc[3], double* partials) {
-    if (partials == nullptr)
-        return;
-    __shared__ double lds[BLOCK][3];
-    for (int k = 0; k < 3; k++)
-        lds[threadIdx.x][k] = acc[k];
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double s[3] = {0, 0, 0};
-        for (int t = 0; t < BLOCK; t++)
-            for (int k = 0; k < 3; k++)
-                s[k] += lds[t][k];
-        for (int k = 0; k < 3; k++)
-            partials[3*blockIdx.x + k] = s[k];
-    }
+    if (partials != nullptr)
+        storeWallPartial<BLOCK>(acc, partials);
 }
 
 /* ---------------------------------------------------------------------------------------------------------------- */

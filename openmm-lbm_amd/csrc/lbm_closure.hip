@@ -218,20 +218,8 @@ __global__ void __launch_bounds__(BLOCK) closureFallback(const S* __restrict__ s
                 acc[k] += wallAcc[k];
         }
     }
-    if (partials != nullptr) {
-        __shared__ double lds[BLOCK][3];
-        for (int k = 0; k < 3; k++)
-            lds[threadIdx.x][k] = acc[k];
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            double s[3] = {0, 0, 0};
-            for (int t = 0; t < BLOCK; t++)
-                for (int k = 0; k < 3; k++)
-                    s[k] += lds[t][k];
-            for (int k = 0; k < 3; k++)
-                partials[3*blockIdx.x + k] = s[k];
-        }
-    }
+    if (partials != nullptr)
+        storeWallPartial<BLOCK>(acc, partials);
 }
 
 unsigned gridOf(int64_t items, int perBlock) {

@@ -414,13 +414,14 @@ void beginStepDevice(lbm_context* c, int64_t step, const double* dPos, double* d
     c->stepPending = true;
     c->wallStepDone = false;
     c->wallOverride = false;
-    if (c->wallPartialCount > 0)
-        launchFill(LBM_STORAGE_FP64, c->wallPartials, 3*(size_t) c->wallPartialCount, c->stream);
     if (c->mask != nullptr && c->numParticles > 0) {
+        // The reflection also clears the step's other wall partial sums.
         ParticleArgs a = particleArgs(c, step, true);
-        launchReflect(dPos, dVel, c->particles, c->mass, c->mask, a, c->wallPartials + 3*(size_t) c->wallReflectOffset,
-                c->stream);
+        launchReflect(dPos, dVel, c->particles, c->mass, c->mask, a, c->wallPartials, c->wallPartialCount,
+                c->wallReflectOffset, c->stream);
     }
+    else if (c->wallPartialCount > 0)
+        launchFill(LBM_STORAGE_FP64, c->wallPartials, 3*(size_t) c->wallPartialCount, c->stream);
     HIP_CHECK(hipGetLastError());
 }
 

@@ -290,6 +290,30 @@ __device__ __forceinline__ void spreadReaction(const StencilSlot& sl, const doub
                 __HIP_MEMORY_SCOPE_AGENT);
 }
 
+/* The wall momentum acc of the work-group's threads, summed into partials[3*blockIdx.x + k] in a fixed order (the
+   butterfly within each wavefront, then the wavefronts in order), so that runs are reproducible.  Every thread of the
+   work-group calls it. */
+template <int THREADS>
+__device__ __forceinline__ void storeWallPartial(const double acc[3], double* __restrict__ partials) {
+    __shared__ double waveAcc[THREADS/64][3];
+    for (int k = 0; k < 3; k++) {
+        double v = acc[k];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1)
+            v += __shfl_xor(v, off, 64);
+        if ((threadIdx.x & 63) == 0)
+            waveAcc[threadIdx.x >> 6][k] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 3; k++) {
+            double s = 0.0;
+            for (int w = 0; w < THREADS/64; w++)
+                s += waveAcc[w][k];
+            partials[3*blockIdx.x + k] = s;
+        }
+}
+
 /* The second half of coupleGroup, after the interpolation: the drag force (written by the group's first lane) and,
    in a lattice step, the spread of the reaction. */
 __device__ __forceinline__ void finishGroup(const StencilSlot& sl, const double u[3], const double* vel, uint32_t index,

@@ -113,8 +113,9 @@ void launchInitEquilibrium(int storage, void* dst, const uint32_t* mask, const G
 void launchFill(int storage, void* dst, size_t count, hipStream_t stream);
 void launchReduceWall(const double* partials, int count, double* out, hipStream_t stream);
 
+/* The reflection writes its wall momentum to allPartials[3*ownOffset...] and clears the rest of allPartials[0, 3*count). */
 void launchReflect(const double* positions, double* velocities, const int* particles, const double* mass,
-        const uint32_t* mask, const ParticleArgs& a, double* partials, hipStream_t stream);
+        const uint32_t* mask, const ParticleArgs& a, double* allPartials, int count, int ownOffset, hipStream_t stream);
 /* order: the coupled particles in the order to process them (a permutation of 0..numParticles-1), or nullptr. */
 void launchCouple(int storage, const void* src, const uint32_t* mask, const double* positions, const double* velocities,
         const int* particles, const int* order, const double* mass, const double* ucm, const ParticleArgs& a,

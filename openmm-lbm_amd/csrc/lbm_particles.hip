@@ -24,9 +24,15 @@
 namespace lbm {
 
 /* ------------------------------------------------------------------------------------------ */
+/* Also clears the other wall partial sums of the step, allPartials[0, 3*count) outside this kernel's own slots (the
+   kernels that write them run later on the stream). */
 __global__ void __launch_bounds__(BLOCK) reflectParticles(const double* __restrict__ positions, double* __restrict__ velocities,
         const int* __restrict__ particles, const double* __restrict__ mass, const uint32_t* __restrict__ mask,
-        ParticleArgs a, double* __restrict__ partials) {
+        ParticleArgs a, double* __restrict__ allPartials, int count, int ownOffset) {
+    double* partials = allPartials + 3*(size_t) ownOffset;
+    for (int j = blockIdx.x*BLOCK + threadIdx.x; j < 3*count; j += gridDim.x*BLOCK)
+        if (j < 3*ownOffset || j >= 3*(ownOffset + (int) gridDim.x))
+            allPartials[j] = 0.0;
     double acc[3] = {0, 0, 0};
     int i = blockIdx.x*BLOCK + threadIdx.x;
     if (i < a.numParticles) {
@@ -67,18 +73,7 @@ __global__ void __launch_bounds__(BLOCK) reflectParticles(const double* __restri
             }
         }
     }
-    __shared__ double lds[BLOCK][3];
-    for (int k = 0; k < 3; k++)
-        lds[threadIdx.x][k] = acc[k];
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double s[3] = {0, 0, 0};
-        for (int t = 0; t < BLOCK; t++)
-            for (int k = 0; k < 3; k++)
-                s[k] += lds[t][k];
-        for (int k = 0; k < 3; k++)
-            partials[3*blockIdx.x + k] = s[k];
-    }
+    storeWallPartial<BLOCK>(acc, partials);
 }
 
 /* ------------------------------------------------------------------------------------------ */
@@ -117,26 +112,8 @@ __global__ void __launch_bounds__(BLOCK) coupleParticles(const S* __restrict__ s
     double wallAcc[3] = {0, 0, 0};
     finishGroup(sl, u, velocities + 3*(size_t) part, (uint32_t) i, m, active, lane, a,
             particleForce + 3*(size_t) (active ? i : 0), reactionFixed, touched, wallAcc, myNoise);
-    if (partials != nullptr) {
-        // Fixed-order sums: within each wavefront by the butterfly, then the wavefronts in order.
-        __shared__ double waveAcc[BLOCK/64][3];
-        for (int k = 0; k < 3; k++) {
-            double v = wallAcc[k];
-#pragma unroll
-            for (int off = 32; off > 0; off >>= 1)
-                v += __shfl_xor(v, off, 64);
-            if ((threadIdx.x & 63) == 0)
-                waveAcc[threadIdx.x >> 6][k] = v;
-        }
-        __syncthreads();
-        if (threadIdx.x == 0)
-            for (int k = 0; k < 3; k++) {
-                double s = 0.0;
-                for (int wv = 0; wv < BLOCK/64; wv++)
-                    s += waveAcc[wv][k];
-                partials[3*blockIdx.x + k] = s;
-            }
-    }
+    if (partials != nullptr)
+        storeWallPartial<BLOCK>(wallAcc, partials);
 }
 
 /* Batched replicas: the coupling of every particle of every replica in one launch.  Each work-group couples up to
@@ -217,10 +194,11 @@ static inline unsigned gridFor(uint64_t n) {
 }
 
 void launchReflect(const double* positions, double* velocities, const int* particles, const double* mass,
-        const uint32_t* mask, const ParticleArgs& a, double* partials, hipStream_t stream) {
+        const uint32_t* mask, const ParticleArgs& a, double* allPartials, int count, int ownOffset, hipStream_t stream) {
     if (a.numParticles == 0)
         return;
-    reflectParticles<<<gridFor(a.numParticles), BLOCK, 0, stream>>>(positions, velocities, particles, mass, mask, a, partials);
+    reflectParticles<<<gridFor(a.numParticles), BLOCK, 0, stream>>>(positions, velocities, particles, mass, mask, a,
+            allPartials, count, ownOffset);
 }
 
 int coupleLanes(int stencil) {
