@@ -123,8 +123,8 @@ void launchCouple(int storage, const void* src, const uint32_t* mask, const doub
         hipStream_t stream);
 
 /*
- * The coupled particles sorted by nearest node (z slowest): processed in this order, the particles of one work-group
- * and of neighbouring work-groups read populations from nearby rows of the lattice.  The order changes no result:
+ * The coupled particles sorted by nearest node (z slowest, descending): processed in this order, the particles of one
+ * work-group and of neighbouring work-groups read populations from nearby rows of the lattice.  The order changes no result:
  * each particle's coupling is independent of the others and the reactions add as integers.  It is refreshed every
  * COUPLE_ORDER_STEPS steps (particles move a small fraction of a node per step).  Throws std::runtime_error.
  */

