@@ -585,7 +585,10 @@ void create(const lbm_params* params, const int* solid, int numSolid, const int*
         try {
             if (p.drag_scheme == LBM_DRAG_CENTERED)
                 c->centred = centredCreate(numParts, p.interpolation_stencil, (uint32_t) n);
-            else
+            else if ((uint64_t) numParts*c->numSlots*4 <= n)
+                // Sparse particles gain DRAM locality from the node order; where the stencils cover more than a quarter
+                // of the lattice, neighbours processed together contend for the same accumulators (measured: 100,000
+                // chain beads with Keys on 128^3 couple 6% slower in node order), so they keep the index order.
                 c->order = coupleOrderCreate(numParts, (uint32_t) n);
         }
         catch (const std::runtime_error& e) {
