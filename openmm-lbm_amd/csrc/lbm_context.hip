@@ -131,7 +131,7 @@ struct lbm_context {
     std::vector<int> hostParticles;
     double* particleForce = nullptr;
     std::vector<double> hostForce;
-    unsigned long long* reactionFixed = nullptr;   // 3N fixed-point accumulators of the particle reactions
+    unsigned long long* reactionFixed = nullptr;   // [node][3] fixed-point accumulators of the particle reactions
     uint32_t* touched = nullptr;                   // one bit per node: accumulators holding a reaction
     double* stagePos = nullptr;
     double* stageVel = nullptr;
