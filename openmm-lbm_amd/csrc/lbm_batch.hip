@@ -237,6 +237,8 @@ void createBatch(lbm_batch* b, int R, const lbm_params* params, const int* numPa
         d.step.step = 0;
         d.step.mu = 3.0*p.fluid_kT*p.dt*p.dt/(cellMass*p.dx*p.dx);
         d.step.roundSeed = 0;
+        int w = stencilWidth(p.interpolation_stencil);
+        d.step.reactionPlanes = denseCoupling((uint64_t) numParticles[r], w*w*w, (uint64_t) g.n) ? 1 : 0;
         ParticleArgs& a = d.part;
         a.g = g;
         a.numParticles = numParticles[r];
@@ -253,6 +255,7 @@ void createBatch(lbm_batch* b, int R, const lbm_params* params, const int* numPa
         a.seed = seed;
         a.step = 0;
         a.hasSolids = 0;
+        a.reactionPlanes = d.step.reactionPlanes;
         d.dt = p.dt;
         d.particleBegin = particleBegin;
         size_t n = g.n;

@@ -76,7 +76,7 @@ __device__ __forceinline__ void addReaction(uint32_t node, bool solid, const dou
     for (int k = 0; k < 3; k++) {
         long long fixed = llrint(r[k]*FIXED_SCALE);
         if (fixed != 0) {
-            atomicAdd(reactionFixed + 3*(size_t) node + k, (unsigned long long) fixed);
+            atomicAdd(reactionFixed + reactionIndex(node, k, a.g.n, a.reactionPlanes), (unsigned long long) fixed);
             any = true;
         }
     }

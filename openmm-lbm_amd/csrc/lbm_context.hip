@@ -126,6 +126,7 @@ struct lbm_context {
     double wallOverrideValue[3] = {0, 0, 0};
     // Particles.
     int numParticles = 0, numSystem = 0, numSlots = 0;
+    bool dense = false;                   // denseCoupling(): index order, component-plane reactions
     int* particles = nullptr;
     double* mass = nullptr;               // lattice units m/m_c
     std::vector<int> hostParticles;
@@ -278,6 +279,7 @@ StepArgs stepArgs(lbm_context* c, int64_t step) {
     a.step = step;
     a.mu = 3.0*c->p.fluid_kT*c->p.dt*c->p.dt/(c->cellMass()*c->p.dx*c->p.dx);
     a.roundSeed = (uint32_t) (c->seed*0x9E3779B9u) ^ (uint32_t) (step*0x85EBCA6Bll) ^ (uint32_t) ((uint64_t) step >> 32);
+    a.reactionPlanes = c->dense ? 1 : 0;
     return a;
 }
 
@@ -298,6 +300,7 @@ ParticleArgs particleArgs(lbm_context* c, int64_t step, bool isStep) {
     a.seed = c->seed;
     a.step = step;
     a.hasSolids = c->mask != nullptr;
+    a.reactionPlanes = c->dense ? 1 : 0;
     return a;
 }
 
@@ -547,6 +550,7 @@ void create(const lbm_params* params, const int* solid, int numSolid, const int*
     c->numSystem = numSys;
     int w = stencilWidth(p.interpolation_stencil);
     c->numSlots = w*w*w;
+    c->dense = denseCoupling((uint64_t) numParts, c->numSlots, n);
     auto blocks = [](int count) { return (count + BLOCK - 1)/BLOCK; };
     int coupleGroups = BLOCK/(p.interpolation_stencil == LBM_STENCIL_KEYS ? 64 : p.interpolation_stencil ==
             LBM_STENCIL_THREE_POINT ? 32 : p.interpolation_stencil == LBM_STENCIL_TRILINEAR ? 8 : 1);
@@ -585,10 +589,10 @@ void create(const lbm_params* params, const int* solid, int numSolid, const int*
         try {
             if (p.drag_scheme == LBM_DRAG_CENTERED)
                 c->centred = centredCreate(numParts, p.interpolation_stencil, (uint32_t) n);
-            else if ((uint64_t) numParts*c->numSlots*4 <= n)
-                // Sparse particles gain DRAM locality from the node order; where the stencils cover more than a quarter
-                // of the lattice, neighbours processed together contend for the same accumulators (measured: 100,000
-                // chain beads with Keys on 128^3 couple 6% slower in node order), so they keep the index order.
+            else if (!c->dense)
+                // Sparse particles gain DRAM locality from the node order; dense ones would contend for the same
+                // accumulators when neighbours are processed together (measured: 100,000 chain beads with Keys on
+                // 128^3 couple 6% slower in node order), so they keep the index order.
                 c->order = coupleOrderCreate(numParts, (uint32_t) n);
         }
         catch (const std::runtime_error& e) {

@@ -279,7 +279,8 @@ __device__ __forceinline__ void spreadReaction(const StencilSlot& sl, const doub
         else {
             long long fixed = llrint(r*FIXED_SCALE);
             if (fixed != 0) {
-                __hip_atomic_fetch_add(asGlobal(reactionFixed + 3*(size_t) sl.node + k), (unsigned long long) fixed,
+                __hip_atomic_fetch_add(asGlobal(reactionFixed + reactionIndex(sl.node, k, a.g.n, a.reactionPlanes)),
+                        (unsigned long long) fixed,
                         __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 any = true;
             }

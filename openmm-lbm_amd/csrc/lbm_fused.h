@@ -58,7 +58,7 @@ __device__ __forceinline__ void fusedNode(const S* __restrict__ src, S* __restri
             C r[3];
 #pragma unroll
             for (int c = 0; c < 3; c++) {
-                auto p = asGlobal(reactionFixed + 3*(size_t) node + c);
+                auto p = asGlobal(reactionFixed + reactionIndex(node, c, a.g.n, a.reactionPlanes));
                 r[c] = C((double) (long long) *p*(1.0/FIXED_SCALE));
                 *p = 0ull;
             }
