@@ -27,8 +27,9 @@ typedef struct lbm_batch lbm_batch;
 int lbm_batch_create(int num_replicas, const lbm_params* params, const int* num_particles, const double* masses,
         const double* positions, const double* velocities, lbm_batch** out);
 
-/* Time each work-group size of the fused pass over a few steps (the state advances) and keep the fastest.
-   times_ms (3 entries, may be NULL) receives the time per step of 256, 512 and 1024 threads; *best the choice. */
+/* Time each work-group size of the fused pass in three interleaved rounds of steps_per_trial steps (the state
+   advances) and keep the fastest.  times_ms (3 entries, may be NULL) receives the best round's time per step of 256,
+   512 and 1024 threads; *best the choice. */
 int lbm_batch_autotune(lbm_batch* batch, int steps_per_trial, double* times_ms, int* best);
 
 /* Advance every replica by num_steps steps (asynchronous; replays the captured graph). */

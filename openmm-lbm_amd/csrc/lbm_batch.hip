@@ -352,28 +352,35 @@ int lbm_batch_create(int R, const lbm_params* params, const int* numParticles, c
 
 int lbm_batch_autotune(lbm_batch* b, int stepsPerTrial, double* timesMs, int* best) {
     BATCH_TRY({
+        // The candidates are timed in interleaved rounds and each keeps its best round, so that drift of the clocks
+        // or of the memory system during the trials does not favour one of them.
         const int candidates[3] = {256, 512, 1024};
-        double bestTime = 1e300;
-        int bestThreads = b->threads;
+        const int rounds = 3;
+        double times[3] = {1e300, 1e300, 1e300};
         hipEvent_t e0, e1;
         hipCheck(hipEventCreate(&e0), "event");
         hipCheck(hipEventCreate(&e1), "event");
         int trials = std::max(stepsPerTrial, STEPS_PER_GRAPH);
+        for (int r = 0; r < rounds; r++)
+            for (int c = 0; c < 3; c++) {
+                buildTiles(b, candidates[c]);
+                buildGraph(b);
+                stepBatch(b, STEPS_PER_GRAPH);             // warm-up
+                hipCheck(hipEventRecord(e0, b->stream), "event");
+                stepBatch(b, trials);
+                hipCheck(hipEventRecord(e1, b->stream), "event");
+                hipCheck(hipEventSynchronize(e1), "event");
+                float ms = 0;
+                hipCheck(hipEventElapsedTime(&ms, e0, e1), "event");
+                times[c] = std::min(times[c], (double) ms/trials);
+            }
+        int bestThreads = candidates[0];
+        double bestTime = times[0];
         for (int c = 0; c < 3; c++) {
-            buildTiles(b, candidates[c]);
-            buildGraph(b);
-            stepBatch(b, STEPS_PER_GRAPH);             // warm-up
-            hipCheck(hipEventRecord(e0, b->stream), "event");
-            stepBatch(b, trials);
-            hipCheck(hipEventRecord(e1, b->stream), "event");
-            hipCheck(hipEventSynchronize(e1), "event");
-            float ms = 0;
-            hipCheck(hipEventElapsedTime(&ms, e0, e1), "event");
-            double perStep = ms/trials;
             if (timesMs)
-                timesMs[c] = perStep;
-            if (perStep < bestTime) {
-                bestTime = perStep;
+                timesMs[c] = times[c];
+            if (times[c] < bestTime) {
+                bestTime = times[c];
                 bestThreads = candidates[c];
             }
         }

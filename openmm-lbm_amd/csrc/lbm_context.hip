@@ -150,6 +150,10 @@ struct lbm_context {
     double* stageForce = nullptr;
     // The processing order of the stencil coupling (explicit drag).
     CoupleOrder* order = nullptr;
+    // Set by lbm_execute_device for the duration of one call: the caller's force array, into which the stencil coupling
+    // adds the forces of the coupled particles itself (forcesAdded then says so, and no separate addForces runs).
+    double* forceTarget = nullptr;
+    bool forcesAdded = false;
     // Timing of the fused pass with HIP events on the context's stream (diagnostics).
     bool profile = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
@@ -344,7 +348,8 @@ void couple(lbm_context* c, const double* dPos, const double* dVel, const double
         throw Error(LBM_ERROR_RUNTIME, e.what());
     }
     launchCouple(c->p.storage, c->popA, c->mask, dPos, dVel, c->particles, order, c->mass, c->ucm, a, c->particleForce,
-            c->reactionFixed, c->touched, partials, c->stream);
+            c->reactionFixed, c->touched, partials, c->forceTarget, c->stream);
+    c->forcesAdded = c->forceTarget != nullptr;
 }
 
 void advanceFluid(lbm_context* c, const double* dPos, const double* dVel, const double* dForces) {
@@ -726,8 +731,14 @@ int lbm_execute_device(lbm_context* c, int64_t step, int includeForces, const do
         double* forces) {
     LBM_CHECK_CTX(c);
     LBM_TRY({
+        struct TargetScope {
+            lbm_context* c;
+            ~TargetScope() { c->forceTarget = nullptr; }
+        } scope{c};
+        c->forceTarget = includeForces ? forces : nullptr;
+        c->forcesAdded = false;
         executeDevice(c, step, includeForces != 0, positions, velocities, forces);
-        if (includeForces && forces)
+        if (includeForces && forces && !c->forcesAdded)
             launchAddForces(c->particleForce, c->particles, c->numParticles, forces, c->stream);
         HIP_CHECK(hipGetLastError());
     })

@@ -320,14 +320,19 @@ __device__ __forceinline__ void storeWallPartial(const double acc[3], double* __
 __device__ __forceinline__ void finishGroup(const StencilSlot& sl, const double u[3], const double* vel, uint32_t index,
         double m, bool active, int lane, const ParticleArgs& a, double* forceOut,
         unsigned long long* __restrict__ reactionFixed, uint32_t* __restrict__ touched, double wallAcc[3],
-        const double* noise) {
+        const double* noise, double* systemForce = nullptr) {
     // Every lane of the group holds u; each computes the force (identical values).
     double f[3] = {0, 0, 0};
     if (active) {
         dragForce(vel, u, index, m, a, f, noise);
         if (lane == 0)
-            for (int k = 0; k < 3; k++)
+            for (int k = 0; k < 3; k++) {
                 forceOut[k] = f[k]*a.forceScale;
+                // The caller's force array, when given: one addition per coupled particle (atomic, in case a particle
+                // were coupled twice).
+                if (systemForce != nullptr)
+                    atomicAdd(systemForce + k, f[k]*a.forceScale);
+            }
     }
     if (a.isStep && sl.inStencil)
         spreadReaction(sl, f, a, reactionFixed, touched, wallAcc);

@@ -35,47 +35,44 @@ struct MD {
 
 thread_local std::string g_err;
 
-__global__ void zeroForces(double* f, int count) {
-    int i = blockIdx.x*256 + threadIdx.x;
-    if (i < count)
-        f[i] = 0.0;
-}
-
-__global__ void wallForces(const double* pos, double* f, int n, double eps, double sigma, double zlo, double zhi) {
+/* The forces of the System other than the LBMForce, written (not added) for every particle: the repulsive walls
+   U = eps[(s/d1)^12 + (s/d2)^12] along z (when set) plus the harmonic bonds to the (at most two) chain partners, in
+   that order. */
+__global__ void otherForces(const double* pos, double* f, int n, int wall, double eps, double sigma, double zlo,
+        double zhi, const int* partners, double k, double r0, double bx, double by, double bz) {
     int i = blockIdx.x*256 + threadIdx.x;
     if (i >= n)
         return;
-    double z = pos[3*(size_t) i + 2];
-    double d1 = z - zlo, d2 = zhi - z;
-    // -dU/dz with U = eps[(s/d1)^12 + (s/d2)^12]: 12 eps s^12/d1^13 - 12 eps s^12/d2^13.
-    double s1 = sigma/d1, s2 = sigma/d2;
-    double s1_12 = s1*s1*s1*s1*s1*s1*s1*s1*s1*s1*s1*s1, s2_12 = s2*s2*s2*s2*s2*s2*s2*s2*s2*s2*s2*s2;
-    f[3*(size_t) i + 2] += 12.0*eps*(s1_12/d1 - s2_12/d2);
-}
-
-__global__ void bondForces(const double* pos, double* f, const int* partners, int n, double k, double r0,
-        double bx, double by, double bz) {
-    // One thread per particle sums the harmonic bonds to its (at most two) chain partners, in a fixed order.
-    int i = blockIdx.x*256 + threadIdx.x;
-    if (i >= n)
-        return;
-    double box[3] = {bx, by, bz}, acc[3] = {0, 0, 0};
-    for (int p = 0; p < 2; p++) {
-        int j = partners[2*i + p];
-        if (j < 0)
-            continue;
-        double d[3];
-        for (int a = 0; a < 3; a++) {
-            d[a] = pos[3*(size_t) j + a] - pos[3*(size_t) i + a];
-            d[a] -= box[a]*rint(d[a]/box[a]);
+    double acc[3] = {0, 0, 0};
+    if (wall) {
+        double z = pos[3*(size_t) i + 2];
+        double d1 = z - zlo, d2 = zhi - z;
+        // -dU/dz = 12 eps s^12/d1^13 - 12 eps s^12/d2^13.
+        double s1 = sigma/d1, s2 = sigma/d2;
+        double s1_12 = s1*s1*s1*s1*s1*s1*s1*s1*s1*s1*s1*s1, s2_12 = s2*s2*s2*s2*s2*s2*s2*s2*s2*s2*s2*s2;
+        acc[2] = 12.0*eps*(s1_12/d1 - s2_12/d2);
+    }
+    if (partners != nullptr) {
+        double box[3] = {bx, by, bz}, b[3] = {0, 0, 0};
+        for (int p = 0; p < 2; p++) {
+            int j = partners[2*i + p];
+            if (j < 0)
+                continue;
+            double d[3];
+            for (int a = 0; a < 3; a++) {
+                d[a] = pos[3*(size_t) j + a] - pos[3*(size_t) i + a];
+                d[a] -= box[a]*rint(d[a]/box[a]);
+            }
+            double r = sqrt(d[0]*d[0] + d[1]*d[1] + d[2]*d[2]);
+            double s = r > 0 ? k*(r - r0)/r : 0.0;
+            for (int a = 0; a < 3; a++)
+                b[a] += s*d[a];
         }
-        double r = sqrt(d[0]*d[0] + d[1]*d[1] + d[2]*d[2]);
-        double s = r > 0 ? k*(r - r0)/r : 0.0;
         for (int a = 0; a < 3; a++)
-            acc[a] += s*d[a];
+            acc[a] += b[a];
     }
     for (int a = 0; a < 3; a++)
-        f[3*(size_t) i + a] += acc[a];
+        f[3*(size_t) i + a] = acc[a];
 }
 
 __global__ void verlet(double* pos, double* vel, const double* f, const double* invMass, int n, double dt) {
@@ -190,12 +187,9 @@ int lbm_md_step(lbm_md_system* s, lbm_context* ctx, int64_t step0, int numSteps)
             g_err = lbm_last_error();
             return status;
         }
-        zeroForces<<<(3*md->n + 255)/256, 256, 0, stream>>>(md->force, 3*md->n);
-        if (md->wall)
-            wallForces<<<grid, 256, 0, stream>>>(md->pos, md->force, md->n, md->eps, md->sigma, md->zlo, md->zhi);
-        if (md->numBonds > 0)
-            bondForces<<<grid, 256, 0, stream>>>(md->pos, md->force, md->bonds, md->n, md->bondK, md->bondR0,
-                    md->box[0], md->box[1], md->box[2]);
+        otherForces<<<grid, 256, 0, stream>>>(md->pos, md->force, md->n, md->wall ? 1 : 0, md->eps, md->sigma, md->zlo,
+                md->zhi, md->numBonds > 0 ? md->bonds : nullptr, md->bondK, md->bondR0, md->box[0], md->box[1],
+                md->box[2]);
         status = lbm_execute_device(ctx, step, 1, md->pos, md->vel, md->force);
         if (status != LBM_OK) {
             g_err = lbm_last_error();

@@ -134,11 +134,12 @@ void launchReduceWall(const double* partials, int count, double* out, hipStream_
 /* The reflection writes its wall momentum to allPartials[3*ownOffset...] and clears the rest of allPartials[0, 3*count). */
 void launchReflect(const double* positions, double* velocities, const int* particles, const double* mass,
         const uint32_t* mask, const ParticleArgs& a, double* allPartials, int count, int ownOffset, hipStream_t stream);
-/* order: the coupled particles in the order to process them (a permutation of 0..numParticles-1), or nullptr. */
+/* order: the coupled particles in the order to process them (a permutation of 0..numParticles-1), or nullptr;
+   systemForces: the caller's force array [particle][3] (kJ/mol/nm) to add the coupling forces to, or nullptr. */
 void launchCouple(int storage, const void* src, const uint32_t* mask, const double* positions, const double* velocities,
         const int* particles, const int* order, const double* mass, const double* ucm, const ParticleArgs& a,
         double* particleForce, unsigned long long* reactionFixed, uint32_t* touched, double* partials,
-        hipStream_t stream);
+        double* systemForces, hipStream_t stream);
 
 /*
  * The coupled particles sorted by nearest node (z slowest, descending): processed in this order, the particles of one

@@ -82,7 +82,7 @@ __global__ void __launch_bounds__(BLOCK) coupleParticles(const S* __restrict__ s
         const double* __restrict__ positions, const double* __restrict__ velocities, const int* __restrict__ particles,
         const int* __restrict__ order, const double* __restrict__ mass, const double* __restrict__ ucm, ParticleArgs a,
         double* __restrict__ particleForce, unsigned long long* __restrict__ reactionFixed, uint32_t* __restrict__ touched,
-        double* __restrict__ partials) {
+        double* __restrict__ partials, double* __restrict__ systemForces) {
     constexpr int GROUPS = BLOCK/LANES;
     int lane = threadIdx.x % LANES;
     int rank = blockIdx.x*GROUPS + threadIdx.x/LANES;    // position in the processing order
@@ -111,7 +111,8 @@ __global__ void __launch_bounds__(BLOCK) coupleParticles(const S* __restrict__ s
     }
     double wallAcc[3] = {0, 0, 0};
     finishGroup(sl, u, velocities + 3*(size_t) part, (uint32_t) i, m, active, lane, a,
-            particleForce + 3*(size_t) (active ? i : 0), reactionFixed, touched, wallAcc, myNoise);
+            particleForce + 3*(size_t) (active ? i : 0), reactionFixed, touched, wallAcc, myNoise,
+            systemForces != nullptr ? systemForces + 3*(size_t) part : nullptr);
     if (partials != nullptr)
         storeWallPartial<BLOCK>(wallAcc, partials);
 }
@@ -219,25 +220,25 @@ template <typename S>
 static void launchCoupleT(const void* src, const uint32_t* mask, const double* positions, const double* velocities,
         const int* particles, const int* order, const double* mass, const double* ucm, const ParticleArgs& a,
         double* particleForce, unsigned long long* reactionFixed, uint32_t* touched, double* partials,
-        hipStream_t stream) {
+        double* systemForces, hipStream_t stream) {
     unsigned grid = (unsigned) coupleBlocks(a.stencil, a.numParticles);
     const S* s = (const S*) src;
     switch (coupleLanes(a.stencil)) {
     case 8:
         coupleParticles<S, 8><<<grid, BLOCK, 0, stream>>>(s, mask, positions, velocities, particles, order, mass, ucm, a,
-                particleForce, reactionFixed, touched, partials);
+                particleForce, reactionFixed, touched, partials, systemForces);
         break;
     case 32:
         coupleParticles<S, 32><<<grid, BLOCK, 0, stream>>>(s, mask, positions, velocities, particles, order, mass, ucm, a,
-                particleForce, reactionFixed, touched, partials);
+                particleForce, reactionFixed, touched, partials, systemForces);
         break;
     case 64:
         coupleParticles<S, 64><<<grid, BLOCK, 0, stream>>>(s, mask, positions, velocities, particles, order, mass, ucm, a,
-                particleForce, reactionFixed, touched, partials);
+                particleForce, reactionFixed, touched, partials, systemForces);
         break;
     default:
         coupleParticles<S, 1><<<grid, BLOCK, 0, stream>>>(s, mask, positions, velocities, particles, order, mass, ucm, a,
-                particleForce, reactionFixed, touched, partials);
+                particleForce, reactionFixed, touched, partials, systemForces);
         break;
     }
 }
@@ -245,25 +246,25 @@ static void launchCoupleT(const void* src, const uint32_t* mask, const double* p
 void launchCouple(int storage, const void* src, const uint32_t* mask, const double* positions, const double* velocities,
         const int* particles, const int* order, const double* mass, const double* ucm, const ParticleArgs& a,
         double* particleForce, unsigned long long* reactionFixed, uint32_t* touched, double* partials,
-        hipStream_t stream) {
+        double* systemForces, hipStream_t stream) {
     if (a.numParticles == 0)
         return;
     switch (storage) {
     case LBM_STORAGE_FP16:
         launchCoupleT<__half>(src, mask, positions, velocities, particles, order, mass, ucm, a, particleForce, reactionFixed,
-                touched, partials, stream);
+                touched, partials, systemForces, stream);
         break;
     case LBM_STORAGE_BF16:
         launchCoupleT<__hip_bfloat16>(src, mask, positions, velocities, particles, order, mass, ucm, a, particleForce,
-                reactionFixed, touched, partials, stream);
+                reactionFixed, touched, partials, systemForces, stream);
         break;
     case LBM_STORAGE_FP64:
         launchCoupleT<double>(src, mask, positions, velocities, particles, order, mass, ucm, a, particleForce, reactionFixed,
-                touched, partials, stream);
+                touched, partials, systemForces, stream);
         break;
     default:
         launchCoupleT<float>(src, mask, positions, velocities, particles, order, mass, ucm, a, particleForce, reactionFixed,
-                touched, partials, stream);
+                touched, partials, systemForces, stream);
         break;
     }
 }

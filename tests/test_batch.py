@@ -91,14 +91,14 @@ def test_batch_matches_single_replicas_bit_for_bit():
 def test_batch_autotune_keeps_the_state():
     replicas = uneven_replicas()[:2]
     batch = Batch(replicas)
-    times, best = batch.autotune(10)      # 3 x (10 warm-up + 10 timed) steps
+    times, best = batch.autotune(10)      # 3 rounds x 3 sizes x (10 warm-up + 10 timed) steps
     print("autotune ms/step:", times, "best", best, flush=True)
     assert best in times and all(t > 0 for t in times.values())
     batch.step(15)
     _, step = batch.info()
-    assert step == 75
+    assert step == 195
     for r, replica in enumerate(replicas):
-        rho, u, pos, vel = run_single(replica, 75)
+        rho, u, pos, vel = run_single(replica, 195)
         brho, bu = batch.getFluidFields(r)
         assert np.array_equal(brho, rho) and np.array_equal(bu, u), r
 
