@@ -1,0 +1,278 @@
+"""The closed forms of the reference's own tests (tests/TestLBMFluid.h, TestLBMCoupling.h), run as the reference runs
+them on every platform: on the FP64 oracle, and on the gfx950 library with FP64 and with FP32 storage.  Tolerances
+follow the reference's: the double-precision value for the oracle and FP64 storage; for FP32 storage the storage
+resolution 2e-6 (getStorageTolerance, TestLBMForce.h:37-42), or the test's own single-precision value
+(getFluidTolerance, TestLBMFluid.h:30-35), and at least 2e-6 for coupled particles (getCouplingTolerance,
+TestLBMCoupling.h:27-31)."""
+import math
+
+import numpy as np
+import pytest
+
+import oracle_lib
+from lbm_amd import LBMForce
+from parity import Setup, equilibrium_state, total_moments
+
+BACKENDS = [pytest.param("oracle", id="oracle"),
+            pytest.param("fp64", id="gpu-fp64", marks=pytest.mark.gpu),
+            pytest.param("fp32", id="gpu-fp32", marks=pytest.mark.gpu)]
+
+
+def build(setup, backend):
+    if backend == "oracle":
+        return setup.build(oracle_lib.load_oracle())
+    return setup.build(None, storage=LBMForce.FP64 if backend == "fp64" else LBMForce.FP32)
+
+
+def tol(backend, double, single=2e-6):
+    """getFluidTolerance: the double-precision tolerance, or the single-precision one with FP32 storage."""
+    return single if backend == "fp32" else double
+
+
+def ctol(backend, double):
+    """getCouplingTolerance: at least 2e-6 with FP32 storage."""
+    return max(double, 2e-6) if backend == "fp32" else double
+
+
+@pytest.mark.parametrize("backend", BACKENDS)
+def test_uniform_flow_is_steady(backend):
+    """TestLBMFluid.h:93-108."""
+    s = Setup((6, 5, 4), dx=0.5, dt=0.01, tau=0.8)
+    s.configure.append(lambda f: f.setInitialFluidVelocity((0.03*50, -0.02*50, 0.01*50)))
+    c = build(s, backend)
+    before = c.getFluidState()
+    c.step(50)
+    assert np.abs(c.getFluidState() - before).max() < tol(backend, 1e-13)
+
+
+@pytest.mark.parametrize("backend", BACKENDS)
+def test_conservation(backend):
+    """TestLBMFluid.h:113-135: mass and momentum of a perturbed fluid without forces."""
+    s = Setup((6, 5, 4), dx=0.5, dt=0.01, tau=0.7)
+    c = build(s, backend)
+    state = c.getFluidState()
+    i = np.arange(state.size)
+    state = state + 1e-3*np.sin(1.3*i + 0.7*(i % 11))
+    c.setFluidState(state)
+    m0, p0 = total_moments(state)
+    c.step(100)
+    m1, p1 = total_moments(c.getFluidState())
+    assert m1 == pytest.approx(m0, abs=tol(backend, 1e-13)*m0)
+    assert np.linalg.norm(p0) > 1e-5
+    assert np.abs(p1 - p0).max() < tol(backend, 1e-13)*m0
+
+
+@pytest.mark.parametrize("rho0", [1.0, 1.02])
+@pytest.mark.parametrize("backend", BACKENDS)
+def test_body_force(backend, rho0):
+    """TestLBMFluid.h:141-169: momentum n rho g per node, velocity (n + 1/2) g dt."""
+    dx, dt = 0.5, 0.01
+    s = Setup((4, 3, 5), dx=dx, dt=dt, tau=0.9)
+    g = np.array([0.5, -0.3, 0.2])
+    s.configure.append(lambda f: f.setBodyAcceleration(g))
+    s.state = equilibrium_state(60, rho0, np.zeros(3))
+    c = build(s, backend)
+    c.step(20)
+    m, p = total_moments(c.getFluidState())
+    gl = g*dt*dt/dx
+    assert m == pytest.approx(rho0*60, abs=tol(backend, 1e-12)*60)
+    assert np.allclose(p, gl*20*rho0*60, atol=tol(backend, 1e-13), rtol=0)
+    rho, u = c.getFluidFields()
+    assert np.allclose(u, g*20.5*dt, atol=tol(backend, 1e-12), rtol=0)
+
+
+@pytest.mark.parametrize("backend", BACKENDS)
+def test_momentum_removal_timing(backend):
+    """TestLBMFluid.h:176-199: with frequency 3 the momentum after n steps is ((n-1)%3 + 1) F per node."""
+    dx, dt = 0.5, 0.01
+    s = Setup((4, 4, 4), dx=dx, dt=dt, tau=1.0)
+    s.configure.append(lambda f: (f.setBodyAcceleration((0.5, 0, 0)), f.setFluidMomentumRemovalFrequency(3),
+                                  f.setInitialFluidVelocity((0.5, 1.0, -0.5))))
+    c = build(s, backend)
+    gl = 0.5*dt*dt/dx
+    for n in range(1, 8):
+        c.step(1)
+        m, p = total_moments(c.getFluidState())
+        assert np.allclose(p, [((n - 1) % 3 + 1)*gl*64, 0, 0], atol=tol(backend, 1e-13), rtol=0)
+
+
+@pytest.mark.parametrize("tau", [0.6, 1.0])
+@pytest.mark.parametrize("backend", BACKENDS)
+def test_shear_wave_viscosity(backend, tau):
+    """TestLBMFluid.h:205-251: u_x = U sin(k y) decays as exp(-nu k^2 t), nu = (tau - 1/2)/3, within 2e-3."""
+    nx, ny, nz = 2, 64, 2
+    s = Setup((nx, ny, nz), dx=0.5, dt=0.01, tau=tau)
+    n = nx*ny*nz
+    j = (np.arange(n)//nx) % ny
+    k = 2*np.pi/ny
+    u = np.zeros((n, 3))
+    u[:, 0] = 1e-3*np.sin(k*j)
+    s.state = equilibrium_state(n, 1.0, u)
+    c = build(s, backend)
+
+    def amplitude():
+        rho, vel = c.getFluidFields()
+        return 2*(vel[:, 0]*np.sin(k*j)).sum()/n/(0.5/0.01)
+    c.step(200)
+    a1 = amplitude()
+    c.step(1000)
+    a2 = amplitude()
+    nu = -math.log(a2/a1)/(1000*k*k)
+    assert abs(nu/((tau - 0.5)/3) - 1) < 2e-3
+
+
+def wall_plane(nx, ny, nz):
+    return [i + nx*(0 + ny*kk) for kk in range(nz) for i in range(nx)]
+
+
+@pytest.mark.parametrize("tau", [0.7, 0.875, 1.2])
+@pytest.mark.parametrize("backend", BACKENDS)
+def test_bounce_back_poiseuille(backend, tau):
+    """TestLBMFluid.h:486-535: exact steady profile of halfway bounce-back with the regularized collision,
+    u(y) = g/(2 nu)(y - 1/2)(n - 1/2 - y) + g(16 Lambda - 3)/(24 nu), and the walls carry the body force."""
+    nx, ny, nz = 2, 12, 2
+    dx, dt = 0.5, 0.01
+    g = 1e-5
+    s = Setup((nx, ny, nz), dx=dx, dt=dt, tau=tau)
+    s.configure.append(lambda f: (f.setBodyAcceleration((g*dx/dt**2, 0, 0)), f.setSolidNodes(wall_plane(nx, ny, nz))))
+    c = build(s, backend)
+    nu, lam, h = (tau - 0.5)/3, (tau - 0.5)/2, ny - 1
+    c.step(int(4*h*h/nu))
+    rho, u = c.getFluidFields()
+    umax = g/(2*nu)*(0.5*h)**2 + g*(16*lam - 3)/(24*nu)
+    for node in range(nx*ny*nz):
+        jj = (node//nx) % ny
+        if jj == 0:
+            assert rho[node] == 0 and np.all(u[node] == 0)
+            continue
+        exact = g/(2*nu)*(jj - 0.5)*(ny - 0.5 - jj) + g*(16*lam - 3)/(24*nu)
+        assert u[node, 0]*dt/dx/umax == pytest.approx(exact/umax, abs=tol(backend, 1e-9, 5e-5))
+    mass = rho.sum()*dx**3
+    # Relative to the size of the force, as ASSERT_EQUAL_VEC compares vectors.
+    assert np.allclose(c.getWallForce(), [g*dx/dt**2*mass, 0, 0], rtol=0, atol=tol(backend, 1e-8, 5e-5)*g*dx/dt**2*mass)
+
+
+@pytest.mark.parametrize("tau", [0.8, 1.0])
+@pytest.mark.parametrize("backend", BACKENDS)
+def test_regularized_poiseuille(backend, tau):
+    """TestLBMFluid.h:590-627: u(y) = g/(2 nu) y (n - y) + 3 g (tau - 1)/(tau - 1/2), density of the fluid at rest."""
+    nx, ny, nz = 2, 12, 2
+    dx, dt = 0.5, 0.01
+    g = 1e-5
+    s = Setup((nx, ny, nz), dx=dx, dt=dt, tau=tau)
+    s.configure.append(lambda f: (f.setBodyAcceleration((g*dx/dt**2, 0, 0)), f.setSolidNodes(wall_plane(nx, ny, nz)),
+                                  f.setWallScheme(LBMForce.Regularized)))
+    c = build(s, backend)
+    nu, h = (tau - 0.5)/3, ny
+    c.step(int(4*h*h/nu))
+    rho, u = c.getFluidFields()
+    umax = g/(2*nu)*(0.5*h)**2
+    for node in range(nx*ny*nz):
+        jj = (node//nx) % ny
+        if jj == 0:
+            continue
+        assert rho[node]/602.214 == pytest.approx(1.0, abs=tol(backend, 1e-9, 5e-5))
+        exact = g/(2*nu)*jj*(ny - jj) + 3*g*(tau - 1)/(tau - 0.5)
+        assert u[node, 0]*dt/dx/umax == pytest.approx(exact/umax, abs=tol(backend, 1e-9, 5e-5))
+    mass = rho.sum()*dx**3
+    # Relative to the size of the force, as ASSERT_EQUAL_VEC compares vectors.
+    assert np.allclose(c.getWallForce(), [g*dx/dt**2*mass, 0, 0], rtol=0, atol=tol(backend, 1e-8, 5e-5)*g*dx/dt**2*mass)
+
+
+def wall_test_nodes(nx, ny, nz):
+    return [i + nx*(j + ny*k) for k in range(nz) for j in range(ny) for i in range(nx)
+            if i == 0 or (2 <= i < 4 and 1 <= j < 3 and 1 <= k < 3)]
+
+
+@pytest.mark.parametrize("scheme", [LBMForce.BounceBack, LBMForce.Regularized])
+@pytest.mark.parametrize("backend", BACKENDS)
+def test_wall_momentum_balance(backend, scheme):
+    """TestLBMFluid.h:648-685: mass conserved; P(t+dt) - P(t) = (M g - F_wall) dt at every step."""
+    nx, ny, nz = 6, 5, 4
+    dx, dt = 0.5, 0.01
+    s = Setup((nx, ny, nz), dx=dx, dt=dt, tau=0.7)
+    g = np.array([1e-5, 2e-5, -1e-5])*dx/dt**2
+    s.configure.append(lambda f: (f.setSolidNodes(wall_test_nodes(nx, ny, nz)), f.setWallScheme(scheme),
+                                  f.setInitialFluidVelocity((1.0, -0.5, 0.25)), f.setBodyAcceleration(g)))
+    c = build(s, backend)
+
+    def moments():
+        rho, u = c.getFluidFields()
+        return (rho*dx**3).sum(), ((rho*dx**3)[:, None]*u).sum(axis=0)
+    m0, p0 = moments()
+    scale = m0*0.02*dx/dt
+    for _ in range(30):
+        c.step(1)
+        m, p = moments()
+        assert m == pytest.approx(m0, rel=tol(backend, 1e-13))
+        balance = p - p0 - (g*m - c.getWallForce())*dt
+        assert np.linalg.norm(balance)/scale < tol(backend, 1e-12)
+        p0 = p
+
+
+@pytest.mark.parametrize("boundary", [LBMForce.Velocity, LBMForce.DensityVelocity])
+@pytest.mark.parametrize("backend", BACKENDS)
+def test_couette(backend, boundary):
+    """TestLBMFluid.h:694-716: linear profile u_x(z) = U (z + 1)/(nz + 1), uniform density."""
+    nx, ny, nz = 2, 2, 10
+    dx, dt, tau, U = 0.5, 0.01, 0.9, 0.01
+    s = Setup((nx, ny, nz), dx=dx, dt=dt, tau=tau)
+    s.configure.append(lambda f: (f.setFaceBoundary(LBMForce.ZMin, boundary), f.setFaceBoundary(LBMForce.ZMax, boundary),
+                                  f.setFaceVelocity(LBMForce.ZMax, (U*dx/dt, 0, 0))))
+    c = build(s, backend)
+    nu, h = (tau - 0.5)/3, nz + 1
+    c.step(int(6*h*h/nu))
+    rho, u = c.getFluidFields()
+    k = np.arange(nx*ny*nz)//(nx*ny)
+    assert np.allclose(u[:, 0]*dt/(dx*U), (k + 1)/h, atol=tol(backend, 1e-9, 5e-5), rtol=0)
+    assert np.allclose(rho/602.214, 1.0, atol=tol(backend, 1e-9, 5e-5), rtol=0)
+
+
+@pytest.mark.parametrize("backend", BACKENDS)
+def test_first_step_drag(backend):
+    """TestLBMCoupling.h:84-103: at T = 0 a particle in fluid at rest has v1 = v0 (1 - gamma dt) after one step."""
+    s = Setup((8, 8, 8), dx=0.5, dt=0.01, tau=0.8)
+    v0 = np.array([[0.5, -0.3, 0.2]])
+    s.add_particles([[1.0, 1.2, 2.1]], mass=100.0, velocities=v0)
+    s.configure.append(lambda f: (f.setFriction(10.0), f.setTemperature(0.0)))
+    c = build(s, backend)
+    c.step(1)
+    assert np.allclose(c.velocities[0], v0[0]*(1 - 10.0*0.01), atol=ctol(backend, 1e-13), rtol=0)
+
+
+@pytest.mark.parametrize("stencil", [LBMForce.NearestNode, LBMForce.Keys])
+@pytest.mark.parametrize("backend", BACKENDS)
+def test_total_momentum_conserved_with_particles(backend, stencil):
+    """TestLBMCoupling.h:146-172: sum m v + j m_c dx/dt is conserved with drag and random forces, two particles
+    sharing a node and one crossing the periodic boundary."""
+    dx, dt = 0.5, 0.01
+    s = Setup((8, 8, 8), dx=dx, dt=dt, tau=0.8)
+    pos = [[1.02, 2.01, 0.98], [0.97, 1.99, 1.03], [3.98, 0.02, 3.96], [2.3, 1.1, 0.6]]
+    vel = [[0.5, -0.2, 0.3], [-0.4, 0.1, 0.2], [2.0, -1.5, 1.0], [0.0, 0.3, -0.6]]
+    s.add_particles(pos, mass=100.0, velocities=vel)
+    s.configure.append(lambda f: (f.setFriction(10.0), f.setTemperature(300.0), f.setRandomNumberSeed(3),
+                                  f.setInterpolationStencil(stencil)))
+    c = build(s, backend)
+    cell = 602.214*dx**3
+
+    def momentum():
+        m, j = total_moments(c.getFluidState())
+        return (100.0*c.velocities).sum(axis=0) + j*cell*dx/dt
+    scale = sum(100.0*np.linalg.norm(v) for v in vel)
+    p0 = momentum()
+    c.step(50)
+    p1 = momentum()
+    assert np.linalg.norm(p1 - p0)/scale < ctol(backend, 1e-11)
+    assert c.positions[2, 0] > 4.0
+
+
+@pytest.mark.parametrize("backend", BACKENDS)
+def test_comoving_particle_feels_no_force(backend):
+    """TestLBMCoupling.h:178-193: a particle moving with a uniform flow keeps its velocity."""
+    for u in ([1.0, 0, 0], [0, 1.0, 0], [0.6, -0.5, 0.6]):
+        s = Setup((8, 8, 8), dx=0.5, dt=0.01, tau=0.8)
+        s.add_particles([[1.1, 1.3, 1.7]], mass=100.0, velocities=[u])
+        s.configure.append(lambda f, u=u: (f.setFriction(10.0), f.setTemperature(0.0), f.setInitialFluidVelocity(u)))
+        c = build(s, backend)
+        c.step(100)
+        assert np.allclose(c.velocities[0], u, atol=ctol(backend, 1e-13), rtol=0)
