@@ -1,9 +1,10 @@
-"""The closed forms of the reference's own tests (tests/TestLBMFluid.h, TestLBMCoupling.h), run as the reference runs
-them on every platform: on the FP64 oracle, and on the gfx950 library with FP64 and with FP32 storage.  Tolerances
-follow the reference's: the double-precision value for the oracle and FP64 storage; for FP32 storage the storage
-resolution 2e-6 (getStorageTolerance, TestLBMForce.h:37-42), or the test's own single-precision value
-(getFluidTolerance, TestLBMFluid.h:30-35), and at least 2e-6 for coupled particles (getCouplingTolerance,
+"""The closed forms of the reference's own tests (tests/TestLBMFluid.h, TestLBMCoupling.h, TestLBMStencils.h), run as
+the reference runs them on every platform: on the FP64 oracle, and on the gfx950 library with FP64 and with FP32
+storage.  Tolerances follow the reference's: the double-precision value for the oracle and FP64 storage; for FP32
+storage the storage resolution 2e-6 (getStorageTolerance, TestLBMForce.h:37-42), or the test's own single-precision
+value (getFluidTolerance, TestLBMFluid.h:30-35), and at least 2e-6 for coupled particles (getCouplingTolerance,
 TestLBMCoupling.h:27-31)."""
+import ctypes
 import math
 
 import numpy as np
@@ -276,3 +277,119 @@ def test_comoving_particle_feels_no_force(backend):
         c = build(s, backend)
         c.step(100)
         assert np.allclose(c.velocities[0], u, atol=ctol(backend, 1e-13), rtol=0)
+
+
+# ------------------------------------------------------------------------------------------------
+# Stencils (TestLBMStencils.h), on the 8^3 coupled system of the reference's coupling tests (dx = 0.5 nm, dt = 0.01 ps,
+# rho0 = 602.2 Da/nm^3, tau = 0.8, particles of 100 Da).
+# ------------------------------------------------------------------------------------------------
+STENCILS = [LBMForce.Trilinear, LBMForce.ThreePoint, LBMForce.Keys]
+DX, DT, N8 = 0.5, 0.01, 8
+U0, X0 = np.array([0.01, -0.02, 0.015]), np.array([2.0, 2.0, 2.0])
+GRAD = np.array([[0.002, -0.001, 0.0005], [0.0015, 0.001, -0.002], [-0.001, 0.0005, 0.002]])
+
+
+def coupled_setup(positions, velocities, friction, stencil, state=None, faces=()):
+    s = Setup((N8, N8, N8), dx=DX, dt=DT, tau=0.8, density=602.2)
+    s.add_particles(positions, mass=100.0, velocities=velocities)
+    s.state = state
+
+    def conf(f):
+        f.setFriction(friction)
+        f.setTemperature(0.0)
+        f.setRandomNumberSeed(1)
+        f.setInterpolationStencil(stencil)
+        for face, kind in faces:
+            f.setFaceBoundary(face, kind)
+    s.configure.append(conf)
+    return s
+
+
+def linear_velocity(x):
+    return U0 + (np.asarray(x) - X0) @ GRAD
+
+
+def linear_state():
+    """Equilibrium with a density and a velocity linear in space (TestLBMStencils.h:131-148)."""
+    node = np.arange(N8**3)
+    i, j, k = node % N8, (node//N8) % N8, node//(N8*N8)
+    x = np.stack([i, j, k], axis=1)*DX
+    rho = 1.0 + 0.01*(i - 4) - 0.005*(j - 4) + 0.008*(k - 4)
+    return equilibrium_state(N8**3, rho, linear_velocity(x)*(DT/DX))
+
+
+def varied_state(rho_varies):
+    node = np.arange(N8**3)
+    u = np.stack([0.01*np.sin(node), 0.01*np.cos(node), 0.005*np.sin(3.0*node)], axis=1)
+    rho = 1.0 + 0.001*(node % 7) if rho_varies else np.ones(N8**3)
+    return equilibrium_state(N8**3, rho, u)
+
+
+@pytest.mark.parametrize("stencil", STENCILS)
+@pytest.mark.parametrize("backend", BACKENDS)
+def test_stencil_linear_field(backend, stencil):
+    """TestLBMStencils.h:130-164: in the first step a particle at rest feels gamma m u(X), u the linear field."""
+    friction, position = 5.0, np.array([2.13, 1.87, 2.21])
+    c = build(coupled_setup([position], [[0, 0, 0]], friction, stencil, linear_state()), backend)
+    c.step(1)
+    assert np.allclose(c.velocities[0], linear_velocity(position)*(friction*DT), atol=ctol(backend, 1e-12), rtol=0)
+
+
+@pytest.mark.parametrize("stencil", STENCILS)
+@pytest.mark.parametrize("backend", BACKENDS)
+def test_stencil_open_faces(backend, stencil):
+    """TestLBMStencils.h:172-223: along z the stencil stops at the faces; the particle feels the linear field at the
+    mean node sum_c w_c z_c of its stencil (its own z away from the faces; node n - 1 for trilinear past it)."""
+    lib = oracle_lib.load_oracle()
+    width = {LBMForce.Trilinear: 2, LBMForce.ThreePoint: 3, LBMForce.Keys: 4}[stencil]
+    friction = 5.0
+    for z in (0.1, 0.35, 0.6, 1.3, 2.2, 3.3, 3.62, 3.9):
+        position = np.array([2.13, 1.87, z])
+        faces = ((LBMForce.ZMin, LBMForce.Velocity), (LBMForce.ZMax, LBMForce.Density))
+        c = build(coupled_setup([position], [[0, 0, 0]], friction, stencil, linear_state(), faces), backend)
+        c.step(1)
+        idx = (ctypes.c_int*4)()
+        w = np.zeros(4)
+        lib.oracle_axis_weights(stencil, z/DX, N8, 1, idx, w.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+        mean = sum(w[a]*idx[a] for a in range(width))
+        if width/2 < z/DX < N8 - 1 - width/2:
+            assert mean == pytest.approx(z/DX, abs=1e-13)
+        if stencil == LBMForce.Trilinear and z/DX > N8 - 1:
+            assert mean == pytest.approx(N8 - 1.0, abs=1e-15)
+        seen = linear_velocity([position[0], position[1], mean*DX])
+        assert np.allclose(c.velocities[0], seen*(friction*DT), atol=ctol(backend, 1e-12), rtol=0), z
+
+
+@pytest.mark.parametrize("backend", BACKENDS)
+def test_stencil_at_node(backend):
+    """TestLBMStencils.h:229-259: on a node the trilinear and Keys stencils weigh only that node, so the first step is
+    the nearest node's, bit for bit, for the particle and for the fluid."""
+    results = []
+    for stencil in (LBMForce.NearestNode, LBMForce.Trilinear, LBMForce.Keys):
+        c = build(coupled_setup([[1.0, 1.5, 2.0]], [[0.3, -0.2, 0.1]], 5.0, stencil, varied_state(True)), backend)
+        c.step(1)
+        results.append((c.velocities[0].copy(), c.getFluidState()))
+    for v, fluid in results[1:]:
+        assert np.array_equal(v, results[0][0])
+        assert np.array_equal(fluid, results[0][1])
+
+
+@pytest.mark.parametrize("stencil", STENCILS)
+@pytest.mark.parametrize("backend", BACKENDS)
+def test_stencil_translation(backend, stencil):
+    """TestLBMStencils.h:496-541: moving the particles and the fluid by one node along each axis moves the whole run."""
+    positions = np.array([[1.13, 2.07, 0.91], [1.31, 1.88, 1.17], [2.6, 3.2, 0.4]])
+    velocities = [[0.5, -0.2, 0.3], [-0.4, 0.1, 0.2], [0.2, -0.5, 0.1]]
+
+    def shifted(state):
+        s = state.reshape(19, N8, N8, N8)          # [q][k][j][i]
+        return np.roll(s, (1, 1, 1), axis=(1, 2, 3)).reshape(-1)
+    state = varied_state(False)
+    runs = []
+    for shift in (0, 1):
+        c = build(coupled_setup(positions + DX*shift, velocities, 10.0, stencil, shifted(state) if shift else state),
+                  backend)
+        c.step(30)
+        runs.append((c.getFluidState(), c.positions.copy()))
+    assert np.allclose(shifted(runs[0][0]), runs[1][0], atol=ctol(backend, 1e-12), rtol=0)
+    assert np.allclose(runs[0][1] + DX, runs[1][1], atol=ctol(backend, 1e-12), rtol=0)
