@@ -393,3 +393,145 @@ def test_stencil_translation(backend, stencil):
         runs.append((c.getFluidState(), c.positions.copy()))
     assert np.allclose(shifted(runs[0][0]), runs[1][0], atol=ctol(backend, 1e-12), rtol=0)
     assert np.allclose(runs[0][1] + DX, runs[1][1], atol=ctol(backend, 1e-12), rtol=0)
+
+
+# ------------------------------------------------------------------------------------------------
+# Walls and open faces (TestLBMFluid.h:541-884), on the reference's fluid systems (dx = 0.5 nm, dt = 0.01 ps,
+# rho0 = 602.2 Da/nm^3).
+# ------------------------------------------------------------------------------------------------
+RHO_F = 602.2
+
+
+def fluid_setup(grid, tau):
+    return Setup(grid, dx=DX, dt=DT, tau=tau, density=RHO_F)
+
+
+@pytest.mark.parametrize("backend", BACKENDS)
+def test_wall_conservation(backend):
+    """TestLBMFluid.h:541-580: a solid block conserves the fluid mass; with removal the momentum is removed."""
+    nx, ny, nz = 6, 5, 4
+    block = [i + nx*(j + ny*k) for k in (1, 2) for j in (1, 2) for i in (2, 3)]
+    for removal in (0, 1):
+        s = fluid_setup((nx, ny, nz), 0.7)
+        s.configure.append(lambda f, removal=removal: (
+            f.setSolidNodes(block), f.setInitialFluidVelocity((0.02*DX/DT, -0.01*DX/DT, 0.0)),
+            f.setFluidMomentumRemovalFrequency(removal)))
+        c = build(s, backend)
+
+        def mass():
+            rho, u = c.getFluidFields()
+            return (rho/RHO_F).sum(), rho, u
+        m0, _, _ = mass()
+        assert m0 == pytest.approx(nx*ny*nz - len(block), abs=tol(backend, 1e-13))
+        c.step(100 if removal == 0 else 1)
+        m1, rho, u = mass()
+        assert m1 == pytest.approx(m0, abs=tol(backend, 1e-13))
+        if removal:
+            p = (u*(rho/RHO_F*DT/DX)[:, None]).sum(axis=0)
+            assert np.allclose(p, 0, atol=tol(backend, 1e-13))
+
+
+@pytest.mark.parametrize("inlet,outlet", [(LBMForce.Velocity, LBMForce.Density),
+                                          (LBMForce.DensityVelocity, LBMForce.Density),
+                                          (LBMForce.DensityVelocity, LBMForce.DensityVelocity)])
+@pytest.mark.parametrize("backend", BACKENDS)
+def test_uniform_flow_through_faces(backend, inlet, outlet):
+    """TestLBMFluid.h:724-762: a uniform flow through an inlet and an outlet face stays uniform."""
+    nx, ny, nz = 3, 8, 3
+    u = np.array([0.0, 0.02, 0.0])*(DX/DT)
+    s = fluid_setup((nx, ny, nz), 0.8)
+    s.configure.append(lambda f: (f.setFaceBoundary(LBMForce.YMin, inlet), f.setFaceBoundary(LBMForce.YMax, outlet),
+                                  f.setFaceVelocity(LBMForce.YMin, u), f.setFaceVelocity(LBMForce.YMax, u),
+                                  f.setInitialFluidVelocity(u)))
+    c = build(s, backend)
+    c.step(50)
+    rho, vel = c.getFluidFields()
+    assert np.allclose(vel*(DT/DX), u*(DT/DX), atol=tol(backend, 1e-13), rtol=0)
+    assert np.allclose(rho/RHO_F, 1.0, atol=tol(backend, 1e-13), rtol=0)
+
+
+@pytest.mark.parametrize("backend", BACKENDS)
+def test_density_velocity_reservoir(backend):
+    """TestLBMFluid.h:820-851: fluid at rest between two equal DensityVelocity faces reaches their uniform flow."""
+    nx, ny, nz = 3, 12, 3
+    rho, u = 1.01, np.array([0.01, 0.02, 0.0])
+    s = fluid_setup((nx, ny, nz), 1.0)
+
+    def conf(f):
+        for face in (LBMForce.YMin, LBMForce.YMax):
+            f.setFaceBoundary(face, LBMForce.DensityVelocity)
+            f.setFaceDensity(face, rho*RHO_F)
+            f.setFaceVelocity(face, u*(DX/DT))
+    s.configure.append(conf)
+    c = build(s, backend)
+    c.step(2500)
+    density, vel = c.getFluidFields()
+    assert np.allclose(vel*(DT/DX), u, atol=tol(backend, 1e-9, 1e-5), rtol=0)
+    assert np.allclose(density/RHO_F, rho, atol=tol(backend, 1e-9, 1e-5), rtol=0)
+
+
+@pytest.mark.parametrize("backend", BACKENDS)
+def test_density_velocity_inlet(backend):
+    """TestLBMFluid.h:853-884: a DensityVelocity inlet at 1.01 and a Density outlet at 1 give the uniform flow whose
+    entering populations rho (1 + 3u + 3u^2)/6 are the inlet's, at the outlet's density."""
+    nx, ny, nz = 3, 16, 3
+    rho_in, ub = 1.01, 0.01
+    s = fluid_setup((nx, ny, nz), 0.8)
+    s.configure.append(lambda f: (
+        f.setFaceBoundary(LBMForce.YMin, LBMForce.DensityVelocity), f.setFaceBoundary(LBMForce.YMax, LBMForce.Density),
+        f.setFaceDensity(LBMForce.YMin, rho_in*RHO_F), f.setFaceVelocity(LBMForce.YMin, (0.0, ub*DX/DT, 0.0))))
+    c = build(s, backend)
+    c.step(4000)
+    density, vel = c.getFluidFields()
+    a = rho_in*(1.0 + 3.0*ub + 3.0*ub*ub) - 1.0
+    u = (-3.0 + math.sqrt(9.0 + 12.0*a))/6.0
+    assert np.allclose(vel*(DT/DX), [0.0, u, 0.0], atol=tol(backend, 1e-10, 1e-5), rtol=0)
+    assert np.allclose(density/RHO_F, 1.0, atol=tol(backend, 1e-10, 1e-5), rtol=0)
+
+
+@pytest.mark.parametrize("backend", BACKENDS)
+def test_pressure_driven_duct(backend):
+    """TestLBMFluid.h:764-818: a square duct between two Density faces at 1.01 and 1: the density at the middle on the
+    line between the faces' (held one node beyond them), and the series solution of the rectangular duct with the
+    gradient measured there, within 2% of the peak; the flow is steady."""
+    n, ny, tau = 8, 16, 1.0
+    walls = [i + n*(j + ny*k) for k in range(n) for j in range(ny) for i in range(n) if i == 0 or k == 0]
+    s = fluid_setup((n, ny, n), tau)
+    s.configure.append(lambda f: (
+        f.setSolidNodes(walls), f.setFaceBoundary(LBMForce.YMin, LBMForce.Density),
+        f.setFaceBoundary(LBMForce.YMax, LBMForce.Density), f.setFaceDensity(LBMForce.YMin, 1.01*RHO_F),
+        f.setFaceDensity(LBMForce.YMax, RHO_F)))
+    c = build(s, backend)
+    nu = (tau - 0.5)/3.0
+    c.step(int(3*ny*ny/nu))
+    density, velocity = [], []
+    for _ in range(2):
+        rho, u = c.getFluidFields()
+        density.append(rho)
+        velocity.append(u)
+        c.step(1)
+    middle = ny//2
+    H = n - 1
+
+    def at(i, j, k):
+        return i + n*(j + ny*k)
+    rho = density[0][at(n//2, middle, n//2)]/RHO_F
+    assert rho == pytest.approx(1.0 + 0.01*(ny - middle)/(ny + 1), abs=2e-4)
+    gradient = (density[0][at(n//2, middle - 2, n//2)] - density[0][at(n//2, middle + 2, n//2)])/(4*RHO_F)
+    assert 0.01/(ny + 1) < gradient < 0.01/(ny - 3)
+    K = (gradient/3.0)/(rho*nu)
+    umax = error = change = 0.0
+    for k in range(1, n):
+        for i in range(1, n):
+            node = at(i, middle, k)
+            x, z = i - 0.5*n, k - 0.5*n
+            exact = 0.0
+            for m in range(1, 200, 2):
+                a = m*math.pi/H
+                exact += 4*K*H*H/(m*math.pi)**3*(1 if m % 4 == 1 else -1)*(1 - math.cosh(a*z)/math.cosh(a*H/2))*math.cos(a*x)
+            u = velocity[0][node][1]*DT/DX
+            umax = max(umax, exact)
+            error = max(error, abs(u - exact))
+            change = max(change, abs(velocity[1][node][1] - velocity[0][node][1])*DT/DX)
+    assert error < 0.02*umax
+    assert change < tol(backend, 1e-9, 1e-4)*umax
