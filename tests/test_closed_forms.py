@@ -241,18 +241,24 @@ def test_first_step_drag(backend):
     assert np.allclose(c.velocities[0], v0[0]*(1 - 10.0*0.01), atol=ctol(backend, 1e-13), rtol=0)
 
 
+DRAGS = [pytest.param(LBMForce.Explicit, id="explicit"), pytest.param(LBMForce.Centered, id="centred")]
+
+
+@pytest.mark.parametrize("rho0", [1.0, 0.98, 1.02])
+@pytest.mark.parametrize("drag", DRAGS)
 @pytest.mark.parametrize("stencil", [LBMForce.NearestNode, LBMForce.Keys])
 @pytest.mark.parametrize("backend", BACKENDS)
-def test_total_momentum_conserved_with_particles(backend, stencil):
+def test_total_momentum_conserved_with_particles(backend, stencil, drag, rho0):
     """TestLBMCoupling.h:146-172: sum m v + j m_c dx/dt is conserved with drag and random forces, two particles
-    sharing a node and one crossing the periodic boundary."""
+    sharing a node and one crossing the periodic boundary, in a fluid at rest of lattice density rho0."""
     dx, dt = 0.5, 0.01
     s = Setup((8, 8, 8), dx=dx, dt=dt, tau=0.8)
     pos = [[1.02, 2.01, 0.98], [0.97, 1.99, 1.03], [3.98, 0.02, 3.96], [2.3, 1.1, 0.6]]
     vel = [[0.5, -0.2, 0.3], [-0.4, 0.1, 0.2], [2.0, -1.5, 1.0], [0.0, 0.3, -0.6]]
     s.add_particles(pos, mass=100.0, velocities=vel)
     s.configure.append(lambda f: (f.setFriction(10.0), f.setTemperature(300.0), f.setRandomNumberSeed(3),
-                                  f.setInterpolationStencil(stencil)))
+                                  f.setInterpolationStencil(stencil), f.setDragScheme(drag)))
+    s.state = equilibrium_state(512, rho0, np.zeros(3))
     c = build(s, backend)
     cell = 602.214*dx**3
 
@@ -265,18 +271,165 @@ def test_total_momentum_conserved_with_particles(backend, stencil):
     p1 = momentum()
     assert np.linalg.norm(p1 - p0)/scale < ctol(backend, 1e-11)
     assert c.positions[2, 0] > 4.0
+    assert np.any(c.velocities[3] != vel[3])
 
 
+@pytest.mark.parametrize("drag", DRAGS)
 @pytest.mark.parametrize("backend", BACKENDS)
-def test_comoving_particle_feels_no_force(backend):
+def test_comoving_particle_feels_no_force(backend, drag):
     """TestLBMCoupling.h:178-193: a particle moving with a uniform flow keeps its velocity."""
-    for u in ([1.0, 0, 0], [0, 1.0, 0], [0.6, -0.5, 0.6]):
+    for u in ([1.0, 0, 0], [0, 1.0, 0], [0, 0, 1.0], [0.6, -0.5, 0.6]):
         s = Setup((8, 8, 8), dx=0.5, dt=0.01, tau=0.8)
         s.add_particles([[1.1, 1.3, 1.7]], mass=100.0, velocities=[u])
-        s.configure.append(lambda f, u=u: (f.setFriction(10.0), f.setTemperature(0.0), f.setInitialFluidVelocity(u)))
+        s.configure.append(lambda f, u=u: (f.setFriction(10.0), f.setTemperature(0.0), f.setInitialFluidVelocity(u),
+                                           f.setDragScheme(drag)))
         c = build(s, backend)
         c.step(100)
         assert np.allclose(c.velocities[0], u, atol=ctol(backend, 1e-13), rtol=0)
+
+
+def coupling_system(positions, velocities, friction, temperature, coupled=None, configure=()):
+    """createCoupledSystem (TestLBMCoupling.h:40-61): 8^3 nodes, tau = 0.8, seed 1, particles of 100 Da."""
+    s = Setup((N8, N8, N8), dx=DX, dt=DT, tau=0.8, density=602.2)
+    for i, (x, v) in enumerate(zip(positions, velocities)):
+        s.add_particles([x], mass=100.0, velocities=[v], coupled=coupled is None or i in coupled)
+
+    def conf(f):
+        f.setFriction(friction)
+        f.setTemperature(temperature)
+        f.setRandomNumberSeed(1)
+        for fn in configure:
+            fn(f)
+    s.configure.append(conf)
+    return s
+
+
+def wall_plane_y0():
+    """wallPlane(8, 8, 8) (TestLBMFluid.h:452-458): the plane j = 0."""
+    return [i + N8*N8*k for k in range(N8) for i in range(N8)]
+
+
+@pytest.mark.parametrize("drag", DRAGS)
+@pytest.mark.parametrize("backend", BACKENDS)
+def test_partial_coupling(backend, drag):
+    """TestLBMCoupling.h:198-220: only the coupled particles feel the fluid; the others keep their velocity, no force."""
+    positions = [[0.3, 0.4, 0.5], [1.3, 1.4, 1.5], [2.3, 2.4, 2.5], [3.3, 3.4, 3.5]]
+    v0 = np.array([0.2, 0.1, -0.3])
+    c = build(coupling_system(positions, [v0]*4, 5.0, 0.0, coupled={1, 3},
+                              configure=[lambda f: f.setDragScheme(drag)]), backend)
+    c.step(10)
+    forces = c.getForces()
+    for i in range(4):
+        v = c.velocities[i]
+        if i % 2 == 0:
+            assert np.allclose(v, v0, atol=ctol(backend, 1e-12), rtol=0)
+            assert np.all(forces[i] == 0.0)
+        else:
+            assert v @ v < 0.9*(v0 @ v0)
+
+
+@pytest.mark.parametrize("drag", DRAGS)
+@pytest.mark.parametrize("backend", BACKENDS)
+def test_force_evaluations_and_seeds(backend, drag):
+    """TestLBMCoupling.h:226-254: force evaluations between the steps change nothing; the same seed gives the same
+    run, seed 0 a different one each time."""
+    def run(seed, queries):
+        s = coupling_system([[0.3, 0.4, 0.5], [1.3, 1.4, 1.5], [2.3, 2.4, 2.5]], [[0, 0, 0]]*3, 5.0, 300.0,
+                            configure=[lambda f: (f.setDragScheme(drag), f.setRandomNumberSeed(seed))])
+        c = build(s, backend)
+        for _ in range(20):
+            if queries:
+                c.getForces()
+            c.step(1)
+            if queries:
+                c.getForces()
+        return c.velocities.copy()
+    assert np.array_equal(run(1234, False), run(1234, True))
+    assert run(0, False)[0, 0] != run(0, False)[0, 0]
+
+
+@pytest.mark.parametrize("backend", BACKENDS)
+def test_nve_scheme(backend):
+    """TestLBMCoupling.h:336-357: the NVE scheme has no random force at any temperature."""
+    friction = 5.0
+
+    def run(seed, steps):
+        s = coupling_system([[1.1, 1.3, 1.7], [2.6, 0.4, 3.1]], [[0.3, -0.2, 0.1], [-0.1, 0.4, 0.2]], friction, 300.0,
+                            configure=[lambda f: (f.setCouplingScheme(LBMForce.NVE), f.setRandomNumberSeed(seed))])
+        c = build(s, backend)
+        c.step(steps)
+        return c.velocities.copy()
+    v1 = run(1, 1)
+    assert np.allclose(v1[0], np.array([0.3, -0.2, 0.1])*(1 - friction*DT), atol=ctol(backend, 1e-14), rtol=0)
+    assert np.array_equal(run(1, 30), run(2, 30))
+
+
+@pytest.mark.parametrize("backend", BACKENDS)
+def test_wall_reflection(backend):
+    """TestLBMCoupling.h:363-385: a coupled particle reaching a solid node while moving into the wall is reversed and
+    slowed by the drag of the wall at rest; an uncoupled one does not see the wall."""
+    friction = 1.0
+    v0 = np.array([0.5, -10.0, 0.2])
+    c = build(coupling_system([[1.1, 0.30, 1.7]]*2, [v0]*2, friction, 0.0, coupled={0},
+                              configure=[lambda f: f.setSolidNodes(wall_plane_y0())]), backend)
+    decay = 1.0 - friction*DT
+    c.step(1)
+    assert np.allclose(c.velocities[0], v0*decay, atol=ctol(backend, 1e-14), rtol=0)
+    assert c.positions[0, 1] < 0.25
+    c.step(1)
+    assert np.allclose(c.velocities[0], -v0*decay*decay, atol=ctol(backend, 1e-14), rtol=0)
+    assert c.positions[0, 1] > 0.25
+    assert np.allclose(c.velocities[1], v0, atol=ctol(backend, 1e-12), rtol=0)
+
+
+@pytest.mark.parametrize("backend", BACKENDS)
+def test_wall_reflection_direction(backend):
+    """TestLBMCoupling.h:391-409: reversed only when moving into the wall (the plane j = 0 has fluid on both sides)."""
+    friction = 1.0
+    v0 = np.array([[0.3, 5.0, 0.1], [0.3, 5.0, 0.1], [0.3, -5.0, 0.1]])
+    c = build(coupling_system([[1.1, 0.20, 1.7], [2.1, -0.10, 1.7], [3.1, -0.10, 1.7]], v0, friction, 0.0,
+                              configure=[lambda f: f.setSolidNodes(wall_plane_y0())]), backend)
+    c.step(1)
+    decay = 1.0 - friction*DT
+    assert np.allclose(c.velocities[0], v0[0]*decay, atol=ctol(backend, 1e-14), rtol=0)
+    assert np.allclose(c.velocities[1], -v0[1]*decay, atol=ctol(backend, 1e-14), rtol=0)
+    assert np.allclose(c.velocities[2], v0[2]*decay, atol=ctol(backend, 1e-14), rtol=0)
+
+
+@pytest.mark.parametrize("scheme", [pytest.param(LBMForce.BounceBack, id="bounce-back"),
+                                    pytest.param(LBMForce.Regularized, id="regularized")])
+@pytest.mark.parametrize("drag", DRAGS)
+@pytest.mark.parametrize("backend", BACKENDS)
+def test_wall_momentum_balance_with_particles(backend, drag, scheme):
+    """TestLBMCoupling.h:416-476: particles, fluid and walls conserve momentum, the walls' share being the sum of
+    getWallForce() dt; one particle is reflected by the wall j = 0."""
+    walls = wall_plane_y0() + [node + N8*4 for node in wall_plane_y0()]
+    positions = [[1.1, 0.30, 1.7], [2.3, 1.2, 0.6], [0.7, 3.1, 2.9], [3.3, 1.6, 3.6]]
+    velocities = [[0.5, -10.0, 0.2], [0.4, 0.3, -0.2], [-0.6, 0.2, 0.5], [0.1, -0.4, 0.3]]
+    c = build(coupling_system(positions, velocities, 10.0, 300.0, configure=[lambda f: (
+        f.setDragScheme(drag), f.setWallScheme(scheme), f.setSolidNodes(walls),
+        f.setInitialFluidVelocity((0.3, 0.4, -0.2)))]), backend)
+
+    def momentum():
+        p = (100.0*c.velocities).sum(axis=0)
+        scale = (100.0*np.linalg.norm(c.velocities, axis=1)).sum()
+        rho, u = c.getFluidFields()
+        nodes = u*(rho*DX**3)[:, None]
+        return p + nodes.sum(axis=0), scale + np.linalg.norm(nodes, axis=1).sum()
+    p0, _ = momentum()
+    assert np.all(np.asarray(c.getWallForce()) == 0.0)
+    wall = np.zeros(3)
+    for n in range(100):
+        c.step(1)
+        wall += np.asarray(c.getWallForce())*DT
+        if n == 1:
+            assert c.velocities[0, 1] > 0
+    p1, scale = momentum()
+    assert wall @ wall > 1.0
+    tolerance = ctol(backend, 1e-11 if scheme == LBMForce.Regularized else 1e-12)
+    if scheme == LBMForce.Regularized and backend == "fp32":
+        tolerance = 1e-5
+    assert np.linalg.norm(p1 + wall - p0)/scale < tolerance
 
 
 # ------------------------------------------------------------------------------------------------
