@@ -84,9 +84,6 @@ static size_t storageSize(int storage) {
 static const int HCX[NQ] = {0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 0, 0, 0, 0, 1, -1, -1, 1};
 static const int HCY[NQ] = {0, 0, 0, 1, -1, 0, 0, 1, -1, -1, 1, 1, -1, 1, -1, 0, 0, 0, 0};
 static const int HCZ[NQ] = {0, 0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 1, -1, -1, 1, 1, -1, 1, -1};
-static const double HW[NQ] = {1.0/3.0, 1.0/18.0, 1.0/18.0, 1.0/18.0, 1.0/18.0, 1.0/18.0, 1.0/18.0,
-        1.0/36.0, 1.0/36.0, 1.0/36.0, 1.0/36.0, 1.0/36.0, 1.0/36.0, 1.0/36.0, 1.0/36.0, 1.0/36.0,
-        1.0/36.0, 1.0/36.0, 1.0/36.0};
 
 struct CheckpointHeader {
     char magic[8];
@@ -188,6 +185,9 @@ void findWalls(lbm_context* c, std::vector<int>& nodes, std::vector<int>& links)
         return;
     int size[3] = {p.nx, p.ny, p.nz};
     int n = p.nx*p.ny*p.nz;
+    // sum 2 c_q w_q over the links, in units of 1/18 (2 w_q = 2/18 along the axes, 1/18 on the diagonals): summed as
+    // integers it is exact, and exactly zero for a wall with fluid on both sides.
+    long long units[3] = {0, 0, 0};
     for (int node = 0; node < n; node++) {
         if (!c->hostFluid[node])
             continue;
@@ -204,7 +204,7 @@ void findWalls(lbm_context* c, std::vector<int>& nodes, std::vector<int>& links)
             if (!crosses && !c->hostFluid[c->nodeIndex(t[0], t[1], t[2])]) {
                 bits |= 1 << q;
                 for (int a = 0; a < 3; a++)
-                    c->staticWall[a] += 2.0*cq[a]*HW[q];
+                    units[a] += cq[a]*(q < 7 ? 2 : 1);
             }
         }
         if (bits) {
@@ -212,6 +212,8 @@ void findWalls(lbm_context* c, std::vector<int>& nodes, std::vector<int>& links)
             links.push_back(bits);
         }
     }
+    for (int a = 0; a < 3; a++)
+        c->staticWall[a] = (double) units[a]/18.0;
 }
 
 void findBoundaries(lbm_context* c, std::vector<BoundaryNode>& list) {

@@ -688,3 +688,48 @@ def test_pressure_driven_duct(backend):
             change = max(change, abs(velocity[1][node][1] - velocity[0][node][1])*DT/DX)
     assert error < 0.02*umax
     assert change < tol(backend, 1e-9, 1e-4)*umax
+
+
+@pytest.mark.parametrize("stencil", STENCILS)
+@pytest.mark.parametrize("drag", DRAGS)
+@pytest.mark.parametrize("backend", BACKENDS)
+def test_stencil_momentum_conservation(backend, drag, stencil):
+    """TestLBMStencils.h:265-335: with every stencil the momentum of particles and fluid is conserved (overlapping
+    stencils, random force, a particle crossing the periodic boundary); with a wall the momentum given to the solid
+    nodes of the stencils closes the balance."""
+    positions = [[1.02, 2.01, 0.98], [1.27, 1.79, 1.13], [3.98, 0.02, 3.96], [2.3, 1.1, 0.6]]
+    velocities = [[0.5, -0.2, 0.3], [-0.4, 0.1, 0.2], [2.0, -1.5, 1.0], [0.0, 0.3, -0.6]]
+    s = coupling_system(positions, velocities, 10.0, 300.0,
+                        configure=[lambda f: (f.setInterpolationStencil(stencil), f.setDragScheme(drag))])
+    s.state = equilibrium_state(512, 1.01, np.zeros(3))
+    c = build(s, backend)
+    cell = 602.2*DX**3
+
+    def momentum():
+        m, j = total_moments(c.getFluidState())
+        return (100.0*c.velocities).sum(axis=0) + j*cell*DX/DT
+    scale = sum(100.0*np.linalg.norm(v) for v in velocities)
+    p0 = momentum()
+    c.step(50)
+    assert np.linalg.norm(momentum() - p0)/scale < ctol(backend, 1e-11)
+    assert c.positions[2, 0] > 4.0
+
+    # Near the wall j = 0, whose solid nodes the stencils cover; the fluid's momentum from its fields.
+    c = build(coupling_system([[1.1, 0.40, 1.7], [2.3, 0.61, 0.6], [0.7, 3.55, 2.9]],
+                              [[0.5, -1.0, 0.2], [0.4, 0.3, -0.2], [-0.6, 0.2, 0.5]], 10.0, 300.0,
+                              configure=[lambda f: (f.setInterpolationStencil(stencil), f.setDragScheme(drag),
+                                                    f.setSolidNodes(wall_plane_y0()))]), backend)
+
+    def fields_momentum():
+        p = (100.0*c.velocities).sum(axis=0)
+        scale = (100.0*np.linalg.norm(c.velocities, axis=1)).sum()
+        rho, u = c.getFluidFields()
+        return p + (u*(rho*DX**3)[:, None]).sum(axis=0), scale
+    p0, _ = fields_momentum()
+    wall = np.zeros(3)
+    for _ in range(50):
+        c.step(1)
+        wall += np.asarray(c.getWallForce())*DT
+    p1, scale = fields_momentum()
+    assert wall @ wall > 0
+    assert np.linalg.norm(p1 + wall - p0)/scale < (1e-5 if backend == "fp32" else 1e-11)
