@@ -733,3 +733,92 @@ def test_stencil_momentum_conservation(backend, drag, stencil):
     p1, scale = fields_momentum()
     assert wall @ wall > 0
     assert np.linalg.norm(p1 + wall - p0)/scale < (1e-5 if backend == "fp32" else 1e-11)
+
+
+# ------------------------------------------------------------------------------------------------
+# Checkpoints (TestLBMCoupling.h:479-617).  The host's own checkpoint (positions, velocities, step count) is copied
+# from one Context to the other, as OpenMM's Context checkpoint would.
+# ------------------------------------------------------------------------------------------------
+def host_state(c):
+    return c.positions.copy(), c.velocities.copy(), c.getStepCount()
+
+
+def set_host_state(c, state):
+    positions, velocities, step = state
+    c.setPositions(positions)
+    c.setVelocities(velocities)
+    c.setStepCount(step)
+
+
+@pytest.mark.parametrize("drag", DRAGS)
+@pytest.mark.parametrize("backend", BACKENDS)
+def test_checkpoint_with_random_force(backend, drag):
+    """TestLBMCoupling.h:479-527: a run restarted from checkpoints (with the random force, a wall and the removal, a
+    force evaluation just before them) is bit-identical to the uninterrupted run; the wall force is restored."""
+    num_steps, split = 23, 9
+
+    def system():
+        s = coupling_system([[0.3, 0.8, 0.5], [1.3, 1.4, 1.5], [2.3, 2.4, 2.5], [3.1, 0.9, 0.2]],
+                            [[0.1, -0.2, 0.3], [-0.3, 0.2, 0.1], [0.2, 0.2, -0.2], [0.0, -0.4, 0.1]], 10.0, 300.0,
+                            configure=[lambda f: (f.setDragScheme(drag), f.setRandomNumberSeed(3),
+                                                  f.setFluidMomentumRemovalFrequency(4),
+                                                  f.setSolidNodes(wall_plane_y0()))])
+        return build(s, backend)
+
+    def finish(c):
+        for step in range(split, num_steps):
+            c.step(1)
+            if step % 3 == 0:
+                c.getForces()
+        return c.positions.copy(), c.velocities.copy(), c.getFluidState()
+    first = system()
+    first.step(split)
+    first.getForces()
+    data, host, wall = first.createCheckpoint(), host_state(first), np.asarray(first.getWallForce()).copy()
+    uninterrupted = finish(first)
+    second = system()
+    set_host_state(second, host)
+    second.loadCheckpoint(data)
+    assert wall @ wall > 0
+    assert np.array_equal(np.asarray(second.getWallForce()), wall)
+    for a, b in zip(uninterrupted, finish(second)):
+        assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("backend", BACKENDS)
+def test_checkpoint_mismatch(backend):
+    """TestLBMCoupling.h:536-567: a checkpoint is refused with another number of coupled particles, and data that are
+    not a checkpoint are refused."""
+    from lbm_amd import LBMError
+    c2 = build(coupling_system([[0.3, 0.4, 0.5], [1.3, 1.4, 1.5]], [[0, 0, 0]]*2, 5.0, 0.0), backend)
+    data = c2.createCheckpoint()
+    c3 = build(coupling_system([[0.3, 0.4, 0.5], [1.3, 1.4, 1.5], [2.3, 2.4, 2.5]], [[0, 0, 0]]*3, 5.0, 0.0), backend)
+    with pytest.raises(LBMError):
+        c3.loadCheckpoint(data)
+    with pytest.raises(LBMError):
+        c2.loadCheckpoint(b"this is not a checkpoint")
+
+
+@pytest.mark.parametrize("drag", DRAGS)
+@pytest.mark.parametrize("backend", BACKENDS)
+def test_restart_with_particles(backend, drag):
+    """TestLBMCoupling.h:574-617: a run restarted with the fluid restored by setFluidState() is identical to the
+    uninterrupted run (T = 0: the random state is not part of a fluid state)."""
+    num_steps, split = 35, 13
+
+    def system():
+        s = coupling_system([[0.3, 0.4, 0.5], [1.3, 1.4, 1.5], [2.3, 2.4, 2.5]],
+                            [[0.5, -0.2, 0.3], [-0.4, 0.1, 0.2], [1.0, -0.5, 0.0]], 5.0, 0.0,
+                            configure=[lambda f: (f.setDragScheme(drag), f.setFluidMomentumRemovalFrequency(5))])
+        return build(s, backend)
+    a = system()
+    a.step(num_steps)
+    b = system()
+    b.step(split)
+    saved = b.getFluidState()
+    c = system()
+    set_host_state(c, host_state(b))
+    c.setFluidState(saved)
+    c.step(num_steps - split)
+    assert np.array_equal(a.velocities, c.velocities)
+    assert np.array_equal(a.getFluidState(), c.getFluidState())
