@@ -85,10 +85,15 @@ static const int HCX[NQ] = {0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 0, 0, 0, 0, 1, -
 static const int HCY[NQ] = {0, 0, 0, 1, -1, 0, 0, 1, -1, -1, 1, 1, -1, 1, -1, 0, 0, 0, 0};
 static const int HCZ[NQ] = {0, 0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 1, -1, -1, 1, 1, -1, 1, -1};
 
+/* The kernel's part of a checkpoint: this header, the FP64 state [q*N + node], the wall momentum of the last step (3
+   doubles).  Version 2 adds the seed of the random numbers, which a seed of 0 chooses per Context: with it a restart
+   draws the same numbers.  Version 1 (without the seed) is still read. */
 struct CheckpointHeader {
     char magic[8];
     int32_t version, nx, ny, nz, numParticles, dragScheme;
+    uint32_t seed;
 };
+constexpr size_t CHECKPOINT_V1_HEADER = 8 + 6*sizeof(int32_t);
 
 }  // namespace lbm
 
@@ -886,7 +891,8 @@ int lbm_create_checkpoint(lbm_context* c, void* buffer, size_t size) {
             throw Error(LBM_ERROR_INVALID, "lbm_create_checkpoint: buffer too small");
         CheckpointHeader h;
         std::memcpy(h.magic, "LBMAMDC1", 8);
-        h.version = 1;
+        h.version = 2;
+        h.seed = c->seed;
         h.nx = c->p.nx;
         h.ny = c->p.ny;
         h.nz = c->p.nz;
@@ -907,21 +913,29 @@ int lbm_load_checkpoint(lbm_context* c, const void* buffer, size_t size) {
     LBM_TRY({
         size_t stateBytes = NQ*(size_t) c->g.n*sizeof(double);
         CheckpointHeader h;
-        if (size < sizeof(h))
+        if (size < CHECKPOINT_V1_HEADER)
             throw Error(LBM_ERROR_INVALID, "LBMForce: the checkpoint is truncated");
-        std::memcpy(&h, buffer, sizeof(h));
-        if (std::memcmp(h.magic, "LBMAMDC1", 8) != 0 || h.version != 1)
+        std::memcpy(&h, buffer, CHECKPOINT_V1_HEADER);
+        if (std::memcmp(h.magic, "LBMAMDC1", 8) != 0 || (h.version != 1 && h.version != 2))
             throw Error(LBM_ERROR_INVALID, "LBMForce: the data is not a checkpoint of this LBMForce platform");
+        size_t headerBytes = h.version == 1 ? CHECKPOINT_V1_HEADER : sizeof(h);
+        if (size < headerBytes)
+            throw Error(LBM_ERROR_INVALID, "LBMForce: the checkpoint is truncated");
+        std::memcpy(&h, buffer, headerBytes);
         if (h.nx != c->p.nx || h.ny != c->p.ny || h.nz != c->p.nz || h.numParticles != c->numParticles)
             throw Error(LBM_ERROR_INVALID, "LBMForce: the checkpoint was written for another lattice or System");
         if (h.dragScheme != c->p.drag_scheme)
             throw Error(LBM_ERROR_INVALID, "LBMForce: the checkpoint was written with a different drag scheme");
-        if (size < sizeof(h) + stateBytes + 3*sizeof(double))
+        if (size < headerBytes + stateBytes + 3*sizeof(double))
             throw Error(LBM_ERROR_INVALID, "LBMForce: the checkpoint is truncated");
         const char* in = (const char*) buffer;
-        setState(c, (const double*) (in + sizeof(h)));
-        std::memcpy(c->wallOverrideValue, in + sizeof(h) + stateBytes, 3*sizeof(double));
+        std::vector<double> state(NQ*(size_t) c->g.n);
+        std::memcpy(state.data(), in + headerBytes, stateBytes);
+        setState(c, state.data());
+        std::memcpy(c->wallOverrideValue, in + headerBytes + stateBytes, 3*sizeof(double));
         c->wallOverride = true;
+        if (h.version >= 2)
+            c->seed = h.seed;
     })
 }
 

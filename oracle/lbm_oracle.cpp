@@ -1137,10 +1137,13 @@ public:
     }
 };
 
+// The kernel's checkpoint of the GPU library (version 2: with the seed of the random numbers; version 1 without).
 struct CheckpointHeader {
     char magic[8];
     int32_t version, nx, ny, nz, numParticles, dragScheme;
+    uint32_t seed;
 };
+constexpr size_t CHECKPOINT_V1_HEADER = 8 + 6*sizeof(int32_t);
 
 int fail(const OracleError& e) {
     g_lastError = e.what();
@@ -1261,7 +1264,8 @@ int lbm_create_checkpoint(lbm_context* ctx, void* buffer, size_t size) {
             throw OracleError(LBM_ERROR_INVALID, "lbm_create_checkpoint: buffer too small");
         CheckpointHeader h;
         std::memcpy(h.magic, "LBMAMDC1", 8);
-        h.version = 1; h.nx = k.p.nx; h.ny = k.p.ny; h.nz = k.p.nz; h.numParticles = (int) k.particles.size(); h.dragScheme = k.p.drag_scheme;
+        h.version = 2; h.nx = k.p.nx; h.ny = k.p.ny; h.nz = k.p.nz; h.numParticles = (int) k.particles.size(); h.dragScheme = k.p.drag_scheme;
+        h.seed = k.seed;
         char* out = (char*) buffer;
         std::memcpy(out, &h, sizeof(h));
         std::memcpy(out + sizeof(h), k.pop.data(), k.pop.size()*sizeof(double));
@@ -1272,13 +1276,17 @@ int lbm_create_checkpoint(lbm_context* ctx, void* buffer, size_t size) {
 int lbm_load_checkpoint(lbm_context* ctx, const void* buffer, size_t size) {
     ORACLE_TRY({
         OracleKernel& k = ctx->kernel;
-        size_t need = sizeof(CheckpointHeader) + k.pop.size()*sizeof(double) + 3*sizeof(double);
         CheckpointHeader h;
-        if (size < sizeof(h))
+        if (size < CHECKPOINT_V1_HEADER)
             throw OracleError(LBM_ERROR_INVALID, "LBMForce: the checkpoint is truncated");
-        std::memcpy(&h, buffer, sizeof(h));
-        if (std::memcmp(h.magic, "LBMAMDC1", 8) != 0 || h.version != 1)
+        std::memcpy(&h, buffer, CHECKPOINT_V1_HEADER);
+        if (std::memcmp(h.magic, "LBMAMDC1", 8) != 0 || (h.version != 1 && h.version != 2))
             throw OracleError(LBM_ERROR_INVALID, "LBMForce: the data is not a checkpoint of this LBMForce platform");
+        size_t headerBytes = h.version == 1 ? CHECKPOINT_V1_HEADER : sizeof(h);
+        if (size < headerBytes)
+            throw OracleError(LBM_ERROR_INVALID, "LBMForce: the checkpoint is truncated");
+        std::memcpy(&h, buffer, headerBytes);
+        size_t need = headerBytes + k.pop.size()*sizeof(double) + 3*sizeof(double);
         if (h.nx != k.p.nx || h.ny != k.p.ny || h.nz != k.p.nz || h.numParticles != (int) k.particles.size())
             throw OracleError(LBM_ERROR_INVALID, "LBMForce: the checkpoint was written for another lattice or System");
         if (h.dragScheme != k.p.drag_scheme)
@@ -1286,8 +1294,10 @@ int lbm_load_checkpoint(lbm_context* ctx, const void* buffer, size_t size) {
         if (size < need)
             throw OracleError(LBM_ERROR_INVALID, "LBMForce: the checkpoint is truncated");
         const char* in = (const char*) buffer;
-        std::memcpy(k.pop.data(), in + sizeof(h), k.pop.size()*sizeof(double));
-        std::memcpy(k.wallMomentum.x, in + sizeof(h) + k.pop.size()*sizeof(double), 3*sizeof(double));
+        std::memcpy(k.pop.data(), in + headerBytes, k.pop.size()*sizeof(double));
+        std::memcpy(k.wallMomentum.x, in + headerBytes + k.pop.size()*sizeof(double), 3*sizeof(double));
+        if (h.version >= 2)
+            k.seed = h.seed;
         k.stepForcesCurrent = false;
     })
 }

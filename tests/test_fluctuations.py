@@ -255,3 +255,208 @@ def test_velocity_autocorrelation_gpu_against_oracle():
     sigma = np.sqrt(errors[0]**2 + errors[1]**2)
     print("C(t)/C(0) gpu", np.round(curves[0][:10], 3), "oracle", np.round(curves[1][:10], 3))
     assert np.all(diff[1:] <= 5*sigma[1:] + 1e-3)
+
+
+# ------------------------------------------------------------------------------------------------
+# Fluctuations next to boundaries and their bookkeeping (TestLBMFluctuations.h:155-505), on the reference's systems:
+# dx = 0.5 nm, dt = 0.01 ps, rho0 = 602.2 Da/nm^3.
+# ------------------------------------------------------------------------------------------------
+RDX, RRHO = 0.5, 602.2
+
+
+def reference_kT(temperature):
+    """latticeKT (TestLBMFluctuations.h:50-52)."""
+    return BOLTZ*temperature*DT*DT/(RRHO*RDX**5)
+
+
+def low_modes(state, n):
+    """Modes 0-3 (density and momentum) of every node (nodeModes, TestLBMFluctuations.h:40-47)."""
+    s = state.reshape(19, n)
+    cx = np.array([0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 0, 0, 0, 0, 1, -1, -1, 1])
+    cy = np.array([0, 0, 0, 1, -1, 0, 0, 1, -1, -1, 1, 1, -1, 1, -1, 0, 0, 0, 0])
+    cz = np.array([0, 0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 1, -1, -1, 1, 1, -1, 1, -1])
+    return np.stack([s.sum(axis=0), cx @ s, cy @ s, cz @ s], axis=1)
+
+
+def reference_fluid(grid, tau, temperature, seed, configure=()):
+    s = Setup(grid, dx=RDX, dt=DT, tau=tau, density=RRHO)
+
+    def conf(f):
+        f.setFluidFluctuations(True)
+        f.setTemperature(temperature)
+        f.setRandomNumberSeed(seed)
+        for fn in configure:
+            fn(f)
+    s.configure.append(conf)
+    return s
+
+
+@pytest.mark.parametrize("walls", [pytest.param(True, id="regularized-walls"), pytest.param(False, id="dv-faces")])
+@pytest.mark.parametrize("backend", BACKENDS)
+def test_boundary_fluctuations(backend, walls):
+    """TestLBMFluctuations.h:155-213: next to regularized walls and DensityVelocity faces the fluctuating fluid is at
+    equilibrium: momentum along the walls rho kT (2%), across the faces rho kT and density mu rho (5%)."""
+    n, nz, T = 8, 10, 300.0
+    kT = reference_kT(T)
+    mu = 3*kT
+    if walls:
+        solid = [node + n*n*k for k in (0, nz - 1) for node in range(n*n)]
+        conf = [lambda f: (f.setSolidNodes(solid), f.setWallScheme(LBMForce.Regularized))]
+        planes = (1, nz - 2)
+    else:
+        conf = [lambda f: [f.setFaceBoundary(face, LBMForce.DensityVelocity) for face in (LBMForce.ZMin, LBMForce.ZMax)]]
+        planes = (0, nz - 1)
+    ctx = reference_fluid((n, n, nz), 0.62, T, 23, conf).build(library(backend))
+    samples = 1500 if walls else 500
+    ctx.step(500)
+    along = across = density = 0.0
+    for _ in range(samples):
+        ctx.step(10)
+        m = low_modes(ctx.getFluidState(), n*n*nz)
+        for k in planes:
+            p = m[n*n*k:n*n*(k + 1)]
+            along += 0.5*(p[:, 1]**2 + p[:, 2]**2).sum()
+            across += (p[:, 3]**2).sum()
+            density += (p[:, 0]**2).sum()
+    count = 2.0*n*n*samples
+    if walls:
+        assert along/(count*kT) == pytest.approx(1.0, abs=0.02)
+    else:
+        assert across/(count*kT) == pytest.approx(1.0, abs=0.05)
+        assert density/(count*mu) == pytest.approx(1.0, abs=0.05)
+
+
+@pytest.mark.parametrize("backend", BACKENDS)
+def test_moving_face_fluctuations(backend):
+    """TestLBMFluctuations.h:215-264: between DensityVelocity faces moving at -U and +U the mean profile is linear and
+    the face planes are at equilibrium around it (5%)."""
+    n, nz, T, U = 8, 10, 300.0, 0.02
+    kT = reference_kT(T)
+    mu = 3*kT
+    conf = [lambda f: (f.setFaceBoundary(LBMForce.ZMin, LBMForce.DensityVelocity),
+                       f.setFaceBoundary(LBMForce.ZMax, LBMForce.DensityVelocity),
+                       f.setFaceVelocity(LBMForce.ZMin, (-U*RDX/DT, 0, 0)),
+                       f.setFaceVelocity(LBMForce.ZMax, (U*RDX/DT, 0, 0)))]
+    ctx = reference_fluid((n, n, nz), 0.62, T, 29, conf).build(library(backend))
+    ctx.step(3000)
+    samples = 500
+    total = np.zeros((nz, 4))
+    total2 = np.zeros((nz, 4))
+    for _ in range(samples):
+        ctx.step(10)
+        m = low_modes(ctx.getFluidState(), n*n*nz).reshape(nz, n*n, 4)
+        total += m.sum(axis=1)
+        total2 += (m**2).sum(axis=1)
+    count = n*n*samples
+    mean = total/count
+    for k in range(nz):
+        assert mean[k, 1] == pytest.approx(U*(2.0*(k + 1)/(nz + 1) - 1.0), abs=0.05*U)
+    for k in (0, nz - 1):
+        var = total2[k]/count - mean[k]**2
+        assert var[0]/mu == pytest.approx(1.0, abs=0.05)
+        assert var[1]/kT == pytest.approx(1.0, abs=0.05)
+        assert var[3]/kT == pytest.approx(1.0, abs=0.05)
+
+
+def reference_coupled(positions, temperature, seed, configure=()):
+    """createCoupledSystem (TestLBMCoupling.h:40-61) with fluid fluctuations: 8^3, tau = 0.8, friction 5."""
+    s = Setup((8, 8, 8), dx=RDX, dt=DT, tau=0.8, density=RRHO)
+    s.add_particles(positions, mass=100.0)
+
+    def conf(f):
+        f.setFriction(5.0)
+        f.setTemperature(temperature)
+        f.setRandomNumberSeed(seed)
+        f.setFluidFluctuations(True)
+        for fn in configure:
+            fn(f)
+    s.configure.append(conf)
+    return s
+
+
+THREE = [[0.3, 0.8, 0.5], [1.3, 1.4, 1.5], [2.3, 2.4, 2.5]]
+
+
+@pytest.mark.parametrize("backend", BACKENDS)
+def test_fluctuations_reproducible(backend):
+    """TestLBMFluctuations.h:298-330: the same seed gives the same run, also with force evaluations between steps;
+    another seed another run."""
+    runs = []
+    for run in range(3):
+        ctx = reference_coupled(THREE, 300.0, 8 if run == 2 else 7).build(library(backend))
+        for _ in range(20):
+            ctx.step(1)
+            if run == 1:
+                ctx.getForces()
+        runs.append((ctx.getFluidState(), ctx.velocities.copy()))
+    assert np.array_equal(runs[0][0], runs[1][0]) and np.array_equal(runs[0][1], runs[1][1])
+    assert np.abs(runs[0][0] - runs[2][0]).max() > 1e-6
+
+
+@pytest.mark.parametrize("variant", [
+    pytest.param((False, LBMForce.BounceBack, LBMForce.Explicit), id="plain"),
+    pytest.param((True, LBMForce.BounceBack, LBMForce.Explicit), id="bounce-back"),
+    pytest.param((True, LBMForce.Regularized, LBMForce.Explicit), id="regularized"),
+    pytest.param((True, LBMForce.BounceBack, LBMForce.Centered), id="centred")])
+@pytest.mark.parametrize("backend", BACKENDS)
+def test_fluctuations_restart(backend, variant):
+    """TestLBMFluctuations.h:332-398: with seed 0 the two Contexts choose different seeds, and the checkpoint restores
+    the generator: the restarted run is bit-identical, with a wall, open faces and a body force too."""
+    boundaries, wall_scheme, drag = variant
+    num_steps, split = 23, 9
+    conf = [lambda f: f.setDragScheme(drag)]
+    if boundaries:
+        wall = [i + 64*k for k in range(8) for i in range(8)]
+        conf.append(lambda f: (
+            f.setSolidNodes(wall), f.setWallScheme(wall_scheme), f.setFaceBoundary(LBMForce.XMin, LBMForce.Velocity),
+            f.setFaceBoundary(LBMForce.XMax, LBMForce.DensityVelocity), f.setFaceVelocity(LBMForce.XMin, (0.5, 0.2, 0.0)),
+            f.setFaceVelocity(LBMForce.XMax, (0.5, 0.1, 0.0)), f.setFaceDensity(LBMForce.XMax, 1.002*RRHO),
+            f.setBodyAcceleration((0.0, 0.0, 5.0))))
+    first = reference_coupled(THREE, 300.0, 0, conf).build(library(backend))
+    first.step(split)
+    first.getForces()
+    data = first.createCheckpoint()
+    host = (first.positions.copy(), first.velocities.copy(), first.getStepCount())
+    first.step(num_steps - split)
+    second = reference_coupled(THREE, 300.0, 0, conf).build(library(backend))
+    second.setPositions(host[0])
+    second.setVelocities(host[1])
+    second.setStepCount(host[2])
+    second.loadCheckpoint(data)
+    second.step(num_steps - split)
+    assert np.array_equal(first.getFluidState(), second.getFluidState())
+    assert np.array_equal(first.velocities, second.velocities)
+    assert np.array_equal(np.asarray(first.getWallForce()), np.asarray(second.getWallForce()))
+
+
+@pytest.mark.parametrize("backend", BACKENDS)
+def test_fluctuations_with_nve(backend):
+    """TestLBMFluctuations.h:400-425: NVE particles at rest are set in motion by a fluctuating fluid only."""
+    for fluct in (False, True):
+        s = reference_coupled(THREE[:2], 300.0, 1, [lambda f: (f.setCouplingScheme(LBMForce.NVE),
+                                                              f.setFluidFluctuations(fluct))])
+        ctx = s.build(library(backend))
+        ctx.step(20)
+        speed2 = (ctx.velocities**2).sum()
+        assert (speed2 > 0) if fluct else (speed2 == 0.0)
+
+
+@pytest.mark.parametrize("backend", BACKENDS)
+def test_density_face_with_fluctuations(backend):
+    """TestLBMFluctuations.h:471-505: Density faces are refused with fluid fluctuations on every axis, also at zero
+    temperature; without fluctuations, or with a DensityVelocity face, the same lattices are accepted."""
+    from lbm_amd import LBMError
+    for axis in range(3):
+        for run in range(3):
+            density_face = 2*axis + axis % 2
+            other_face = 2*axis + 1 - axis % 2
+            s = reference_coupled([[1.3, 1.4, 1.5]], 0.0 if axis == 1 else 300.0, 1, [lambda f: (
+                f.setFluidFluctuations(run != 1), f.setDragScheme(LBMForce.Centered),
+                f.setFaceBoundary(density_face, LBMForce.DensityVelocity if run == 2 else LBMForce.Density),
+                f.setFaceBoundary(other_face, LBMForce.DensityVelocity if axis == 2 else LBMForce.Velocity),
+                f.setFaceDensity(density_face, 1.001*RRHO))])
+            if run == 0:
+                with pytest.raises(LBMError, match="Density faces cannot be used with fluid fluctuations"):
+                    s.build(library(backend)).step(1)
+            else:
+                s.build(library(backend)).step(1)
