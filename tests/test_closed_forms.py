@@ -822,3 +822,109 @@ def test_restart_with_particles(backend, drag):
     c.step(num_steps - split)
     assert np.array_equal(a.velocities, c.velocities)
     assert np.array_equal(a.getFluidState(), c.getFluidState())
+
+
+# ------------------------------------------------------------------------------------------------
+# Parameters, the Mach check and fluid restarts (TestLBMFluid.h:287-405, TestLBMFluctuations.h:427-469).
+# ------------------------------------------------------------------------------------------------
+@pytest.mark.parametrize("backend", BACKENDS)
+def test_restart_from_fluid_state(backend):
+    """TestLBMFluid.h:287-320: a run restarted with the fluid of setFluidState() is bit-identical (body force, initial
+    velocity, removal every 5 steps)."""
+    num_steps, split = 35, 13
+
+    def system():
+        s = fluid_setup((4, 4, 4), 0.8)
+        s.configure.append(lambda f: (f.setBodyAcceleration((0.5, -0.2, 0.1)),
+                                      f.setInitialFluidVelocity((0.01*DX/DT, 0.0, 0.0)),
+                                      f.setFluidMomentumRemovalFrequency(5)))
+        return build(s, backend)
+    a = system()
+    a.step(num_steps)
+    b = system()
+    b.step(split)
+    saved = b.getFluidState()
+    c = system()
+    c.setStepCount(split)
+    c.setFluidState(saved)
+    c.step(num_steps - split)
+    assert c.getStepCount() == num_steps
+    assert np.array_equal(a.getFluidState(), c.getFluidState())
+
+
+@pytest.mark.parametrize("backend", BACKENDS)
+def test_update_parameters(backend):
+    """TestLBMFluid.h:322-377: updateParametersInContext changes the body force, the removal and the Mach check, and
+    refuses a change of the viscosity."""
+    from lbm_amd import LBMError
+    s = fluid_setup((4, 4, 4), 0.8)
+    s.configure.append(lambda f: f.setBodyAcceleration((0.5, 0, 0)))
+    c = build(s, backend)
+    force = c.force
+    c.step(5)
+    assert total_moments(c.getFluidState())[1][0] > 1e-6
+    force.setBodyAcceleration((0, 0, 0))
+    force.setFluidMomentumRemovalFrequency(1)
+    c.updateParametersInContext()
+    c.step(1)
+    assert np.allclose(total_moments(c.getFluidState())[1], 0, atol=tol(backend, 1e-13), rtol=0)
+    force.setBodyAcceleration((0.5, 0, 0))
+    force.setFluidMomentumRemovalFrequency(0)
+    c.updateParametersInContext()
+    c.step(5)
+    force.setMachCheckFrequency(1)
+    force.setMachNumberLimit(0.5*c.getFluidMachNumber())
+    c.updateParametersInContext()
+    with pytest.raises(LBMError, match="Mach number"):
+        c.step(1)
+    force.setKinematicViscosity(2.0*force.getKinematicViscosity())
+    with pytest.raises(LBMError, match="cannot be changed"):
+        c.updateParametersInContext()
+
+
+@pytest.mark.parametrize("variant", [0, 1, 2])
+@pytest.mark.parametrize("backend", BACKENDS)
+def test_mach_number_check(backend, variant):
+    """TestLBMFluid.h:379-405: the Mach number is reported and checked every N steps: at Ma = 0.35 the step that checks
+    throws with the default limit, not without the check or with the limit 0.5."""
+    from lbm_amd import LBMError
+    mach = 0.35
+    speed = mach/math.sqrt(3.0)
+    s = fluid_setup((4, 4, 4), 0.8)
+
+    def conf(f):
+        f.setInitialFluidVelocity((speed*DX/DT, 0, 0))
+        f.setMachCheckFrequency(0 if variant == 1 else 10)
+        if variant == 2:
+            f.setMachNumberLimit(0.5)
+    s.configure.append(conf)
+    c = build(s, backend)
+    assert c.getFluidMachNumber() == pytest.approx(mach, abs=tol(backend, 1e-12))
+    c.step(9)
+    if variant == 0:
+        with pytest.raises(LBMError, match="Mach number"):
+            c.step(1)
+    else:
+        c.step(1)
+
+
+@pytest.mark.parametrize("backend", BACKENDS)
+def test_fluctuation_parameters(backend):
+    """TestLBMFluctuations.h:427-469: updateParametersInContext changes the temperature but cannot switch the
+    fluctuations; a checkpoint is refused by a Context with the fluctuations switched differently."""
+    from lbm_amd import LBMError
+    positions = [[0.3, 0.8, 0.5], [1.3, 1.4, 1.5]]
+    c = build(coupling_system(positions, [[0, 0, 0]]*2, 5.0, 300.0,
+                              configure=[lambda f: f.setFluidFluctuations(True)]), backend)
+    c.step(2)
+    c.force.setTemperature(310.0)
+    c.updateParametersInContext()
+    c.step(1)
+    c.force.setFluidFluctuations(False)
+    with pytest.raises(LBMError):
+        c.updateParametersInContext()
+    c.force.setFluidFluctuations(True)
+    data = c.createCheckpoint()
+    other = build(coupling_system(positions, [[0, 0, 0]]*2, 5.0, 300.0), backend)
+    with pytest.raises(LBMError):
+        other.loadCheckpoint(data)
