@@ -6,6 +6,8 @@ evaluation, then the integrator update v(t+dt/2) = v(t-dt/2) + dt F(t)/m, x(t+dt
 Particles are held on the host in FP64; the kernel library is the gfx950 library unless another library with
 the same C-ABI is given (the tests pass the CPU oracle)."""
 import ctypes
+import math
+import sys
 
 import numpy as np
 
@@ -44,11 +46,15 @@ class Context:
     the System, evaluated before the LBMForce as OpenMM does when LBMForce is the last force."""
 
     def __init__(self, system, integrator, force, library=None, extraForces=()):
+        if not isinstance(integrator, VerletIntegrator):
+            raise _lib.LBMError(_lib.LBM_ERROR_INVALID, "LBMForce requires a VerletIntegrator: drag and random forces "
+                                                        "are part of the force")
         self.system = system
         self.integrator = integrator
         self.force = force
         self.lib = library if library is not None else _lib.load_library()
         self.params = force.computeLatticeParameters(system.box, integrator.stepSize, system.masses)
+        self._warn()
         n = system.getNumParticles()
         self.masses = np.array(system.masses, dtype=np.float64)
         self.positions = np.zeros((n, 3))
@@ -58,6 +64,7 @@ class Context:
         parts = [force.getParticle(i) for i in range(force.getNumParticles())]
         self.coupled = parts
         solid = sorted(force.getSolidNodes())
+        self.solid = solid
         self.numNodes = self.params.nx*self.params.ny*self.params.nz
         solidArr = (ctypes.c_int*max(1, len(solid)))(*solid)
         partArr = (ctypes.c_int*max(1, len(parts)))(*parts)
@@ -69,6 +76,69 @@ class Context:
 
     def _check(self, status):
         _lib.check(self.lib, status)
+
+    def _warn(self):
+        """The warnings of LBMForceImpl::initialize (LBMForceImpl.cpp:256-355), printed on stderr when the Context is
+        created: a relaxation time outside [0.505, 2]; tau > 1.7 with the explicit drag (the hydrodynamic
+        self-mobility of a particle at a node is small, and negative above 1.79), except with the three-point stencil;
+        friction*dt > 1 with the explicit drag; the heating of the explicit drag in a fluctuating fluid,
+        friction*dt*m*K/(2 m_c) for the heaviest coupled particle with K = 1, 8/27, 1/8, (57/70)^3 the self weight of
+        the stencil averaged over a cell; the size of the fluctuations in lattice units (validated up to
+        kT = 1/3000); and regularized walls or DensityVelocity faces with fluctuations outside 0.586 <= tau <= 3.41,
+        where the stress amplitude sqrt(1 - c), c = (1 - omega)^2/(omega (2 - omega)), of their noise is not real."""
+        p, f = self.params, self.force
+        out = []
+        coupled = f.getNumParticles() > 0
+        explicit = p.drag_scheme == f.Explicit
+        if p.tau < 0.505 or p.tau > 2.0:
+            out.append(f"Warning: LBMForce: the relaxation time tau = {p.tau:.6g} is outside the range [0.505, 2] in "
+                       "which the lattice Boltzmann model is accurate. tau = 3 nu dt/dx^2 + 1/2: change the viscosity, "
+                       "the time step or the lattice spacing.")
+        if coupled and explicit and p.tau > 1.7 and p.interpolation_stencil != f.ThreePoint:
+            out.append(f"Warning: LBMForce: tau = {p.tau:.6g} > 1.7: with the explicit drag at the nearest node the "
+                       "hydrodynamic self-mobility of a coupled particle is small, and negative above tau = 1.79, so "
+                       "particles move less than they should. Reduce the viscosity or the time step, or use a coarser "
+                       "lattice.")
+        gammaDt = p.friction*p.dt
+        if coupled and explicit and gammaDt > 1.0:
+            grows = ", and grows without bound since friction*dt >= 2" if gammaDt >= 2.0 else ""
+            out.append(f"Warning: LBMForce: friction*dt = {gammaDt:.6g} > 1: with the explicit drag the velocity of a "
+                       f"particle relative to the fluid changes sign at every step{grows}. Reduce the friction or the "
+                       "time step.")
+        cellMass = p.density*p.dx**3
+        if coupled and explicit and p.fluid_fluctuations and p.kT > 0 and gammaDt > 0:
+            heaviest = max(self.system.masses[i] for i in
+                           (f.getParticle(k) for k in range(f.getNumParticles())))
+            self_weight = {f.Trilinear: 8.0/27.0, f.ThreePoint: 0.125,
+                           f.Keys: (57.0/70.0)**3}.get(p.interpolation_stencil, 1.0)
+            text = ("Warning: LBMForce: with fluid fluctuations the explicit drag makes the coupled particles hotter "
+                    f"than the set temperature, by about friction*dt*m{'*K' if self_weight < 1 else ''}/(2 m_c) = "
+                    f"{100.0*gammaDt*heaviest*self_weight/(2.0*cellMass):.6g}% for the heaviest one "
+                    f"(m_c = {cellMass:.6g} Da is the mass of fluid in a cell")
+            if self_weight < 1:
+                text += f", K = {self_weight:.6g} the self weight of the interpolation stencil averaged over a cell"
+            out.append(text + "). Use the Centered drag scheme with fluid fluctuations.")
+        if p.fluid_fluctuations and p.fluid_kT > 0:
+            latticeKT = p.fluid_kT*p.dt**2/(cellMass*p.dx**2)
+            validated = 1.0/3000.0
+            out.append(f"LBMForce: fluctuating fluid: kT/(m_c (dx/dt)^2) = {latticeKT:.2g} in lattice units, thermal "
+                       f"Mach number sqrt(3 kT) = {math.sqrt(3*latticeKT):.2g} (validated up to kT = 1/3000, Mach number "
+                       f"{math.sqrt(3*validated):.2g})")
+            if latticeKT > validated*(1.0 + 1e-6):
+                out.append(f"Warning: LBMForce: the fluctuations of the fluid, kT = {latticeKT:.2g} in lattice units, "
+                           "are larger than those validated, kT = 1/3000: the fluctuating fluid may become unstable "
+                           "(at 1/3000 it is for tau <= 0.501). A larger lattice spacing or a smaller time step makes "
+                           "them smaller (kT in lattice units goes as dt^2/dx^5).")
+            noisy = (len(f.getSolidNodes()) > 0 and p.wall_scheme == f.Regularized) or \
+                any(b == f.DensityVelocity for b in p.face_boundary)
+            w = p.omega
+            if noisy and (1.0 - w)**2/(w*(2.0 - w)) > 1.0:
+                out.append("Warning: LBMForce: with fluid fluctuations, regularized walls and DensityVelocity faces "
+                           "keep the fluid next to them exactly at equilibrium only for 0.586 <= tau <= 3.41; "
+                           f"tau = {p.tau:.6g} is outside, and the fluid next to them is slightly away from "
+                           "equilibrium.")
+        for line in out:
+            print(line, file=sys.stderr)
 
     def __del__(self):
         handle = getattr(self, "handle", None)
@@ -95,7 +165,15 @@ class Context:
             forces += f(self.positions)
         return forces
 
+    def _checkStepSize(self):
+        """LBMForceImpl::calcForcesAndEnergy (LBMForceImpl.cpp:373-380): the lattice time step is the step size of
+        the integrator when the Context was created."""
+        if self.integrator.stepSize != self.params.dt:
+            raise _lib.LBMError(_lib.LBM_ERROR_INVALID, "LBMForce: the integrator step size changed after the Context "
+                                                        "was created; reinitialize the Context")
+
     def step(self, numSteps=1):
+        self._checkStepSize()
         dt = self.integrator.stepSize
         invMass = np.where(self.masses > 0, 1.0/np.where(self.masses > 0, self.masses, 1.0), 0.0)[:, None]
         for _ in range(int(numSteps)):
@@ -111,6 +189,7 @@ class Context:
 
     def getForces(self):
         """The forces of a State (a force evaluation between steps): those of the next step."""
+        self._checkStepSize()
         pos = np.ascontiguousarray(self.positions)
         vel = np.ascontiguousarray(self.velocities)
         forces = np.ascontiguousarray(self._otherForces())
@@ -168,11 +247,15 @@ class Context:
 
     def updateParametersInContext(self):
         """LBMForceImpl::updateParametersInContext (LBMForceImpl.cpp:388-416)."""
-        updated = self.force.computeLatticeParameters(self.system.box, self.integrator.stepSize, self.system.masses)
+        force = self.force
+        updated = force.computeLatticeParameters(self.system.box, self.params.dt, self.system.masses)
         old = self.params
         checks = [
             ((updated.nx, updated.ny, updated.nz) != (old.nx, old.ny, old.nz), "the grid size cannot be changed"),
             (updated.density != old.density or updated.tau != old.tau, "the fluid density and viscosity cannot be changed"),
+            ([force.getParticle(i) for i in range(force.getNumParticles())] != self.coupled,
+             "the set of coupled particles cannot be changed"),
+            (sorted(force.getSolidNodes()) != self.solid, "the solid nodes cannot be changed"),
             (updated.wall_scheme != old.wall_scheme, "the wall scheme cannot be changed"),
             (list(updated.face_boundary) != list(old.face_boundary), "the boundary types of the faces cannot be changed"),
             (updated.drag_scheme != old.drag_scheme, "the drag scheme cannot be changed"),

@@ -1377,6 +1377,18 @@ int oracle_num_boundary_nodes(lbm_context* ctx) {
     return (int) ctx->kernel.bNodes.size();
 }
 
+/* The kind (internal/LBMBoundaries.h:30) and the face of a boundary node; 0 if the node is not one. */
+int oracle_boundary_node(lbm_context* ctx, int node, int* kind, int* face) {
+    const auto& k = ctx->kernel;
+    for (size_t b = 0; b < k.bNodes.size(); b++)
+        if (k.bNodes[b] == node) {
+            *kind = k.bKind[b];
+            *face = k.bFace[b];
+            return 1;
+        }
+    return 0;
+}
+
 uint32_t oracle_seed(lbm_context* ctx) {
     return ctx->kernel.seed;
 }

@@ -26,6 +26,7 @@ HOOKS = {
     "oracle_equilibrium_noise": (None, [_D, ctypes.c_double, ctypes.c_double, ctypes.c_double, _D]),
     "oracle_num_boundary_nodes": (ctypes.c_int, [ctypes.c_void_p]),
     "oracle_seed": (ctypes.c_uint32, [ctypes.c_void_p]),
+    "oracle_boundary_node": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _I, _I]),
 }
 
 REF_FUNCTIONS = {

@@ -928,3 +928,248 @@ def test_fluctuation_parameters(backend):
     other = build(coupling_system(positions, [[0, 0, 0]]*2, 5.0, 300.0), backend)
     with pytest.raises(LBMError):
         other.loadCheckpoint(data)
+
+
+# ------------------------------------------------------------------------------------------------
+# The force's own suite (TestLBMForce.h:126-218) and the remaining fluid, boundary and coupling cases.
+# ------------------------------------------------------------------------------------------------
+def stol(backend):
+    """getStorageTolerance (TestLBMForce.h:37-42)."""
+    return 2e-6 if backend == "fp32" else 1e-12
+
+
+def force_system(configure=()):
+    """createSystem and createPositions (TestLBMForce.h:47-65): 8^3 nodes in a 4 nm box, ten coupled particles of
+    100 Da, the default density, viscosity and removal of LBMForce."""
+    d = LBMForce()
+    s = Setup((8, 8, 8), dx=0.5, dt=0.01, density=d.getFluidDensity())
+    s.add_particles([[0.37*i, 0.53*i + 0.1, 0.71*i + 0.2] for i in range(10)], mass=100.0)
+
+    def conf(f):
+        f.setKinematicViscosity(d.getKinematicViscosity())
+        f.setFluidMomentumRemovalFrequency(d.getFluidMomentumRemovalFrequency())
+        for fn in configure:
+            fn(f)
+    s.configure.append(conf)
+    return s
+
+
+@pytest.mark.parametrize("backend", BACKENDS)
+def test_zero_force(backend):
+    """TestLBMForce.h:126-138: no coupling force before the first step."""
+    c = build(force_system(), backend)
+    assert np.all(c.getForces() == 0.0)
+    c.step(5)
+
+
+@pytest.mark.parametrize("backend", BACKENDS)
+def test_initial_fluid_fields(backend):
+    """TestLBMForce.h:143-163: the fluid starts at equilibrium with the density and velocity of the force."""
+    u0 = np.array([0.1, -0.05, 0.02])
+    s = force_system([lambda f: (f.setFluidDensity(500.0), f.setInitialFluidVelocity(u0))])
+    s.density = 500.0
+    c = build(s, backend)
+    rho, u = c.getFluidFields()
+    assert rho.shape == (512,) and u.shape == (512, 3)
+    assert np.abs(rho/500.0 - 1).max() <= 5*stol(backend)
+    assert np.abs(u - u0).max() <= 1e3*stol(backend)
+
+
+@pytest.mark.parametrize("backend", BACKENDS)
+def test_fluid_state_round_trip(backend):
+    """TestLBMForce.h:168-218: the state (deviations from the rest equilibrium) reads zero at rest; moving 0.01 of
+    the rest population of node 5 to population 1 keeps its density and moves it along +x at 0.01 dx/dt; a state
+    of the wrong size is refused."""
+    from lbm_amd import LBMError
+    c = build(force_system(), backend)
+    t = stol(backend)
+    state = c.getFluidState()
+    n = 512
+    assert state.size == 19*n
+    assert np.abs(state).max() <= t
+    delta = 0.01
+    state[5] -= delta
+    state[n + 5] += delta
+    c.setFluidState(state)
+    assert np.abs(c.getFluidState() - state).max() <= t
+    rho, u = c.getFluidFields()
+    density = c.force.getFluidDensity()
+    assert abs(rho[5]/density - 1) <= 5*t
+    assert np.abs(u[5] - [delta*4.0/8/0.01, 0, 0]).max() <= 1e3*t
+    assert np.abs(u[6]).max() <= 1e3*t
+    with pytest.raises(LBMError):
+        c.setFluidState(state[:10])
+
+
+@pytest.mark.parametrize("backend", BACKENDS)
+def test_lattice_parameters(backend):
+    """TestLBMFluid.h:412-424."""
+    dx, dt, tau = build(fluid_setup((4, 6, 8), 0.9), backend).getLatticeParameters()
+    assert dx == pytest.approx(DX, rel=1e-14) and dt == pytest.approx(DT, rel=1e-14)
+    assert tau == pytest.approx(0.9, abs=1e-12)
+
+
+def boundary_setup(scheme, y_faces):
+    """TestLBMFluid.h:965-975: 4x5x4, the wall plane j = 0, open faces perpendicular to y."""
+    s = fluid_setup((4, 5, 4), 0.8)
+    s.positions = []
+    s.add_particles([[0.1, 1.2, 0.3]], mass=1.0, coupled=False)
+    s.configure.append(lambda f: (f.setSolidNodes(wall_plane(4, 5, 4)), f.setWallScheme(scheme),
+                                  f.setFaceBoundary(LBMForce.YMin, y_faces), f.setFaceBoundary(LBMForce.YMax, y_faces)))
+    return s
+
+
+@pytest.mark.parametrize("backend", BACKENDS)
+def test_checkpoint_boundaries(backend):
+    """TestLBMFluid.h:965-1008: a checkpoint is loaded with the same wall scheme and faces, and refused with another
+    wall scheme or other faces."""
+    from lbm_amd import LBMError
+    s = boundary_setup(LBMForce.Regularized, LBMForce.Density)
+    s.masses, s.positions, s.velocities, s.coupled = [], [], [], []
+    s.add_particles([[0.1, 1.2, 0.3]], mass=1.0, coupled=False)
+    c = build(s, backend)
+    c.step(3)
+    data = c.createCheckpoint()
+    for scheme, faces, message in [(LBMForce.Regularized, LBMForce.Density, None),
+                                   (LBMForce.BounceBack, LBMForce.Density, "wall scheme"),
+                                   (LBMForce.Regularized, LBMForce.Velocity, "boundary types of the faces")]:
+        other = build(boundary_setup(scheme, faces), backend)
+        if message is None:
+            other.loadCheckpoint(data)
+            assert np.array_equal(other.getFluidState(), c.getFluidState())
+        else:
+            with pytest.raises(LBMError, match=message):
+                other.loadCheckpoint(data)
+
+
+@pytest.mark.parametrize("scheme", [LBMForce.BounceBack, LBMForce.Regularized], ids=["bounce-back", "regularized"])
+@pytest.mark.parametrize("backend", BACKENDS)
+def test_fluid_state_restart_with_boundaries(backend, scheme):
+    """TestLBMFluid.h:1015-1043: with walls, a Velocity inlet, a Density outlet (whose filter reads the previous
+    velocity from the populations) and a body force, a run restarted from the fluid state is bit-identical."""
+    num_steps, split = 30, 11
+
+    def system():
+        s = fluid_setup((6, 5, 4), 0.8)
+        s.configure.append(lambda f: (
+            f.setSolidNodes(wall_plane(6, 5, 4)), f.setWallScheme(scheme),
+            f.setFaceBoundary(LBMForce.XMin, LBMForce.Velocity), f.setFaceBoundary(LBMForce.XMax, LBMForce.Density),
+            f.setFaceVelocity(LBMForce.XMin, (0.4, 0.1, 0.0)), f.setFaceDensity(LBMForce.XMax, 1.003*RHO_F),
+            f.setBodyAcceleration((0.0, 0.0, 2.0))))
+        return build(s, backend)
+    a = system()
+    a.step(split)
+    saved = a.getFluidState()
+    a.step(num_steps - split)
+    b = system()
+    b.setFluidState(saved)
+    b.step(num_steps - split)
+    assert np.array_equal(a.getFluidState(), b.getFluidState())
+
+
+SIX_FACES = [LBMForce.Density, LBMForce.Velocity, LBMForce.DensityVelocity, LBMForce.Density, LBMForce.Density,
+             LBMForce.DensityVelocity]
+
+
+def six_face_setup():
+    """TestLBMFluid.h:885-895: 4^3 with every face open, of the types Density, Velocity, DensityVelocity, Density,
+    Density, DensityVelocity (XMin ... ZMax), here with a velocity and a density of its own on every face."""
+    s = fluid_setup((4, 4, 4), 0.8)
+    velocity = {1: (0.01, 0.002, -0.003), 2: (-0.004, 0.012, 0.001), 5: (0.003, -0.002, 0.008)}
+    density = {0: 1.02, 2: 1.01, 3: 0.98, 4: 1.03, 5: 0.99}
+
+    def conf(f):
+        for face, kind in enumerate(SIX_FACES):
+            f.setFaceBoundary(face, kind)
+        for face, v in velocity.items():
+            f.setFaceVelocity(face, tuple(x*DX/DT for x in v))
+        for face, r in density.items():
+            f.setFaceDensity(face, r*RHO_F)
+    s.configure.append(conf)
+    return s
+
+
+def test_boundary_kinds():
+    """TestLBMFluid.h:885-915: on nodes shared by several faces the first face (XMin ... ZMax) that sets the velocity
+    gives the velocity, and the density too if it is a DensityVelocity face; nodes of Density faces take the density
+    of their face, or, on several Density faces, of the first one with zero velocity (the oracle's classification)."""
+    WALL, VELOCITY, DENSITY, AT_REST, DENSITY_VELOCITY = range(5)
+    c = build(six_face_setup(), "oracle")
+    lib = c.lib
+    assert lib.oracle_num_boundary_nodes(c.handle) == 4**3 - 2**3
+    expected = [((0, 1, 1), DENSITY, 0), ((3, 1, 1), VELOCITY, 1), ((1, 0, 1), DENSITY_VELOCITY, 2),
+                ((0, 0, 1), DENSITY_VELOCITY, 2), ((3, 0, 1), VELOCITY, 1), ((1, 3, 0), AT_REST, 3),
+                ((0, 3, 0), AT_REST, 0), ((0, 3, 3), DENSITY_VELOCITY, 5), ((3, 0, 3), VELOCITY, 1)]
+    for (i, j, k), kind, face in expected:
+        got_kind, got_face = ctypes.c_int(), ctypes.c_int()
+        assert lib.oracle_boundary_node(c.handle, i + 4*(j + 4*k), ctypes.byref(got_kind), ctypes.byref(got_face))
+        assert (got_kind.value, got_face.value) == (kind, face), (i, j, k)
+    assert not lib.oracle_boundary_node(c.handle, 1 + 4*(1 + 4*1), ctypes.byref(got_kind), ctypes.byref(got_face))
+
+
+@pytest.mark.gpu
+def test_boundary_kinds_gpu():
+    """The six open faces of TestLBMFluid.h:885-915 on the gfx950 library (FP64 storage) against the oracle, step by
+    step: a node classified differently (kind or face) takes another density or velocity at once."""
+    from parity import lockstep
+    worst, _, _ = lockstep(six_face_setup(), 20, storage=LBMForce.FP64, forces=False)
+    assert worst["density"] < 1e-12 and worst["velocity"] < 1e-11, worst
+
+
+@pytest.mark.parametrize("drag", DRAGS)
+@pytest.mark.parametrize("backend", BACKENDS)
+def test_full_step_kinetic_energy(backend, drag):
+    """TestLBMCoupling.h:111-140: OpenMM's kinetic energy of VerletIntegrator, from v(t - dt/2) + dt F(t)/(2m) with the
+    forces of a State, is that of the full step, sum m ((v(t - dt/2) + v(t + dt/2))/2)^2/2: the force between steps is
+    the one of the next step (random force and removal every 2 steps included)."""
+    n = 6
+    positions = [[0.6*i, 0.4*i + 0.3, 3.7 - 0.5*i] for i in range(n)]
+    velocities = [[0.2*math.cos(i), -0.3*math.sin(i), 0.1*i - 0.25] for i in range(n)]
+    c = build(coupling_system(positions, velocities, 10.0, 300.0, configure=[
+        lambda f: (f.setDragScheme(drag), f.setRandomNumberSeed(5), f.setFluidMomentumRemovalFrequency(2))]), backend)
+    c.step(5)
+    for _ in range(4):
+        v = c.velocities.copy()
+        f = c.getForces()
+        reported = 0.5*100.0*((v + 0.5*DT*f/100.0)**2).sum()
+        c.step(1)
+        expected = 0.5*100.0*((0.5*(v + c.velocities))**2).sum()
+        assert reported == pytest.approx(expected, rel=ctol(backend, 1e-10))
+
+
+@pytest.mark.parametrize("backend", BACKENDS)
+def test_repeated_force_evaluation(backend):
+    """TestLBMCoupling.h:309-330 and CommonLBMKernels.cpp:1365-1374: OpenMM repeats every force evaluation of a step
+    whose neighbour list overflowed, before the step count advances.  The repeated evaluation returns the coupling
+    forces of the step again, bit for bit, and leaves the fluid alone: the step conserves the total momentum and the
+    run is the same as without the repeat."""
+    positions = [[0.3 + 0.45*i, 0.4 + 0.3*i, 0.5 + 0.35*i] for i in range(8)]
+    velocities = [[0.2*math.cos(i), -0.3*math.sin(i), 0.1*i - 0.25] for i in range(8)]
+
+    def system():
+        c = build(coupling_system(positions, velocities, 10.0, 300.0), backend)
+        c.step(5)
+        return c
+    a, b = system(), system()
+    cell = RHO_F*DX**3
+
+    def momentum(c):
+        m, j = total_moments(c.getFluidState())
+        return (100.0*c.velocities).sum(axis=0) + j*cell*DX/DT
+    scale = (100.0*np.linalg.norm(a.velocities, axis=1)).sum()
+    p0 = momentum(a)
+    ptr = lambda x: x.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+    pos, vel = a.positions.copy(), a.velocities.copy()
+    a._check(a.lib.lbm_begin_step(a.handle, a.stepCount, ptr(pos), ptr(vel)))
+    first = np.zeros_like(pos)
+    a._check(a.lib.lbm_execute(a.handle, a.stepCount, 1, ptr(pos), ptr(vel), ptr(first)))
+    repeated = np.zeros_like(pos)
+    a._check(a.lib.lbm_execute(a.handle, a.stepCount, 1, ptr(pos), ptr(vel), ptr(repeated)))
+    assert np.array_equal(first, repeated)
+    vel = vel + DT*repeated*(1.0/100.0)        # Context.step's update
+    a.positions, a.velocities = pos + DT*vel, vel
+    a.stepCount += 1
+    assert np.linalg.norm(momentum(a) - p0)/scale < ctol(backend, 1e-11)
+    b.step(1)
+    assert np.array_equal(a.velocities, b.velocities)
+    assert np.array_equal(a.getFluidState(), b.getFluidState())
