@@ -225,3 +225,84 @@ def test_centred_parity_with_fluctuations(stencil):
     print("worst relative errors", worst)
     for key in ("density", "velocity", "force"):
         assert worst[key] < 1e-4, (key, worst)
+
+
+SOLVE_BACKENDS = [pytest.param("oracle", id="oracle"),
+                  pytest.param("fp64", id="gpu-fp64", marks=pytest.mark.gpu),
+                  pytest.param("fp32", id="gpu-fp32", marks=pytest.mark.gpu)]
+
+
+@pytest.mark.parametrize("open_z", [False, True], ids=["periodic", "open-z"])
+@pytest.mark.parametrize("stencil", [LBMForce.Trilinear, LBMForce.ThreePoint, LBMForce.Keys])
+@pytest.mark.parametrize("backend", SOLVE_BACKENDS)
+def test_centred_stencil_solve(backend, stencil, open_z):
+    """TestLBMStencils.h:341-491: the coupling forces of the first step against the solution, by Gaussian elimination,
+    of (1 + a) F_k + a m_k sum_l K_kl F_l = -gamma m_k (v~_k - U_k) (lattice units, m_k = m/m_c, a = gamma/2,
+    K_kl = sum_j xi_jk xi_jl/rho_j and U_k = sum_j xi_jk u_j over the fluid nodes), built here from the fluid state, the
+    positions, the velocities and a constant field (3, -2, 1) kJ/mol/nm: a non-uniform fluid, the wall plane j = 0 under
+    the stencils, three particles with overlapping stencils and an isolated one, masses 100 to 1000 Da; then with
+    Velocity faces along z and two particles near them, one between node n - 1 and the end of the box.  The weights
+    xi come from the oracle's stencil (pinned by the stencil moments and the reference's stencil tests)."""
+    import ctypes
+    n, num = 8, 4
+    nodes = n**3
+    friction, h = 20.0, 0.5
+    masses = np.array([100.0, 300.0, 1000.0, 150.0])
+    positions = np.array([[1.13, 0.62, 1.71], [1.37, 0.81, 1.52], [1.02, 1.04, 1.93], [3.1, 2.9, 0.4]])
+    if open_z:
+        positions[2, 2], positions[3, 2] = 0.15, 3.85
+    velocities = np.array([[0.5, -0.2, 0.3], [-0.4, 0.1, 0.2], [0.2, -0.5, 0.1], [0.3, 0.3, -0.6]])
+    node = np.arange(nodes)
+    from parity import CX, CY, CZ, equilibrium_state
+    state = equilibrium_state(nodes, 1.0 + 0.002*(node % 5),
+                              np.stack([0.01*np.sin(node), 0.01*np.cos(node), 0.005*np.sin(3.0*node)], axis=1))
+    solid = [i + n*n*k for k in range(n) for i in range(n)]
+    field = np.array([3.0, -2.0, 1.0])
+    s = centred_setup(list(masses), positions, velocities, friction, stencil=stencil,
+                      external=np.tile(field, (num, 1)), solid=solid)
+    if open_z:
+        s.configure.append(lambda f: (f.setFaceBoundary(LBMForce.ZMin, LBMForce.Velocity),
+                                      f.setFaceBoundary(LBMForce.ZMax, LBMForce.Velocity)))
+    s.state = state
+    if backend == "oracle":
+        ctx = s.build(library(backend))
+    else:
+        ctx = s.build(None, storage=LBMForce.FP64 if backend == "fp64" else LBMForce.FP32)
+
+    # The fluid before the step and the system of the centred drag, in lattice units.
+    f = state.reshape(19, nodes)
+    fluid = np.ones(nodes, dtype=bool)
+    fluid[solid] = False
+    rho = 1.0 + f.sum(axis=0)
+    u = np.stack([(CX[:, None]*f).sum(axis=0), (CY[:, None]*f).sum(axis=0), (CZ[:, None]*f).sum(axis=0)], axis=1)/rho[:, None]
+    lib = oracle_lib.load_oracle()
+    width = {LBMForce.Trilinear: 2, LBMForce.ThreePoint: 3, LBMForce.Keys: 4}[stencil]
+    xi = np.zeros((num, nodes))
+    for i in range(num):
+        axes = []
+        for k in range(3):
+            idx, wt = (ctypes.c_int*4)(), (ctypes.c_double*4)()
+            lib.oracle_axis_weights(stencil, positions[i, k]/DX, n, int(open_z and k == 2), idx, wt)
+            axes.append((list(idx)[:width], list(wt)[:width]))
+        for c in range(width):
+            for b in range(width):
+                for aa in range(width):
+                    xi[i, axes[0][0][aa] + n*(axes[1][0][b] + n*axes[2][0][c])] += \
+                        axes[0][1][aa]*axes[1][1][b]*axes[2][1][c]
+    xi[:, ~fluid] = 0.0
+    force_scale = CELL*DX/DT**2
+    gamma = friction*DT
+    a = gamma*h
+    m = masses/CELL
+    known = velocities*(DT/DX) + (field/force_scale)[None, :]*(h/m)[:, None]
+    U = xi @ u
+    K = (xi/np.where(fluid, rho, 1.0)[None, :]) @ xi.T
+    A = np.diag(np.full(num, 1.0 + a)) + a*m[:, None]*K
+    expected = np.linalg.solve(A, -gamma*m[:, None]*(known - U))*force_scale
+
+    ctx.step(1)
+    coupling = (ctx.velocities - velocities)*(masses/DT)[:, None] - field[None, :]
+    err = np.linalg.norm(coupling - expected, axis=1)/np.linalg.norm(expected, axis=1)
+    print("relative error of the coupling forces", err.max())
+    # FP32 storage rounds the fluid, the populations of the solve, to 6e-8 (measured: 1.2e-8 at worst).
+    assert err.max() <= {"oracle": 1e-10, "fp64": 1e-10, "fp32": 1e-7}[backend]

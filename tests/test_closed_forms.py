@@ -1173,3 +1173,24 @@ def test_repeated_force_evaluation(backend):
     b.step(1)
     assert np.array_equal(a.velocities, b.velocities)
     assert np.array_equal(a.getFluidState(), b.getFluidState())
+
+
+@pytest.mark.parametrize("backend", BACKENDS)
+def test_queries_do_not_advance_fluid(backend):
+    """TestLBMFluid.h:257-280: force evaluations and new particle velocities between steps leave the fluid of a run
+    with a body force and an initial flow bit-identical."""
+    states = []
+    for queried in (False, True):
+        s = fluid_setup((4, 4, 4), 0.8)
+        s.configure.append(lambda f: (f.setBodyAcceleration((0.5, -0.2, 0.1)),
+                                      f.setInitialFluidVelocity((0.01*DX/DT, 0.0, 0.0))))
+        c = build(s, backend)
+        rng = np.random.default_rng(1234)
+        for _ in range(10):
+            c.step(1)
+            if queried:
+                c.getForces()
+                c.getForces()
+                c.setVelocities(rng.normal(size=(1, 3)))
+        states.append(c.getFluidState())
+    assert np.array_equal(states[0], states[1])
