@@ -201,8 +201,11 @@ void createBatch(lbm_batch* b, int R, const lbm_params* params, const int* numPa
             throw BatchError(LBM_ERROR_UNSUPPORTED, "lbm_batch_create: replicas are stored in FP32");
         if (p.momentum_removal_frequency != 0)
             throw BatchError(LBM_ERROR_UNSUPPORTED, "lbm_batch_create: momentum removal is not supported in a batch");
-        if (p.nx <= 0 || p.ny <= 0 || p.nz <= 0 || (int64_t) p.nx*p.ny*p.nz > (1ll << 30))
+        if (p.nx <= 0 || p.ny <= 0 || p.nz <= 0)
             throw BatchError(LBM_ERROR_INVALID, "lbm_batch_create: invalid grid size");
+        if ((int64_t) NQ*p.nx*p.ny*p.nz*(int64_t) sizeof(float) > (1ll << 32))
+            throw BatchError(LBM_ERROR_INVALID, "lbm_batch_create: a replica holds at most 56.5 M nodes (its "
+                    "populations within 4 GiB)");
         if (numParticles[r] < 0)
             throw BatchError(LBM_ERROR_INVALID, "lbm_batch_create: negative particle count");
     }

@@ -24,6 +24,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <sstream>
 #include <stdexcept>
@@ -129,6 +130,7 @@ struct lbm_context {
     // Particles.
     int numParticles = 0, numSystem = 0, numSlots = 0;
     bool dense = false;                   // denseCoupling(): index order, component-plane reactions
+    bool wide = false;                    // 64-bit element indices in the fused pass (populations above 4 GiB)
     int* particles = nullptr;
     double* mass = nullptr;               // lattice units m/m_c
     std::vector<int> hostParticles;
@@ -372,8 +374,8 @@ void advanceFluid(lbm_context* c, const double* dPos, const double* dVel, const 
         HIP_CHECK(hipEventCreate(&ev.second));
         HIP_CHECK(hipEventRecord(ev.first, c->stream));
     }
-    launchFused(c->p.storage, c->mask != nullptr, particles, c->fluct(), c->popA, c->popB, c->mask, c->reactionFixed,
-            c->touched, c->ucm, a, c->stream);
+    launchFused(c->p.storage, c->wide, c->mask != nullptr, particles, c->fluct(), c->popA, c->popB, c->mask,
+            c->reactionFixed, c->touched, c->ucm, a, c->stream);
     if (c->profile) {
         HIP_CHECK(hipEventRecord(ev.second, c->stream));
         c->events.push_back(ev);
@@ -513,9 +515,12 @@ void create(const lbm_params* params, const int* solid, int numSolid, const int*
     c->g.openY = c->openAxis(1);
     c->g.openZ = c->openAxis(2);
     c->elem = storageSize(p.storage);
-    if (n*c->elem > (1ull << 32))
-        throw Error(LBM_ERROR_INVALID, "LBMForce: the lattice is too large for one GPU: a population plane would exceed "
-                "4 GiB (at most 2^30 nodes with FP32 storage, 2^29 with FP64)");
+    if (n > (1ull << 30))
+        throw Error(LBM_ERROR_INVALID, "LBMForce: the lattice is too large for one GPU: at most 2^30 nodes");
+    // The fused pass addresses populations by 32-bit byte offsets while an array of them fits in 4 GiB (up to 56.5 M
+    // nodes with FP32 storage), and by 64-bit indices beyond.  LBM_AMD_WIDE_ADDRESSING=1 takes the 64-bit path always.
+    const char* wideEnv = getenv("LBM_AMD_WIDE_ADDRESSING");
+    c->wide = NQ*n*c->elem > (1ull << 32) || (wideEnv && wideEnv[0] == '1');
     c->seed = (uint32_t) p.random_number_seed;
     if (c->seed == 0)
         c->seed = (uint32_t) std::chrono::high_resolution_clock::now().time_since_epoch().count() | 1u;

@@ -152,9 +152,8 @@ __device__ __forceinline__ double nodeMoments(const S* __restrict__ src, const G
         const double* __restrict__ ucm, int removal, double j[3]) {
     uint32_t ni, nj, nk;
     nodeCoords(g, node, ni, nj, nk);
-    Neighbours nb = neighbours(g, ni, nj, nk);
     typename StorageTraits<S>::Real f[NQ];
-    pullNode(src, g.n, node, nb, f);
+    pullNode(src, g, node, ni, nj, nk, f);
     double dr = 0, jx = 0, jy = 0, jz = 0;
     staticFor<NQ>([&](auto I) {
         constexpr int q = decltype(I)::value;

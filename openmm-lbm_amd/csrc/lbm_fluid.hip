@@ -23,14 +23,14 @@ namespace lbm {
 /* ------------------------------------------------------------------------------------------ */
 /* The fused pull-stream + regularized collision + Guo forcing (+ Philox noise) pass.          */
 /* ------------------------------------------------------------------------------------------ */
-template <typename S, bool SOLIDS, bool REACTION, bool FLUCT, int THREADS>
+template <typename S, bool SOLIDS, bool REACTION, bool FLUCT, int THREADS, bool WIDE>
 __global__ void __launch_bounds__(THREADS) fusedStep(const S* __restrict__ src, S* __restrict__ dst,
         const uint32_t* __restrict__ mask, unsigned long long* __restrict__ reactionFixed, uint32_t* __restrict__ touched,
         const double* __restrict__ ucm, StepArgs a) {
     uint32_t node = blockIdx.x*THREADS + threadIdx.x;
     if (node >= a.g.n)
         return;
-    fusedNode<S, SOLIDS, REACTION, FLUCT>(src, dst, mask, reactionFixed, touched, ucm, a, node);
+    fusedNode<S, SOLIDS, REACTION, FLUCT, WIDE>(src, dst, mask, reactionFixed, touched, ucm, a, node);
 }
 
 /*
@@ -120,9 +120,8 @@ __global__ void __launch_bounds__(BLOCK) momentReduce(const S* __restrict__ src,
                 continue;
         uint32_t i, j, k;
         nodeCoords(g, node, i, j, k);
-        Neighbours nb = neighbours(g, i, j, k);
         R f[NQ];
-        pullNode(src, g.n, node, nb, f);
+        pullNode(src, g, node, i, j, k, f);
         double dr = 0, jx = 0, jy = 0, jz = 0;
         staticFor<NQ>([&](auto I) {
             constexpr int q = decltype(I)::value;
@@ -197,13 +196,12 @@ __global__ void __launch_bounds__(BLOCK) bounceBack(S* __restrict__ dst, const i
         int links = wallLinks[b];
         uint32_t i, j, k;
         nodeCoords(g, node, i, j, k);
-        Neighbours nb = neighbours(g, i, j, k);
+        const NodeSlots at = nodeSlots(g, node, i, j, k);
         staticFor<NQ - 1>([&](auto I) {
             constexpr int q = decltype(I)::value + 1;
             if (links & (1 << q)) {
-                uint32_t solid = (uint32_t) ((int) node + pushOffset<q>(nb));
-                S v = dst[(size_t) q*g.n + node];
-                dst[(size_t) OPP[q]*g.n + solid] = v;
+                S v = dst[at.own(q)];
+                dst[at.template pushed<q, OPP[q]>()] = v;      // population opp(q) of the solid node x + c_q
                 double dv = (double) loadPop(&v);
                 acc[0] = addTerm<CX[q]>(acc[0], 2.0*dv);
                 acc[1] = addTerm<CY[q]>(acc[1], 2.0*dv);
@@ -265,15 +263,14 @@ __global__ void __launch_bounds__(BLOCK) boundaryKernel(const S* __restrict__ sr
         int unknown = bn.unknown, solid = bn.solid, kind = bn.kindFace & 7, face = (bn.kindFace >> 3) - 1;
         uint32_t i, j, k;
         nodeCoords(a.g, node, i, j, k);
-        Neighbours nb = neighbours(a.g, i, j, k);
+        const NodeSlots at = nodeSlots(a.g, node, i, j, k);
         // Pre-collision moments of the step (as the fused pass computed them).
         double f[NQ], post[NQ], own[NQ];
         staticFor<NQ>([&](auto I) {
             constexpr int q = decltype(I)::value;
-            uint32_t from = (uint32_t) ((int) node + pullOffset<q>(nb));
-            f[q] = (double) loadPop(src + (size_t) q*a.g.n + from);
-            post[q] = (double) loadPop(dst + (size_t) q*a.g.n + from);
-            own[q] = (double) loadPop(dst + (size_t) q*a.g.n + node);
+            f[q] = (double) loadPop(src + at.template pulled<q>());
+            post[q] = (double) loadPop(dst + at.template pulled<q>());
+            own[q] = (double) loadPop(dst + at.own(q));
         });
         Moments<double> m = nodeMoments(f);
         double pi[6];
@@ -391,9 +388,8 @@ __global__ void __launch_bounds__(BLOCK) boundaryKernel(const S* __restrict__ sr
                     acc[1] = addTerm<-CY[q]>(acc[1], t);
                     acc[2] = addTerm<-CZ[q]>(acc[2], t);
                 }
-                uint32_t to = (uint32_t) ((int) node + pullOffset<q>(nb));
                 uint32_t r = (rnd ^ (uint32_t) (q*0x9E3779B9u))*0x85EBCA6Bu;
-                storePop(dst + (size_t) q*a.g.n + to, (typename StorageTraits<S>::Real) value, r ^ (r >> 13));
+                storePop(dst + at.template pulled<q>(), (typename StorageTraits<S>::Real) value, r ^ (r >> 13));
             }
         });
     }
@@ -414,10 +410,10 @@ __global__ void __launch_bounds__(BLOCK) gatherState(const S* __restrict__ src, 
         return;
     uint32_t i, j, k;
     nodeCoords(g, node, i, j, k);
-    Neighbours nb = neighbours(g, i, j, k);
+    const NodeSlots at = nodeSlots(g, node, i, j, k);
     staticFor<NQ>([&](auto I) {
         constexpr int q = decltype(I)::value;
-        out[(size_t) q*g.n + node] = (double) loadPop(src + (size_t) q*g.n + (uint32_t) ((int) node + pullOffset<q>(nb)));
+        out[(size_t) q*g.n + node] = (double) loadPop(src + at.template pulled<q>());
     });
 }
 
@@ -438,10 +434,10 @@ __global__ void __launch_bounds__(BLOCK) scatterState(S* __restrict__ dst, const
         return;
     uint32_t i, j, k;
     nodeCoords(g, node, i, j, k);
-    Neighbours nb = neighbours(g, i, j, k);
+    const NodeSlots at = nodeSlots(g, node, i, j, k);
     staticFor<NQ>([&](auto I) {
         constexpr int q = decltype(I)::value;
-        storeNearest(dst + (size_t) q*g.n + (uint32_t) ((int) node + pullOffset<q>(nb)), in[(size_t) q*g.n + node]);
+        storeNearest(dst + at.template pulled<q>(), in[(size_t) q*g.n + node]);
     });
 }
 
@@ -461,11 +457,11 @@ __global__ void __launch_bounds__(BLOCK) fieldsKernel(const S* __restrict__ src,
     }
     uint32_t i, j, k;
     nodeCoords(g, node, i, j, k);
-    Neighbours nb = neighbours(g, i, j, k);
+    const NodeSlots at = nodeSlots(g, node, i, j, k);
     double r = 1.0, jx = 0, jy = 0, jz = 0;
     staticFor<NQ>([&](auto I) {
         constexpr int q = decltype(I)::value;
-        double v = (double) loadPop(src + (size_t) q*g.n + (uint32_t) ((int) node + pullOffset<q>(nb)));
+        double v = (double) loadPop(src + at.template pulled<q>());
         r += v;
         jx = addTerm<CX[q]>(jx, v);
         jy = addTerm<CY[q]>(jy, v);
@@ -490,12 +486,12 @@ __global__ void __launch_bounds__(BLOCK) initEquilibrium(S* __restrict__ dst, co
     bool solid = mask && isSolid(mask, node);
     uint32_t i, j, k;
     nodeCoords(g, node, i, j, k);
-    Neighbours nb = neighbours(g, i, j, k);
+    const NodeSlots at = nodeSlots(g, node, i, j, k);
     double eq[NQ];
     equilibriumDeviationD(0.0, ux, uy, uz, eq);
     staticFor<NQ>([&](auto I) {
         constexpr int q = decltype(I)::value;
-        storeNearest(dst + (size_t) q*g.n + (uint32_t) ((int) node + pullOffset<q>(nb)), solid ? -WQ[q] : eq[q]);
+        storeNearest(dst + at.template pulled<q>(), solid ? -WQ[q] : eq[q]);
     });
 }
 
@@ -521,28 +517,40 @@ static inline unsigned gridFor(uint64_t n) {
     default: { using S = float; __VA_ARGS__; break; } \
     }
 
-template <typename S, bool SOL, bool REA>
+template <typename S, bool SOL, bool REA, bool WIDE>
 static void launchFusedT(bool fluct, const void* src, void* dst, const uint32_t* mask, unsigned long long* reactionFixed,
         uint32_t* touched, const double* ucm, const StepArgs& a, hipStream_t stream) {
     constexpr int T = FUSED_THREADS;
     unsigned grid = (unsigned) ((a.g.n + T - 1)/T);
     if (fluct)
-        fusedStep<S, SOL, REA, true, T><<<grid, T, 0, stream>>>((const S*) src, (S*) dst, mask, reactionFixed, touched, ucm, a);
+        fusedStep<S, SOL, REA, true, T, WIDE><<<grid, T, 0, stream>>>((const S*) src, (S*) dst, mask, reactionFixed,
+                touched, ucm, a);
     else
-        fusedStep<S, SOL, REA, false, T><<<grid, T, 0, stream>>>((const S*) src, (S*) dst, mask, reactionFixed, touched, ucm, a);
+        fusedStep<S, SOL, REA, false, T, WIDE><<<grid, T, 0, stream>>>((const S*) src, (S*) dst, mask, reactionFixed,
+                touched, ucm, a);
 }
 
-void launchFused(int storage, bool solids, bool reaction, bool fluct, const void* src, void* dst, const uint32_t* mask,
+template <typename S, bool WIDE>
+static void launchFusedW(bool solids, bool reaction, bool fluct, const void* src, void* dst, const uint32_t* mask,
         unsigned long long* reactionFixed, uint32_t* touched, const double* ucm, const StepArgs& a, hipStream_t stream) {
+    if (solids) {
+        if (reaction) launchFusedT<S, true, true, WIDE>(fluct, src, dst, mask, reactionFixed, touched, ucm, a, stream);
+        else launchFusedT<S, true, false, WIDE>(fluct, src, dst, mask, reactionFixed, touched, ucm, a, stream);
+    }
+    else {
+        if (reaction) launchFusedT<S, false, true, WIDE>(fluct, src, dst, mask, reactionFixed, touched, ucm, a, stream);
+        else launchFusedT<S, false, false, WIDE>(fluct, src, dst, mask, reactionFixed, touched, ucm, a, stream);
+    }
+}
+
+void launchFused(int storage, bool wide, bool solids, bool reaction, bool fluct, const void* src, void* dst,
+        const uint32_t* mask, unsigned long long* reactionFixed, uint32_t* touched, const double* ucm, const StepArgs& a,
+        hipStream_t stream) {
     LBM_DISPATCH_STORAGE(storage, S, {
-        if (solids) {
-            if (reaction) launchFusedT<S, true, true>(fluct, src, dst, mask, reactionFixed, touched, ucm, a, stream);
-            else launchFusedT<S, true, false>(fluct, src, dst, mask, reactionFixed, touched, ucm, a, stream);
-        }
-        else {
-            if (reaction) launchFusedT<S, false, true>(fluct, src, dst, mask, reactionFixed, touched, ucm, a, stream);
-            else launchFusedT<S, false, false>(fluct, src, dst, mask, reactionFixed, touched, ucm, a, stream);
-        }
+        if (wide)
+            launchFusedW<S, true>(solids, reaction, fluct, src, dst, mask, reactionFixed, touched, ucm, a, stream);
+        else
+            launchFusedW<S, false>(solids, reaction, fluct, src, dst, mask, reactionFixed, touched, ucm, a, stream);
     })
 }
 

@@ -15,7 +15,7 @@ namespace lbm {
  * particle reaction (consumed from the fixed-point accumulators where the touched bit is set), collide, add the
  * fluctuating part, store C_q(x) to dst at x.  Returns without writing for solid nodes.
  */
-template <typename S, bool SOLIDS, bool REACTION, bool FLUCT>
+template <typename S, bool SOLIDS, bool REACTION, bool FLUCT, bool WIDE = false>
 __device__ __forceinline__ void fusedNode(const S* __restrict__ src, S* __restrict__ dst, const uint32_t* __restrict__ mask,
         unsigned long long* __restrict__ reactionFixed, uint32_t* __restrict__ touched, const double* __restrict__ ucm,
         const StepArgs& a, uint32_t node) {
@@ -34,9 +34,10 @@ __device__ __forceinline__ void fusedNode(const S* __restrict__ src, S* __restri
             return;
     uint32_t i, j, k;
     nodeCoords(a.g, node, i, j, k);
-    Neighbours nb = neighbours(a.g, i, j, k);
+    using Offset = typename std::conditional<WIDE, long long, int>::type;
+    const ElementIndex<WIDE> e = elementOf<ElementIndex<WIDE>>(node, i);
     R fr[NQ];
-    pullNodeStream(src, a.g.n, node, nb, fr);
+    pullNodeStream<S, WIDE>(src, a.g.nx, e, elementNeighbours<Offset>(a.g, i, j, k), fr);
     C f[NQ];
 #pragma unroll
     for (int q = 0; q < NQ; q++)
@@ -97,7 +98,6 @@ __device__ __forceinline__ void fusedNode(const S* __restrict__ src, S* __restri
     uint32_t rnd = 0;
     if constexpr (isReduced<S>())
         rnd = mixBits(node ^ a.roundSeed);
-    const uint32_t at = node*(uint32_t) sizeof(S);
 #pragma unroll
     for (int q = 0; q < NQ; q++) {
         uint32_t r = 0;
@@ -108,7 +108,10 @@ __device__ __forceinline__ void fusedNode(const S* __restrict__ src, S* __restri
         R v = (R) out[q];
         if constexpr (FLUCT)
             v += noise[q];
-        storePopStream(atByte(dst + (size_t) q*a.g.n, at), v, r);
+        if constexpr (WIDE)
+            storePopStream(dst + (uint64_t) q*a.g.nx + e, v, r);
+        else
+            storePopStream(atByte(dst + (size_t) q*a.g.nx, e*(uint32_t) sizeof(S)), v, r);
     }
 }
 

@@ -116,8 +116,9 @@ void launchCoupleBatch(int stencil, const ReplicaDesc* table, const uint16_t* pa
 int coupleLanes(int stencil);
 void launchVerletBatch(const ReplicaDesc* table, const uint16_t* particleReplica, int numParticles, double* positions,
         double* velocities, const double* forces, const double* invMass, int64_t* stepCounter, hipStream_t stream);
-void launchFused(int storage, bool solids, bool reaction, bool fluct, const void* src, void* dst, const uint32_t* mask,
-        unsigned long long* reactionFixed, uint32_t* touched, const double* ucm, const StepArgs& a, hipStream_t stream);
+void launchFused(int storage, bool wide, bool solids, bool reaction, bool fluct, const void* src, void* dst,
+        const uint32_t* mask, unsigned long long* reactionFixed, uint32_t* touched, const double* ucm, const StepArgs& a,
+        hipStream_t stream);
 void launchMomentReduce(int storage, bool solids, const void* src, const uint32_t* mask, const Geometry& g,
         double* partials, hipStream_t stream);
 void launchFinishReduce(const double* partials, double numFluid, double* totals, double* ucm, hipStream_t stream);

@@ -291,3 +291,36 @@ def test_fluid_totals_fp64():
     mass_ref, p_ref = total_moments(gpu.getFluidState())
     assert abs(m1 - mass_ref) < 1e-9*mass_ref
     assert abs(m1 - m0) < 1e-6*m0
+
+
+@pytest.mark.parametrize("storage", [LBMForce.FP32, LBMForce.FP16, LBMForce.FP64], ids=["fp32", "fp16", "fp64"])
+def test_wide_addressing_is_bit_identical(storage, monkeypatch):
+    """The fused pass addresses the populations by 32-bit byte offsets while an array of them fits in 4 GiB and by
+    64-bit indices beyond (LBM_AMD_WIDE_ADDRESSING=1 forces the 64-bit path): both paths give the same run bit for bit,
+    with regularized walls, open faces, fluctuations and coupled particles."""
+    nx, ny, nz = 8, 6, 7
+    s = Setup((nx, ny, nz), dx=2.0, dt=0.01, tau=0.8)
+    rng = np.random.default_rng(5)
+    s.add_particles(rng.uniform([0, 2, 2], [16, 10, 12], size=(12, 3)), mass=200.0)
+
+    def conf(f):
+        f.setSolidNodes([i + nx*ny*k for k in range(nz) for i in range(nx)])      # the plane j = 0
+        f.setWallScheme(LBMForce.Regularized)
+        f.setFaceBoundary(LBMForce.ZMin, LBMForce.Velocity)
+        f.setFaceBoundary(LBMForce.ZMax, LBMForce.DensityVelocity)
+        f.setFaceVelocity(LBMForce.ZMin, (0.0, 0.0, 0.5))
+        f.setFluidFluctuations(True)
+        f.setTemperature(300.0)
+        f.setRandomNumberSeed(17)
+        f.setFriction(5.0)
+        f.setInterpolationStencil(LBMForce.Keys)
+    s.configure.append(conf)
+    runs = []
+    for wide in ("0", "1"):
+        monkeypatch.setenv("LBM_AMD_WIDE_ADDRESSING", wide)
+        c = s.build(None, storage=storage)
+        c.step(12)
+        runs.append((c.getFluidState(), c.positions.copy(), c.getWallForce()))
+    assert np.array_equal(runs[0][0], runs[1][0])
+    assert np.array_equal(runs[0][1], runs[1][1])
+    assert np.array_equal(runs[0][2], runs[1][2])
