@@ -511,11 +511,8 @@ struct Geometry {
  * Population layout.  Each row (j, k) of the lattice holds the 19 populations of its nx nodes one after the other:
  * population q of node (i, j, k) is the element ((k*ny + j)*19 + q)*nx + i = q*nx + 19*node - 18*i.  A work-group
  * of consecutive nodes then reads and writes a few contiguous row blocks of 19*nx values instead of 19 planes
- * N values apart, which the HBM serves faster (on 256^3 a plain pull runs at 73% of the peak instead of 69%).
+ * N values apart, which the HBM serves faster (on 256^3 a bare pull measured 73% of the peak instead of 69%).
  */
-__host__ __device__ inline uint64_t popCount(uint64_t n) {
-    return (uint64_t) NQ*n;
-}
 
 /* Element of population 0 of a node (add q*nx for population q). */
 template <typename I>

@@ -1,9 +1,9 @@
 /*
  * Fluid kernels of the gfx950 LBMForce platform.
  *
- * Layout in HBM: two population arrays A and B of 19*N values (N = nx*ny*nz), structure of
- * arrays [q][z][y][x] with x contiguous, holding the POST-collision deviations C_q - w_q of every
- * node at the node itself.  One lattice step is one pass of fusedStep over all nodes: pull the 19
+ * Layout in HBM: two population arrays A and B of 19*N values (N = nx*ny*nz), [z][y][q][x]: each
+ * lattice row holds its 19 populations one after the other, x contiguous (lbm_device.h), holding
+ * the POST-collision deviations C_q - w_q of every node at the node itself.  One lattice step is one pass of fusedStep over all nodes: pull the 19
  * values C_q(x - c_q) from A (the pre-collision populations of x), form the moments, collide with
  * Guo forcing (and the fluctuating part), write the 19 results to B at x.  Every population is
  * read once and written once: 2*19*sizeof(storage) bytes per node per step.
