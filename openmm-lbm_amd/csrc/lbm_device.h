@@ -598,13 +598,13 @@ __device__ __forceinline__ NodeSlots nodeSlots(const Geometry& g, uint32_t node,
     return NodeSlots{elementOf<uint64_t>(node, i), elementNeighbours<long long>(g, i, j, k), g.nx};
 }
 
-/* The element at a 32-bit byte offset from a population plane: the plane's base stays in scalar registers and the
-   access takes the offset from one vector register.  Used where the whole array is at most 4 GiB (checked at context
+/* The element at a 32-bit byte offset from a base (the array plus q*nx for population q): the base stays in scalar
+   registers and the access takes the offset from one vector register.  Used where the whole array is at most 4 GiB (checked at context
    creation; larger lattices take the 64-bit path). */
 template <typename T>
-__device__ __forceinline__ T* atByte(T* plane, uint32_t offset) {
+__device__ __forceinline__ T* atByte(T* base, uint32_t offset) {
     using Byte = typename std::conditional<std::is_const<T>::value, const char, char>::type;
-    return reinterpret_cast<T*>(reinterpret_cast<Byte*>(plane) + offset);
+    return reinterpret_cast<T*>(reinterpret_cast<Byte*>(base) + offset);
 }
 
 /* Pull the 19 pre-collision deviations of a node: f_q(x) = post-collision f_q stored at x - c_q (64-bit indices). */

@@ -324,3 +324,31 @@ def test_wide_addressing_is_bit_identical(storage, monkeypatch):
     assert np.array_equal(runs[0][0], runs[1][0])
     assert np.array_equal(runs[0][1], runs[1][1])
     assert np.array_equal(runs[0][2], runs[1][2])
+
+
+def test_lattice_beyond_4_gib():
+    """A 400^3 lattice in FP32 (4.9 GB per population array, so the fused pass takes 64-bit indices): beads near the
+    top of the array (element offsets above 2^32 bytes) exchange momentum with the fluid, and the total momentum of
+    beads and fluid is conserved over 20 steps, to FP32 storage precision."""
+    n = 400
+    s = Setup((n, n, n), dx=1.0, dt=0.01, tau=0.8)
+    rng = np.random.default_rng(2)
+    pos = np.column_stack([rng.uniform(0, n, 8), rng.uniform(0, n, 8), np.linspace(300, 399, 8)])
+    vel = rng.normal(scale=1.0, size=(8, 3))
+    s.add_particles(pos, mass=100.0, velocities=vel)
+    s.configure.append(lambda f: (f.setFriction(10.0), f.setTemperature(0.0),
+                                  f.setInterpolationStencil(LBMForce.Keys)))
+    c = s.build()
+    cell = 602.214
+
+    def momentum():
+        mass, j = c.getFluidTotals()
+        return (100.0*c.velocities).sum(axis=0) + j*cell*1.0/0.01
+    p0 = momentum()
+    c.step(20)
+    p1 = momentum()
+    scale = (100.0*np.linalg.norm(vel, axis=1)).sum()
+    moved = (100.0*np.linalg.norm(c.velocities - vel, axis=1)).sum()
+    print(f"relative momentum change {np.linalg.norm(p1 - p0)/scale:.2e}, momentum given to the fluid {moved/scale:.2f}")
+    assert moved > 0.5*scale
+    assert np.linalg.norm(p1 - p0)/scale < 1e-5
