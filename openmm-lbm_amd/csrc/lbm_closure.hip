@@ -187,7 +187,7 @@ __global__ void __launch_bounds__(BLOCK) closureCouple(const double* __restrict_
         sl.inStencil = true;
         sl.solid = false;
         double wallAcc[3] = {0, 0, 0};
-        spreadReaction(sl, f, a, reactionFixed, touched, wallAcc);
+        markTouched<1>(touched, sl.node, spreadReaction(sl, f, a, reactionFixed, wallAcc), 0);
     }
 }
 

@@ -265,11 +265,11 @@ __device__ __forceinline__ void dragForce(const double* vel, const double u[3], 
     }
 }
 
-/* The reaction -w F of one slot: on a fluid node into its fixed-point accumulators (flagged in the touched mask, which
-   the fused pass reads and clears); on a solid node it leaves the fluid and goes to the wall
-   (ReferenceLBMKernels.cpp:959-971). */
-__device__ __forceinline__ void spreadReaction(const StencilSlot& sl, const double f[3], const ParticleArgs& a,
-        unsigned long long* __restrict__ reactionFixed, uint32_t* __restrict__ touched, double wallAcc[3]) {
+/* The reaction -w F of one slot: on a fluid node into its fixed-point accumulators; on a solid node it leaves the
+   fluid and goes to the wall (ReferenceLBMKernels.cpp:959-971).  Returns whether an accumulator received a nonzero
+   value (the node is then flagged in the touched mask, which the fused pass reads and clears). */
+__device__ __forceinline__ bool spreadReaction(const StencilSlot& sl, const double f[3], const ParticleArgs& a,
+        unsigned long long* __restrict__ reactionFixed, double wallAcc[3]) {
     bool any = false;
     for (int k = 0; k < 3; k++) {
         double r = -sl.weight*f[k];
@@ -285,9 +285,36 @@ __device__ __forceinline__ void spreadReaction(const StencilSlot& sl, const doub
             }
         }
     }
-    if (any)
-        __hip_atomic_fetch_or(asGlobal(touched + (sl.node >> 5)), 1u << (sl.node & 31), __ATOMIC_RELAXED,
-                __HIP_MEMORY_SCOPE_AGENT);
+    return any;
+}
+
+/*
+ * Flags the nodes of the slots with `flagged` in the touched mask (bit node & 31 of word node >> 5).  The W slots of
+ * a stencil row are consecutive lanes on x-consecutive nodes, which mostly share a mask word: the first lane of each
+ * run of lanes on one word ORs in the bits of the row's lanes on that word, one atomic instead of up to W.  A bit may
+ * be set twice where a row's words are not in runs (a lattice narrower than the stencil); OR makes that harmless.
+ * Every lane of the wavefront calls it.
+ */
+template <int W>
+__device__ __forceinline__ void markTouched(uint32_t* __restrict__ touched, uint32_t node, bool flagged, int lane) {
+    const uint32_t word = node >> 5;
+    const uint32_t bit = flagged ? 1u << (node & 31) : 0u;
+    uint32_t bits = bit;
+    bool first = true;
+    if constexpr (W > 1) {
+        const int x = lane % W;                     // the slot's place in its stencil row
+#pragma unroll
+        for (int d = 1; d < W; d++) {
+            uint32_t w2 = __shfl_down(word, d, 64);
+            uint32_t b2 = __shfl_down(bit, d, 64);
+            if (x + d < W && w2 == word)
+                bits |= b2;
+        }
+        uint32_t before = __shfl_up(word, 1, 64);
+        first = x == 0 || before != word;
+    }
+    if (first && bits != 0u)
+        __hip_atomic_fetch_or(asGlobal(touched + word), bits, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 /* The wall momentum acc of the work-group's threads, summed into partials[3*blockIdx.x + k] in a fixed order (the
@@ -316,6 +343,7 @@ __device__ __forceinline__ void storeWallPartial(const double acc[3], double* __
 
 /* The second half of coupleGroup, after the interpolation: the drag force (written by the group's first lane) and,
    in a lattice step, the spread of the reaction. */
+template <int LANES>
 __device__ __forceinline__ void finishGroup(const StencilSlot& sl, const double u[3], const double* vel, uint32_t index,
         double m, bool active, int lane, const ParticleArgs& a, double* forceOut,
         unsigned long long* __restrict__ reactionFixed, uint32_t* __restrict__ touched, double wallAcc[3],
@@ -333,8 +361,13 @@ __device__ __forceinline__ void finishGroup(const StencilSlot& sl, const double 
                     atomicAdd(systemForce + k, f[k]*a.forceScale);
             }
     }
-    if (a.isStep && sl.inStencil)
-        spreadReaction(sl, f, a, reactionFixed, touched, wallAcc);
+    if (a.isStep) {
+        bool flagged = sl.inStencil && spreadReaction(sl, f, a, reactionFixed, wallAcc);
+        constexpr int STENCIL = groupStencil<LANES>();
+        constexpr int W = STENCIL == LBM_STENCIL_KEYS ? 4 : STENCIL == LBM_STENCIL_THREE_POINT ? 3 :
+                          STENCIL == LBM_STENCIL_TRILINEAR ? 2 : 1;
+        markTouched<W>(touched, sl.node, flagged, lane);
+    }
 }
 
 /*
@@ -353,7 +386,7 @@ __device__ __forceinline__ void coupleGroup(const S* __restrict__ src, const uin
         const double* noise = nullptr) {
     double u[3];
     StencilSlot sl = interpolateGroup<S, LANES>(src, mask, x, active, lane, ucm, a, u);
-    finishGroup(sl, u, vel, index, m, active, lane, a, forceOut, reactionFixed, touched, wallAcc, noise);
+    finishGroup<LANES>(sl, u, vel, index, m, active, lane, a, forceOut, reactionFixed, touched, wallAcc, noise);
 }
 
 }  // namespace lbm

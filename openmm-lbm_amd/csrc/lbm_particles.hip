@@ -110,7 +110,7 @@ __global__ void __launch_bounds__(BLOCK) coupleParticles(const S* __restrict__ s
         myNoise = noise[threadIdx.x/LANES];
     }
     double wallAcc[3] = {0, 0, 0};
-    finishGroup(sl, u, velocities + 3*(size_t) part, (uint32_t) i, m, active, lane, a,
+    finishGroup<LANES>(sl, u, velocities + 3*(size_t) part, (uint32_t) i, m, active, lane, a,
             particleForce + 3*(size_t) (active ? i : 0), reactionFixed, touched, wallAcc, myNoise,
             systemForces != nullptr ? systemForces + 3*(size_t) part : nullptr);
     if (partials != nullptr)
@@ -153,7 +153,7 @@ __global__ void __launch_bounds__(BLOCK) coupleBatch(const ReplicaDesc* __restri
         myNoise = noise[threadIdx.x/LANES];
     }
     double wallAcc[3] = {0, 0, 0};
-    finishGroup(sl, u, velocities + 3*(size_t) pp, (uint32_t) (pp - d->particleBegin), active ? mass[pp] : 0.0, active,
+    finishGroup<LANES>(sl, u, velocities + 3*(size_t) pp, (uint32_t) (pp - d->particleBegin), active ? mass[pp] : 0.0, active,
             lane, a, forces + 3*(size_t) pp, d->reactionFixed, d->touched, wallAcc, myNoise);
 }
 
