@@ -22,11 +22,15 @@
  * reference's SFMT stream are therefore not reproduced (parity unpinned for that stream;
  * the reference's own GPU platforms differ from it too, TestLBMFluctuations.h:8-15).
  *
- * Pinned by: tests/test_oracle_*.py (closed forms of the reference's tests: uniform flow,
- * conservation, body force, removal timing, shear-wave viscosity, exact bounce-back and
- * regularized Poiseuille, wall balance, Couette, first-step drag, stencil moments and
- * linear-field exactness, Philox known-answer vectors) and, for the D3Q19 algebra,
- * against the reference's own D3Q19.h compiled into oracle/_ref (tests/test_oracle_ref.py).
+ * Pinned by: tests/test_closed_forms.py (the closed forms of the reference's fluid, wall,
+ * face, coupling, stencil, checkpoint and parameter tests: uniform flow, conservation, body
+ * force, removal timing, shear-wave viscosity, exact bounce-back and regularized Poiseuille,
+ * wall balance, Couette, reservoir, inlet, duct, first-step drag, momentum balances, stencil
+ * exactness, restarts), tests/test_centered.py (the centred drag against its closed forms and a
+ * Gaussian-elimination solution), tests/test_fluctuations.py (mode variances, spectra,
+ * equipartition), tests/test_oracle.py (Philox known-answer vectors, algebra identities, stencil
+ * moments) and, for the D3Q19 algebra, the reference's own D3Q19.h compiled into oracle/_ref
+ * (tests/test_oracle.py::test_algebra_matches_reference_header).
  */
 #include "../include/lbm_amd.h"
 #include <algorithm>

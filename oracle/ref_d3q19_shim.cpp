@@ -1,8 +1,9 @@
 /*
  * TEST INFRASTRUCTURE: a C entry-point harness around the reference's own D3Q19 algebra
  * (openmmapi/include/internal/D3Q19.h, which depends only on <cmath>), compiled from the
- * reference tree by oracle/Makefile into oracle/_ref/libref_d3q19.so.  tests/test_oracle_ref.py
- * compares the oracle's restatement of each function with it on random node states.
+ * reference tree by oracle/Makefile into oracle/_ref/libref_d3q19.so.  tests/test_oracle.py
+ * (test_algebra_matches_reference_header) compares the oracle's restatement of each function with
+ * it on random node states, and tests/golden/make_d3q19_vectors.py writes fixtures from it.
  * Nothing else loads it; it is not built on the GPU box.
  */
 #include "internal/D3Q19.h"

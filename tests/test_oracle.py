@@ -111,6 +111,35 @@ def test_algebra_matches_reference_header(lib):
         assert ref.ref_boundary_stress_weight(omega) == pytest.approx((1 - omega)**2/(omega*(2 - omega)))
 
 
+def test_algebra_matches_reference_vectors(lib):
+    """The oracle's D3Q19 algebra against tests/golden/d3q19_vectors.npz, input/output vectors of the reference's own
+    D3Q19.h (equilibriumDeviation, nonEquilibriumMomentFromDeviation, regularizedNonEquilibrium, guoForcing, the
+    collision and equilibrium noise, the mode basis) written by tests/golden/make_d3q19_vectors.py: the same check as
+    above where the reference tree is absent."""
+    import os
+    g = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "d3q19_vectors.npz"))
+    a, p = np.zeros(19), np.zeros(6)
+    for n in range(len(g["dr"])):
+        dr, u, f = float(g["dr"][n]), g["u"][n], g["force"][n]
+        lib.oracle_equilibrium_deviation(dr, *u, dptr(a))
+        assert np.allclose(a, g["equilibrium"][n], rtol=0, atol=1e-16)
+        df = np.ascontiguousarray(g["df"][n])
+        lib.oracle_pi_neq(dptr(df), dr, *(u*(1 + dr)), dptr(p))
+        assert np.allclose(p, g["pi_neq"][n], rtol=0, atol=1e-16)
+        pp = np.ascontiguousarray(g["pi_neq"][n])
+        lib.oracle_regularized_neq(dptr(pp), dptr(a))
+        assert np.allclose(a, g["regularized_neq"][n], rtol=0, atol=1e-17)
+        lib.oracle_guo(*u, *f, dptr(a))
+        assert np.allclose(a, g["guo"][n], rtol=0, atol=1e-18)
+        r = np.ascontiguousarray(g["normals"][n])
+        lib.oracle_collision_noise(dptr(r), 0.3, 0.7, dptr(a))
+        assert np.allclose(a, g["collision_noise"][n], rtol=0, atol=1e-15)
+        lib.oracle_equilibrium_noise(dptr(r), 0.2, 0.3, 0.7, dptr(a))
+        assert np.allclose(a, g["equilibrium_noise"][n], rtol=0, atol=1e-15)
+    modes = np.array([[lib.oracle_mode(k, q) for q in range(19)] for k in range(19)])
+    assert np.array_equal(modes, g["modes"])
+
+
 def test_algebra_identities(lib):
     """sum dfeq = drho, sum c dfeq = rho u; sum S = 0, sum c S = F (D3Q19.h:123-126); Pi_neq of the equilibrium is 0;
     the regularized part carries no mass or momentum and reproduces Pi (H2 projection)."""
