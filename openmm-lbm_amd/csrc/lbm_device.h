@@ -636,12 +636,14 @@ __device__ __forceinline__ void pullNodeStream(const S* __restrict__ src, uint32
         });
     }
     else {
-        constexpr int B = (int) sizeof(S);
-        const Neighbours nbBytes = {nb.xm*B, nb.xp*B, nb.ym*B, nb.yp*B, nb.zm*B, nb.zp*B};
-        const uint32_t at = e*(uint32_t) B;
+        // Byte offsets modulo 2^32: the sums wrap to the right offset, which lies within the 4 GiB array.
+        constexpr uint32_t B = (uint32_t) sizeof(S);
+        const NeighboursOf<uint32_t> nbBytes = {(uint32_t) nb.xm*B, (uint32_t) nb.xp*B, (uint32_t) nb.ym*B,
+                                                (uint32_t) nb.yp*B, (uint32_t) nb.zm*B, (uint32_t) nb.zp*B};
+        const uint32_t at = e*B;
         staticFor<NQ>([&](auto I) {
             constexpr int q = decltype(I)::value;
-            f[q] = loadPopStream(atByte(src + (size_t) q*nx, at + (uint32_t) pullOffset<q>(nbBytes)));
+            f[q] = loadPopStream(atByte(src + (size_t) q*nx, at + pullOffset<q>(nbBytes)));
         });
     }
 }
