@@ -520,30 +520,18 @@ __device__ __forceinline__ I elementOf(uint32_t node, uint32_t i) {
     return (I) NQ*(I) node - (I) (NQ - 1)*(I) i;
 }
 
-/* The offsets that take a node to its neighbours one step along -c (the pull source) on a periodic lattice, in nodes
-   (neighbours) or in population elements (elementNeighbours: rows are 19*nx elements apart). */
+/* The offsets, in population elements (rows are 19*nx elements apart), that take a node to its neighbours one step
+   along -c (the pull source) on a periodic lattice. */
 template <typename I>
 struct NeighboursOf {
     I xm, xp, ym, yp, zm, zp;   // offsets to (i-1), (i+1), (j-1), (j+1), (k-1), (k+1), wrapped
 };
-using Neighbours = NeighboursOf<int>;
 
 __device__ __forceinline__ void nodeCoords(const Geometry& g, uint32_t node, uint32_t& i, uint32_t& j, uint32_t& k) {
     k = fastDiv(node, g.divxy);
     uint32_t rem = node - k*g.nxy;
     j = fastDiv(rem, g.divx);
     i = rem - j*g.nx;
-}
-
-__device__ __forceinline__ Neighbours neighbours(const Geometry& g, uint32_t i, uint32_t j, uint32_t k) {
-    Neighbours nb;
-    nb.xm = i > 0 ? -1 : (int) g.nx - 1;
-    nb.xp = i + 1 < g.nx ? 1 : 1 - (int) g.nx;
-    nb.ym = j > 0 ? -(int) g.nx : (int) (g.nxy - g.nx);
-    nb.yp = j + 1 < g.ny ? (int) g.nx : (int) g.nx - (int) g.nxy;
-    nb.zm = k > 0 ? -(int) g.nxy : (int) (g.n - g.nxy);
-    nb.zp = k + 1 < g.nz ? (int) g.nxy : (int) g.nxy - (int) g.n;
-    return nb;
 }
 
 template <typename I>
