@@ -23,6 +23,8 @@ typedef struct lbm_batch lbm_batch;
  * params[r]: the lattice parameters of replica r (lbm_compute_lattice_parameters), all with the same
  * interpolation stencil and the same fluid_fluctuations setting; num_particles[r]: its coupled particles;
  * masses (Da), positions (nm), velocities (nm/ps): all particles, replica after replica (xyz per particle).
+ * At most 65,535 replicas of at most 56.5 M nodes each (LBM_ERROR_INVALID beyond); periodic faces, no solid
+ * nodes, the explicit drag, FP32 storage and no momentum removal (LBM_ERROR_UNSUPPORTED otherwise).
  */
 int lbm_batch_create(int num_replicas, const lbm_params* params, const int* num_particles, const double* masses,
         const double* positions, const double* velocities, lbm_batch** out);
