@@ -1,6 +1,7 @@
 """Benchmark of the gfx950 LBMForce platform on its headline workload.
 
-python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload config2|config2-fluid|periodic|config3|config4|config5]
+python bench.py [--gpus 1] [--steps K] [--warmup W]
+                [--workload config2|config2-fluid|periodic|config3|config4|config5|centred]
                 [--storage fp32|fp16|bf16] [--no-cpu-baseline]
 
 A step is one coupled integration step: beginStep (reflection), the extra forces, the kernel's lattice step
@@ -216,6 +217,48 @@ def batch_study(steps, warmup, sequential_steps=20):
     }
 
 
+def centred_study(steps, warmup):
+    """The centred drag (LBMForce::Centered) on config2 (256^3, walls, 20,000 beads, fluctuating fluid), against the
+    explicit drag of the same run: coupled steps with the nearest-node and the Keys stencil.  The centred drag solves
+    for the forces of the step implicitly (closed form per node at the nearest node; conjugate gradients over
+    overlapping stencils), so its coupling costs more than the explicit drag's."""
+    from lbm_amd import LBMForce, _lib, workloads
+    out = {}
+    for name, stencil in (("nearest", LBMForce.NearestNode), ("keys", LBMForce.Keys)):
+        for drag_name, drag in (("explicit", LBMForce.Explicit), ("centred", LBMForce.Centered)):
+            w = workloads.config2(stencil=stencil)
+            w.configure.append(lambda f, drag=drag: f.setDragScheme(drag))
+            print(f"bench: config2 {name} stencil, {drag_name} drag", file=sys.stderr, flush=True)
+            ctx = w.build()
+            dp = w.device_particles(ctx)
+            lib = ctx.lib
+            dp.step(warmup)
+            _lib.check(lib, lib.lbm_synchronize(ctx.handle))
+            t0 = time.perf_counter()
+            dp.step(steps)
+            _lib.check(lib, lib.lbm_synchronize(ctx.handle))
+            ms = 1e3*(time.perf_counter() - t0)/steps
+            out.setdefault(name, {})[drag_name] = {"ms_per_step": round(ms, 4), "coupled_steps_per_s": round(1e3/ms, 2),
+                                                   "mlups": round(w.num_nodes/ms/1e3, 1)}
+            del dp, ctx
+        out[name]["centred_cost_vs_explicit"] = round(out[name]["centred"]["ms_per_step"]/
+                                                      out[name]["explicit"]["ms_per_step"], 3)
+    return {
+        "metric": "MLUPS & coupled steps/s (HBM GB/s %) 256^3+20k beads; FP16/MLP vs plain @err",
+        "value": out["keys"]["centred"]["mlups"],
+        "unit": "MLUPS (centred drag, Keys)",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": warmup,
+        "higher_is_better": True,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "config2 with the centred drag: 256^3 Poiseuille, bounce-back walls, 20,000 beads, "
+                               "fluctuating kT=2.494", "stencils": ["nearest", "keys"]},
+        "drags": out,
+    }
+
+
 def closure_study(steps, warmup, snapshots=10, interval=20, epochs=40, batch=2048):
     """config5: the learned closure (nearest node + MLP, Keys fallback) against the Keys stencil on 128^3 with 100,000
     chained beads.  Generates 10 x 100,000 training pairs from the Keys run itself, trains on the GPU, then times the
@@ -321,6 +364,9 @@ def main():
         return 0
     if args.workload == "config5":
         print(json.dumps(closure_study(args.steps, args.warmup)), flush=True)
+        return 0
+    if args.workload == "centred":
+        print(json.dumps(centred_study(args.steps, args.warmup)), flush=True)
         return 0
 
     from lbm_amd import _lib

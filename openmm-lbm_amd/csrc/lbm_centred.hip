@@ -271,14 +271,32 @@ __global__ void __launch_bounds__(BLOCK) cgMaxAbs(const double* __restrict__ x, 
         partials[8*blockIdx.x] = lds[0];
 }
 
+/* The partial rows of the RED_BLOCKS blocks of a reduction, combined by one wavefront (launched with 64 threads):
+   lane l takes the rows l, l + 64, ... in order, then the lanes combine in a fixed butterfly, so the result is the same
+   in every run.  MAX takes the largest value instead of the sum.  Every lane returns the result. */
+template <int COUNT, bool MAX = false>
+__device__ __forceinline__ void combinePartials(const double* __restrict__ partials, double v[COUNT]) {
+    static_assert(RED_BLOCKS % 64 == 0, "whole rows per lane");
+    for (int q = 0; q < COUNT; q++)
+        v[q] = 0.0;
+    for (int b = threadIdx.x; b < RED_BLOCKS; b += 64)
+        for (int q = 0; q < COUNT; q++)
+            v[q] = MAX ? fmax(v[q], partials[8*b + q]) : v[q] + partials[8*b + q];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+        for (int q = 0; q < COUNT; q++) {
+            double o = __shfl_xor(v[q], off, 64);
+            v[q] = MAX ? fmax(v[q], o) : v[q] + o;
+        }
+}
+
 /* Fixed-point scale of the next spread: 2^(52 - e), 2^e >= max |x| (1 if x is zero). */
-__global__ void cgScale(const double* __restrict__ partials, double* __restrict__ scalars) {
+__global__ void __launch_bounds__(64) cgScale(const double* __restrict__ partials, double* __restrict__ scalars) {
+    double m[1];
+    combinePartials<1, true>(partials, m);
     if (threadIdx.x != 0)
         return;
-    double m = 0.0;
-    for (int b = 0; b < RED_BLOCKS; b++)
-        m = fmax(m, partials[8*b]);
-    int e = m > 0 ? ilogb(m) + 1 : 0;
+    int e = m[0] > 0 ? ilogb(m[0]) + 1 : 0;
     scalars[SCALE] = ldexp(1.0, 52 - e);
     scalars[INV_SCALE] = ldexp(1.0, e - 52);
 }
@@ -375,21 +393,17 @@ __global__ void __launch_bounds__(BLOCK) cgInit(int n, CgArgs c, double* partial
     reduceBlock(w, 3, partialsB);
 }
 
-__global__ void cgFinishInit(const double* __restrict__ partials, const double* __restrict__ partialsB,
-        double* __restrict__ scalars) {
+__global__ void __launch_bounds__(64) cgFinishInit(const double* __restrict__ partials,
+        const double* __restrict__ partialsB, double* __restrict__ scalars) {
+    double v[6], w[3];
+    combinePartials<6>(partials, v);
+    combinePartials<3>(partialsB, w);
     if (threadIdx.x != 0)
         return;
-    double v[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-    for (int b = 0; b < RED_BLOCKS; b++) {
-        for (int q = 0; q < 6; q++)
-            v[q] += partials[8*b + q];
-        for (int q = 0; q < 3; q++)
-            v[6 + q] += partialsB[8*b + q];
-    }
     for (int k = 0; k < 3; k++) {
         scalars[RZ + k] = v[k];
         scalars[NORM + k] = v[3 + k];
-        scalars[RR + k] = v[6 + k];
+        scalars[RR + k] = w[k];
     }
 }
 
@@ -402,13 +416,11 @@ __global__ void __launch_bounds__(BLOCK) cgDot(int n, CgArgs c) {
     reduceBlock(v, 3, c.partials);
 }
 
-__global__ void cgFinishAlpha(const double* __restrict__ partials, double* __restrict__ scalars) {
+__global__ void __launch_bounds__(64) cgFinishAlpha(const double* __restrict__ partials, double* __restrict__ scalars) {
+    double v[3];
+    combinePartials<3>(partials, v);
     if (threadIdx.x != 0)
         return;
-    double v[3] = {0, 0, 0};
-    for (int b = 0; b < RED_BLOCKS; b++)
-        for (int k = 0; k < 3; k++)
-            v[k] += partials[8*b + k];
     for (int k = 0; k < 3; k++) {
         scalars[DAD + k] = v[k];
         scalars[ALPHA + k] = v[k] > 0 ? scalars[RZ + k]/v[k] : 0.0;
@@ -434,13 +446,11 @@ __global__ void __launch_bounds__(BLOCK) cgUpdate(int n, CgArgs c) {
 }
 
 /* beta = r.z(new)/r.z(old), r.z <- r.z(new), r.r. */
-__global__ void cgFinishBeta(const double* __restrict__ partials, double* __restrict__ scalars) {
+__global__ void __launch_bounds__(64) cgFinishBeta(const double* __restrict__ partials, double* __restrict__ scalars) {
+    double v[6];
+    combinePartials<6>(partials, v);
     if (threadIdx.x != 0)
         return;
-    double v[6] = {0, 0, 0, 0, 0, 0};
-    for (int b = 0; b < RED_BLOCKS; b++)
-        for (int q = 0; q < 6; q++)
-            v[q] += partials[8*b + q];
     for (int k = 0; k < 3; k++) {
         double rz = scalars[RZ + k];
         scalars[BETA + k] = rz > 0 ? v[k]/rz : 0.0;
@@ -505,6 +515,7 @@ struct CentredState {
     // Stencils: conjugate gradients.
     CgArgs cg{};
     double* partialsB = nullptr;
+    double* hostScalars = nullptr;        // pinned: the convergence check reads the scalars once per iteration
     std::vector<void*> buffers;
 };
 
@@ -551,6 +562,8 @@ CentredState* centredCreate(int numParticles, int stencil, uint32_t numNodes) {
             check(hipMemset(c.spread, 0, 3*(size_t) numNodes*sizeof(unsigned long long)), "memset");
             st->buffers = {c.rhs, c.diag, c.F, c.r, c.z, c.d, c.Ad, c.slotNode, c.slotW, c.slotWInvRho, c.slotSolid,
                            c.spread, c.partials, c.scalars, st->partialsB};
+            check(hipHostMalloc((void**) &st->hostScalars, NUM_SCALARS*sizeof(double), hipHostMallocDefault),
+                    "pinned scalars");
         }
     }
     catch (...) {
@@ -566,6 +579,8 @@ void centredDestroy(CentredState* st) {
     for (void* p : st->buffers)
         if (p)
             (void) hipFree(p);
+    if (st->hostScalars)
+        (void) hipHostFree(st->hostScalars);
     delete st;
 }
 
@@ -588,8 +603,8 @@ void multiply(CentredState* st, const double* x, double* Ax, const double* mass,
 }
 
 bool converged(CentredState* st, hipStream_t stream) {
-    double s[NUM_SCALARS];
-    check(hipMemcpyAsync(s, st->cg.scalars, sizeof(s), hipMemcpyDeviceToHost, stream), "download");
+    double* s = st->hostScalars;
+    check(hipMemcpyAsync(s, st->cg.scalars, NUM_SCALARS*sizeof(double), hipMemcpyDeviceToHost, stream), "download");
     check(hipStreamSynchronize(stream), "sync");
     for (int k = 0; k < 3; k++)
         if (s[RR + k] > CG_TOLERANCE*CG_TOLERANCE*s[NORM + k])
