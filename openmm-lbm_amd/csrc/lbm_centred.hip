@@ -460,13 +460,27 @@ __global__ void __launch_bounds__(64) cgFinishBeta(const double* __restrict__ pa
     }
 }
 
+/* d = z + beta d, and the largest |d| of each block -> partials[block][0] (for the scale of the next spread). */
 __global__ void __launch_bounds__(BLOCK) cgDirection(int n, CgArgs c) {
     double beta[3] = {c.scalars[BETA], c.scalars[BETA + 1], c.scalars[BETA + 2]};
+    double m = 0.0;
     for (int i = blockIdx.x*BLOCK + threadIdx.x; i < n; i += gridDim.x*BLOCK)
         for (int k = 0; k < 3; k++) {
             size_t t = 3*(size_t) i + k;
-            c.d[t] = c.z[t] + beta[k]*c.d[t];
+            double d = c.z[t] + beta[k]*c.d[t];
+            c.d[t] = d;
+            m = fmax(m, fabs(d));
         }
+    __shared__ double lds[BLOCK];
+    lds[threadIdx.x] = m;
+    __syncthreads();
+    for (int s = BLOCK/2; s > 0; s >>= 1) {
+        if (threadIdx.x < s)
+            lds[threadIdx.x] = fmax(lds[threadIdx.x], lds[threadIdx.x + s]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        c.partials[8*blockIdx.x] = lds[0];
 }
 
 /* The forces and the reactions -w F of the solution, one group of LANES lanes per particle (block layout of the explicit
@@ -586,12 +600,12 @@ void centredDestroy(CentredState* st) {
 
 namespace {
 
+/* Ax = A x.  The largest |x| of each block must be in c.partials (cgMaxAbs, or cgDirection for x = d). */
 template <typename S>
 void multiply(CentredState* st, const double* x, double* Ax, const double* mass, double aa, hipStream_t stream) {
     CgArgs& c = st->cg;
     int n = st->n;
     size_t slotItems = (size_t) n*c.slots;
-    cgMaxAbs<<<RED_BLOCKS, BLOCK, 0, stream>>>(x, n, c.partials);
     cgScale<<<1, 64, 0, stream>>>(c.partials, c.scalars);
     cgSpread<<<blocksOf(slotItems, BLOCK), BLOCK, 0, stream>>>(x, n, c);
     switch (st->lanes) {
@@ -647,20 +661,25 @@ void centredCoupleT(CentredState* st, const S* src, const uint32_t* mask, const 
                 g[0], g[1], g[2], c);
         break;
     }
+    cgMaxAbs<<<RED_BLOCKS, BLOCK, 0, stream>>>(c.F, n, c.partials);
     multiply<S>(st, c.F, c.Ad, mass, aa, stream);
     cgInit<<<RED_BLOCKS, BLOCK, 0, stream>>>(n, c, st->partialsB);
     cgFinishInit<<<1, 64, 0, stream>>>(c.partials, st->partialsB, c.scalars);
     check(hipGetLastError(), "launch");
+    // Each iteration starts with the new direction (d = z after cgInit), so that the last one computes none.
     for (int iteration = 0; !converged(st, stream); iteration++) {
         if (iteration == CG_MAX_ITERATIONS)
             throw std::runtime_error("LBMForce: the centred drag with the interpolation stencil did not converge in 1000 "
                     "iterations of the conjugate gradients");
+        if (iteration == 0)
+            cgMaxAbs<<<RED_BLOCKS, BLOCK, 0, stream>>>(c.d, n, c.partials);
+        else
+            cgDirection<<<RED_BLOCKS, BLOCK, 0, stream>>>(n, c);
         multiply<S>(st, c.d, c.Ad, mass, aa, stream);
         cgDot<<<RED_BLOCKS, BLOCK, 0, stream>>>(n, c);
         cgFinishAlpha<<<1, 64, 0, stream>>>(c.partials, c.scalars);
         cgUpdate<<<RED_BLOCKS, BLOCK, 0, stream>>>(n, c);
         cgFinishBeta<<<1, 64, 0, stream>>>(c.partials, c.scalars);
-        cgDirection<<<RED_BLOCKS, BLOCK, 0, stream>>>(n, c);
         check(hipGetLastError(), "launch");
     }
     switch (st->lanes) {
