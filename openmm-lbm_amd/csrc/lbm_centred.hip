@@ -65,12 +65,14 @@ __device__ __forceinline__ void particleKnowns(const double* vel, const double* 
     }
 }
 
-__device__ __forceinline__ void addReaction(uint32_t node, bool solid, const double r[3], const ParticleArgs& a,
-        unsigned long long* reactionFixed, uint32_t* touched, double wallAcc[3]) {
+/* The reaction r on a node: into its fixed-point accumulators, or to the wall on a solid node.  Returns whether an
+   accumulator changed (the node is then flagged in the touched mask by markTouched). */
+__device__ __forceinline__ bool addReaction(uint32_t node, bool solid, const double r[3], const ParticleArgs& a,
+        unsigned long long* reactionFixed, double wallAcc[3]) {
     if (solid) {
         for (int k = 0; k < 3; k++)
             wallAcc[k] += r[k];          // lattice momentum; the host converts the wall partials
-        return;
+        return false;
     }
     bool any = false;
     for (int k = 0; k < 3; k++) {
@@ -80,8 +82,7 @@ __device__ __forceinline__ void addReaction(uint32_t node, bool solid, const dou
             any = true;
         }
     }
-    if (any)
-        atomicOr(touched + (node >> 5), 1u << (node & 31));
+    return any;
 }
 
 __device__ __forceinline__ void blockPartials(const double acc[3], double* partials) {
@@ -154,7 +155,7 @@ __global__ void __launch_bounds__(BLOCK) centredNearest(const S* __restrict__ sr
         }
         if (a.isStep) {
             double r[3] = {-sum[0], -sum[1], -sum[2]};
-            addReaction(node, solid, r, a, reactionFixed, touched, wallAcc);
+            markTouched<1>(touched, node, addReaction(node, solid, r, a, reactionFixed, wallAcc), 0);
         }
     }
     blockPartials(wallAcc, partials);
@@ -492,6 +493,8 @@ __global__ void __launch_bounds__(BLOCK) cgForces(int n, ParticleArgs a, CgArgs 
     int lane = threadIdx.x % LANES;
     int i = blockIdx.x*GROUPS + threadIdx.x/LANES;
     double wallAcc[3] = {0, 0, 0};
+    uint32_t node = 0;
+    bool flagged = false;
     if (i < n) {
         double F[3] = {c.F[3*(size_t) i], c.F[3*(size_t) i + 1], c.F[3*(size_t) i + 2]};
         if (lane == 0)
@@ -500,12 +503,16 @@ __global__ void __launch_bounds__(BLOCK) cgForces(int n, ParticleArgs a, CgArgs 
         if (a.isStep && lane < c.slots) {
             size_t sl = (size_t) i*c.slots + lane;
             double w = c.slotW[sl];
+            node = c.slotNode[sl];
             if (w != 0.0) {
                 double r[3] = {-w*F[0], -w*F[1], -w*F[2]};
-                addReaction(c.slotNode[sl], c.slotSolid[sl] != 0, r, a, reactionFixed, touched, wallAcc);
+                flagged = addReaction(node, c.slotSolid[sl] != 0, r, a, reactionFixed, wallAcc);
             }
         }
     }
+    // The slots of a particle are its lanes, x fastest, as in the explicit coupling: one OR per run of a row's lanes.
+    if (a.isStep)
+        markTouched<LANES == 64 ? 4 : LANES == 32 ? 3 : 2>(touched, node, flagged, lane);
     blockPartials(wallAcc, partials);
 }
 
