@@ -196,7 +196,9 @@ __global__ void __launch_bounds__(BLOCK) cgSetup(const S* __restrict__ src, cons
     bool active = i < a.numParticles;
     int part = active ? particles[i] : 0;
     const double* x = positions + 3*(size_t) part;
-    int w = stencilWidth(a.stencil);
+    // The stencil of the lanes (groupStencil): each lane evaluates its own slot's kernel weight per axis.
+    constexpr int STENCIL = groupStencil<LANES>();
+    constexpr int w = STENCIL == LBM_STENCIL_KEYS ? 4 : STENCIL == LBM_STENCIL_THREE_POINT ? 3 : 2;
     bool inStencil = active && lane < c.slots;
     double ux = 0, uy = 0, uz = 0, self = 0;
     if (inStencil) {
@@ -206,11 +208,9 @@ __global__ void __launch_bounds__(BLOCK) cgSetup(const S* __restrict__ src, cons
         int nodeIdx[3];
         double weight = 1.0;
         for (int k = 0; k < 3; k++) {
-            int ix[4];
-            double wx[4];
-            axisWeights(a.stencil, w, wrapCoordinate(x[k], a.dx, size[k]), size[k], openAxis[k] != 0, ix, wx);
-            nodeIdx[k] = ix[sub[k]];
-            weight *= wx[sub[k]];
+            double wk;
+            axisSlot<STENCIL>(wrapCoordinate(x[k], a.dx, size[k]), size[k], openAxis[k] != 0, sub[k], nodeIdx[k], wk);
+            weight *= wk;
         }
         uint32_t node = (uint32_t) (nodeIdx[0] + (int) a.g.nx*(nodeIdx[1] + (int) a.g.ny*nodeIdx[2]));
         bool solid = solidAt(mask, node);
