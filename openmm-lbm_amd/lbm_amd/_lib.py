@@ -138,7 +138,7 @@ def load_library(path=None):
         return _product
     p = path or LIBRARY_PATH
     if not os.path.exists(p):
-        raise RuntimeError(f"the gfx950 LBMForce library {p} is not built: run __graft_entry__.build()")
+        raise RuntimeError(f"the gfx950 LBMForce library {p} is not built: run make -C openmm-lbm_amd")
     lib = ctypes.CDLL(p)
     declare(lib, KERNEL_FUNCTIONS)
     declare(lib, PRODUCT_FUNCTIONS)

@@ -2,7 +2,7 @@
  * FP64 CPU ORACLE -- TEST INFRASTRUCTURE ONLY.
  *
  * A plain double-precision restatement of the reference's Reference platform for the
- * one-GPU LBMForce path, used by tests/, __graft_entry__.smoke() and bench.py's CPU
+ * one-GPU LBMForce path, used by tests/, the smoke check at the repository root and bench.py's CPU
  * baseline as the checker.  Nothing in the product library links or calls it.
  *
  * It follows, step for step, platforms/reference/src/ReferenceLBMKernels.cpp (push

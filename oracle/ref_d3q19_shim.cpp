@@ -4,7 +4,7 @@
  * reference tree by oracle/Makefile into oracle/_ref/libref_d3q19.so.  tests/test_oracle.py
  * (test_algebra_matches_reference_header) compares the oracle's restatement of each function with
  * it on random node states, and tests/golden/make_d3q19_vectors.py writes fixtures from it.
- * Nothing else loads it; it is not built on the GPU box.
+ * Nothing else loads it; it is built only where the reference tree is present.
  */
 #include "internal/D3Q19.h"
 

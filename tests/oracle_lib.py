@@ -1,5 +1,6 @@
 """Test infrastructure: loads the FP64 CPU oracle (oracle/liblbm_oracle.so), which exports the same kernel C-ABI as
-the gfx950 library, plus a few test hooks.  Only tests, __graft_entry__.smoke() and bench.py's CPU baseline use it."""
+the gfx950 library, plus a few test hooks.  Only tests, the smoke check at the repository root and bench.py's CPU
+baseline use it."""
 import ctypes
 import os
 
