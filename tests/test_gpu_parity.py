@@ -352,3 +352,27 @@ def test_lattice_beyond_4_gib():
     print(f"relative momentum change {np.linalg.norm(p1 - p0)/scale:.2e}, momentum given to the fluid {moved/scale:.2f}")
     assert moved > 0.5*scale
     assert np.linalg.norm(p1 - p0)/scale < 1e-5
+
+
+def test_config2_full_size_properties():
+    """The benchmark workload at its full size (256^3, bounce-back walls at z = 0 and 255, body force, fluctuating
+    fluid, 20,000 Keys beads under a wall potential), 2000 device-resident steps: properties that hold at any size.  The
+    fluid mass is conserved (walls, fluctuations and coupling move no mass) to FP32 storage rounding, the Mach number
+    stays finite and far below the limit, the beads stay between the walls, and the body force has started the flow
+    along +x everywhere in the mean."""
+    from lbm_amd import workloads
+    w = workloads.config2()
+    c = w.build()
+    dp = w.device_particles(c)
+    m0, p0 = c.getFluidTotals()
+    dp.step(2000)
+    m1, p1 = c.getFluidTotals()
+    pos, vel = dp.pull()
+    mach = c.getFluidMachNumber()
+    print(f"mass drift {abs(m1 - m0)/m0:.2e}, momentum {p1}, Mach {mach:.4f}")
+    assert abs(m1 - m0)/m0 < 1e-7
+    assert np.isfinite(mach) and mach < 0.1
+    assert np.all(np.isfinite(pos)) and np.all(np.isfinite(vel))
+    z = pos[:, 2]/w.dx
+    assert z.min() > 0.0 and z.max() < 255.5          # none has crossed a wall (positions are not wrapped)
+    assert p1[0] > p0[0] and abs(p1[1]) < 0.1*p1[0] and abs(p1[2]) < 0.1*p1[0]
