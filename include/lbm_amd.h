@@ -143,6 +143,8 @@ int lbm_compute_lattice_parameters(const lbm_force_settings* settings, const dou
  * particles: system indices of the coupled particles; particle_masses: their masses (Da);
  * num_system_particles: length of the position/velocity/force arrays of the other calls.
  * solid_nodes: node indices i + nx*(j + ny*k), sorted, without repetitions.
+ * At most 2^30 nodes (LBM_ERROR_INVALID beyond).  LBM_AMD_WIDE_ADDRESSING=1 in the environment makes the lattice
+ * pass use 64-bit indices at any size (it does by itself where an array of populations exceeds 4 GiB).
  */
 int lbm_create(const lbm_params* params, const int* solid_nodes, int num_solid, const int* particles,
         const double* particle_masses, int num_particles, int num_system_particles, lbm_context** out);
