@@ -13,6 +13,7 @@ namespace lbm {
 
 constexpr int BLOCK = 256;
 constexpr int FUSED_THREADS = 512;         // work-group size of the fused pass (measured best on MI355X)
+constexpr int FUSED_THREADS_REDUCED = 256; // the same with FP16/BF16 storage
 constexpr int REDUCE_BLOCKS = 1024;       // fixed grid of the FP64 reductions: fixed summation order
 // Fixed-point scale of the particle reactions, 2^50: each contribution is rounded to 8.9e-16 lattice units, and a node
 // holds up to +-8192 lattice units per step (a bead of 6,000 cell masses at gamma dt = 2 and a velocity difference of
