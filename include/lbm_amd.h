@@ -69,8 +69,10 @@ extern "C" {
 
 /*
  * The converted lattice parameters: LBMLatticeParameters (LBMKernels.h:30-92) without
- * the MPI fields (procs, particleCopiesCheck, halo exchange), plus the storage type.
- * The coupled particles and solid nodes are passed to lbm_create() as arrays.
+ * the MPI fields (procs, particleCopiesCheck), plus the storage type.  The coupled
+ * particles and solid nodes are passed to lbm_create() as arrays.  The halo-exchange
+ * flags (LBMKernels.h:40-42) only select which fields lbm_get_fluid_fields() fills
+ * in its halo layer: with the whole lattice in one domain there is nothing to exchange.
  */
 typedef struct lbm_params {
     int nx, ny, nz;
@@ -94,6 +96,8 @@ typedef struct lbm_params {
     double face_velocity[6][3];      /* lattice units */
     double face_density[6];          /* lattice units, 1 = fluid at rest */
     int storage;                     /* LBM_STORAGE_* */
+    int density_halo_exchange;       /* densityHaloExchange */
+    int velocity_halo_exchange;      /* velocityHaloExchange */
 } lbm_params;
 
 /*
@@ -117,6 +121,7 @@ typedef struct lbm_force_settings {
     double face_velocity[6][3];      /* nm/ps */
     double face_density[6];          /* Da/nm^3, 0 = fluid at rest */
     int storage;
+    int density_halo_exchange, velocity_halo_exchange;   /* LBMForce.h:242-263, off by default */
 } lbm_force_settings;
 
 typedef struct lbm_context lbm_context;
@@ -175,15 +180,26 @@ int lbm_begin_step_device(lbm_context* ctx, int64_t step_count, const double* po
 int lbm_execute_device(lbm_context* ctx, int64_t step_count, int include_forces, const double* positions,
         const double* velocities, double* forces);
 
-/* Replaces CalcLBMForceKernel::copyParametersToContext (LBMKernels.h:135). */
+/*
+ * Replaces CalcLBMForceKernel::copyParametersToContext (LBMKernels.h:135), with the checks
+ * that LBMForceImpl::updateParametersInContext makes before it (LBMForceImpl.cpp:388-414):
+ * a change of the grid, the fluid density or viscosity, the wall scheme, the halo exchange,
+ * the face boundary types, the drag scheme, the interpolation stencil or the fluctuations is
+ * refused with LBM_ERROR_INVALID and the reference's message, and the context is unchanged.
+ * (The coupled particles and the solid nodes are not part of lbm_params: the caller keeps them.)
+ */
 int lbm_copy_parameters(lbm_context* ctx, const lbm_params* params);
 
 /*
- * Replaces CalcLBMForceKernel::getFluidFields with halo = false (LBMKernels.h:144;
- * ReferenceLBMKernels.cpp:1349-1368, 1403-1436): density (Da/nm^3, N values) and velocity
- * (nm/ps, 3N values, xyz per node) in node order; zero at solid nodes.  Either may be NULL.
+ * Replaces CalcLBMForceKernel::getFluidFields (LBMKernels.h:136-145; ReferenceLBMKernels.cpp:1349-1368,
+ * 1403-1436): density (Da/nm^3) and velocity (nm/ps, xyz per node); zero at solid nodes.  Either may be
+ * NULL.  halo = 0: N values in node order i + nx*(j + ny*k).  halo = 1: the lattice with a layer one node
+ * thick around it, (nx+2)(ny+2)(nz+2) values, node (i, j, k) for i from -1 to nx and so on at index
+ * (i+1) + (nx+2)*((j+1) + (ny+2)*(k+1)) (LBMDecomposition.cpp:183-204 with one domain): the layer holds the
+ * fields of the node across the periodic boundary for the fields whose halo is exchanged
+ * (density_halo_exchange, velocity_halo_exchange), and NaN for the other fields and beyond open faces.
  */
-int lbm_get_fluid_fields(lbm_context* ctx, double* density, double* velocity);
+int lbm_get_fluid_fields(lbm_context* ctx, double* density, double* velocity, int halo);
 
 /* Replaces getFluidMachNumber (LBMKernels.h:149; ReferenceLBMKernels.cpp:1449-1468). */
 int lbm_get_fluid_mach_number(lbm_context* ctx, double* mach);

@@ -438,7 +438,7 @@ int lbm_batch_get_fluid_fields(lbm_batch* b, int r, double* density, double* vel
         double* dRho = alloc<double>(n, "fields");
         double* dU = alloc<double>(3*n, "fields");
         const float* cur = (b->step & 1) ? d.popB : d.popA;
-        launchFields(LBM_STORAGE_FP32, cur, nullptr, d.step.g, p.density, p.dx/p.dt, p.body_acceleration[0],
+        launchFields(LBM_STORAGE_FP32, cur, nullptr, d.step.g, FieldsLayout(), p.density, p.dx/p.dt, p.body_acceleration[0],
                 p.body_acceleration[1], p.body_acceleration[2], dRho, dU, b->stream);
         hipError_t e1 = density ? hipMemcpyAsync(density, dRho, n*sizeof(double), hipMemcpyDeviceToHost, b->stream) : hipSuccess;
         hipError_t e2 = velocity ? hipMemcpyAsync(velocity, dU, 3*n*sizeof(double), hipMemcpyDeviceToHost, b->stream) : hipSuccess;

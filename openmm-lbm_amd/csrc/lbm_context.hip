@@ -802,31 +802,70 @@ int lbm_copy_parameters(lbm_context* c, const lbm_params* params) {
     LBM_TRY({
         if (!params)
             throw Error(LBM_ERROR_INVALID, "lbm_copy_parameters: null argument");
-        if (params->nx != c->p.nx || params->ny != c->p.ny || params->nz != c->p.nz)
-            throw Error(LBM_ERROR_INVALID, "updateParametersInContext: the grid size cannot be changed");
+        // The context's device layout, boundary lists, coupling buffers and random streams follow from these
+        // settings at creation; LBMForceImpl::updateParametersInContext refuses to change them
+        // (LBMForceImpl.cpp:388-414), and so does this entry point, which a caller may use without that check.
+        const lbm_params& o = c->p;
+        const lbm_params& u = *params;
+        auto refuse = [](const char* what) {
+            throw Error(LBM_ERROR_INVALID, std::string("updateParametersInContext: ") + what);
+        };
+        if (u.nx != o.nx || u.ny != o.ny || u.nz != o.nz)
+            refuse("the grid size cannot be changed");
+        if (u.density != o.density || u.tau != o.tau)
+            refuse("the fluid density and viscosity cannot be changed");
+        if (u.wall_scheme != o.wall_scheme)
+            refuse("the wall scheme cannot be changed");
+        if (u.density_halo_exchange != o.density_halo_exchange || u.velocity_halo_exchange != o.velocity_halo_exchange)
+            refuse("the exchange of the halo cannot be switched on or off");
+        for (int f = 0; f < 6; f++)
+            if (u.face_boundary[f] != o.face_boundary[f])
+                refuse("the boundary types of the faces cannot be changed");
+        if (u.drag_scheme != o.drag_scheme)
+            refuse("the drag scheme cannot be changed");
+        if (u.interpolation_stencil != o.interpolation_stencil)
+            refuse("the interpolation stencil cannot be changed");
+        if ((u.fluid_fluctuations != 0) != (o.fluid_fluctuations != 0))
+            refuse("the fluid fluctuations cannot be switched on or off");
         int storage = c->p.storage;
         c->p = *params;
         c->p.storage = storage;
     })
 }
 
-int lbm_get_fluid_fields(lbm_context* c, double* density, double* velocity) {
+int lbm_get_fluid_fields(lbm_context* c, double* density, double* velocity, int halo) {
     LBM_CHECK_CTX(c);
     LBM_TRY({
-        size_t n = c->g.n;
-        double* dRho = density ? deviceAlloc<double>(n) : nullptr;
-        double* dU = velocity ? deviceAlloc<double>(3*n) : nullptr;
-        launchFields(c->p.storage, c->popA, c->mask, c->g, c->p.density, c->p.dx/c->p.dt, c->p.body_acceleration[0],
-                c->p.body_acceleration[1], c->p.body_acceleration[2], dRho, dU, c->stream);
-        if (density)
-            HIP_CHECK(hipMemcpyAsync(density, dRho, n*sizeof(double), hipMemcpyDeviceToHost, c->stream));
-        if (velocity)
-            HIP_CHECK(hipMemcpyAsync(velocity, dU, 3*n*sizeof(double), hipMemcpyDeviceToHost, c->stream));
-        HIP_CHECK(hipStreamSynchronize(c->stream));
-        if (dRho)
-            HIP_CHECK(hipFree(dRho));
-        if (dU)
-            HIP_CHECK(hipFree(dU));
+        FieldsLayout layout;
+        layout.pad = halo ? 1 : 0;
+        layout.densityHalo = c->p.density_halo_exchange;
+        layout.velocityHalo = c->p.velocity_halo_exchange;
+        size_t n = fieldsCount(c->g, layout);
+        if (!density && !velocity) {
+            g_lastError.clear();
+            return LBM_OK;
+        }
+        // One staging buffer for both fields: n densities, then 3n velocities.
+        double* stage = deviceAlloc<double>(4*n);
+        double* dRho = density ? stage : nullptr;
+        double* dU = velocity ? stage + n : nullptr;
+        try {
+            launchFields(c->p.storage, c->popA, c->mask, c->g, layout, c->p.density, c->p.dx/c->p.dt,
+                    c->p.body_acceleration[0], c->p.body_acceleration[1], c->p.body_acceleration[2], dRho, dU,
+                    c->stream);
+            HIP_CHECK(hipGetLastError());
+            if (density)
+                HIP_CHECK(hipMemcpyAsync(density, dRho, n*sizeof(double), hipMemcpyDeviceToHost, c->stream));
+            if (velocity)
+                HIP_CHECK(hipMemcpyAsync(velocity, dU, 3*n*sizeof(double), hipMemcpyDeviceToHost, c->stream));
+            HIP_CHECK(hipStreamSynchronize(c->stream));
+        }
+        catch (...) {
+            (void) hipStreamSynchronize(c->stream);
+            (void) hipFree(stage);
+            throw;
+        }
+        HIP_CHECK(hipFree(stage));
     })
 }
 

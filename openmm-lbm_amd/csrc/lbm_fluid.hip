@@ -441,18 +441,51 @@ __global__ void __launch_bounds__(BLOCK) scatterState(S* __restrict__ dst, const
     });
 }
 
+/*
+ * Density and velocity at the entries of the output layout (ReferenceLBMKernels.cpp:1349-1368, 1403-1436): entry l of
+ * the lattice padded by `pad` nodes on every side, i fastest.  A padded entry holds the node it wraps onto across a
+ * periodic boundary if the halo of that field is exchanged, NaN otherwise and beyond an open face
+ * (LBMDecomposition.cpp:183-204 with one domain).
+ */
 template <typename S>
 __global__ void __launch_bounds__(BLOCK) fieldsKernel(const S* __restrict__ src, const uint32_t* __restrict__ mask,
-        Geometry g, double density, double velocityScale, double gx, double gy, double gz, double* __restrict__ rhoOut,
-        double* __restrict__ uOut) {
-    uint32_t node = blockIdx.x*BLOCK + threadIdx.x;
-    if (node >= g.n)
+        Geometry g, FieldsLayout out, uint32_t count, double density, double velocityScale, double gx, double gy,
+        double gz, double* __restrict__ rhoOut, double* __restrict__ uOut) {
+    uint32_t l = blockIdx.x*BLOCK + threadIdx.x;
+    if (l >= count)
         return;
+    uint32_t node = l;
+    bool wantRho = rhoOut != nullptr, wantU = uOut != nullptr;
+    if (out.pad > 0) {
+        uint32_t wx = g.nx + 2*out.pad, wy = g.ny + 2*out.pad;
+        int li = (int) (l % wx) - out.pad, lj = (int) ((l/wx) % wy) - out.pad, lk = (int) (l/(wx*wy)) - out.pad;
+        int c[3] = {li, lj, lk}, n[3] = {(int) g.nx, (int) g.ny, (int) g.nz}, open[3] = {g.openX, g.openY, g.openZ};
+        bool inside = true, beyondFace = false;
+        for (int a = 0; a < 3; a++) {
+            if (c[a] < 0 || c[a] >= n[a]) {
+                inside = false;
+                beyondFace = beyondFace || open[a] != 0;
+                c[a] = (c[a] + n[a]) % n[a];
+            }
+        }
+        bool rhoValid = !beyondFace && (inside || out.densityHalo != 0);
+        bool uValid = !beyondFace && (inside || out.velocityHalo != 0);
+        const double nan = __builtin_nan("");
+        if (wantRho && !rhoValid)
+            rhoOut[l] = nan;
+        if (wantU && !uValid)
+            uOut[3*(size_t) l] = uOut[3*(size_t) l + 1] = uOut[3*(size_t) l + 2] = nan;
+        wantRho = wantRho && rhoValid;
+        wantU = wantU && uValid;
+        if (!wantRho && !wantU)
+            return;
+        node = (uint32_t) c[0] + g.nx*((uint32_t) c[1] + g.ny*(uint32_t) c[2]);
+    }
     if (mask && isSolid(mask, node)) {
-        if (rhoOut)
-            rhoOut[node] = 0.0;
-        if (uOut)
-            uOut[3*(size_t) node] = uOut[3*(size_t) node + 1] = uOut[3*(size_t) node + 2] = 0.0;
+        if (wantRho)
+            rhoOut[l] = 0.0;
+        if (wantU)
+            uOut[3*(size_t) l] = uOut[3*(size_t) l + 1] = uOut[3*(size_t) l + 2] = 0.0;
         return;
     }
     uint32_t i, j, k;
@@ -467,12 +500,12 @@ __global__ void __launch_bounds__(BLOCK) fieldsKernel(const S* __restrict__ src,
         jy = addTerm<CY[q]>(jy, v);
         jz = addTerm<CZ[q]>(jz, v);
     });
-    if (rhoOut)
-        rhoOut[node] = r*density;
-    if (uOut) {
-        uOut[3*(size_t) node] = (jx*(1.0/r) + gx*0.5)*velocityScale;
-        uOut[3*(size_t) node + 1] = (jy*(1.0/r) + gy*0.5)*velocityScale;
-        uOut[3*(size_t) node + 2] = (jz*(1.0/r) + gz*0.5)*velocityScale;
+    if (wantRho)
+        rhoOut[l] = r*density;
+    if (wantU) {
+        uOut[3*(size_t) l] = (jx*(1.0/r) + gx*0.5)*velocityScale;
+        uOut[3*(size_t) l + 1] = (jy*(1.0/r) + gy*0.5)*velocityScale;
+        uOut[3*(size_t) l + 2] = (jz*(1.0/r) + gz*0.5)*velocityScale;
     }
 }
 
@@ -599,11 +632,19 @@ void launchScatter(int storage, void* dst, const double* in, const Geometry& g, 
     LBM_DISPATCH_STORAGE(storage, S, { scatterState<S><<<gridFor(g.n), BLOCK, 0, stream>>>((S*) dst, in, g); })
 }
 
-void launchFields(int storage, const void* src, const uint32_t* mask, const Geometry& g, double density, double velocityScale,
-        double gx, double gy, double gz, double* rhoOut, double* uOut, hipStream_t stream) {
+uint64_t fieldsCount(const Geometry& g, const FieldsLayout& out) {
+    return (uint64_t) (g.nx + 2*out.pad)*(g.ny + 2*out.pad)*(g.nz + 2*out.pad);
+}
+
+void launchFields(int storage, const void* src, const uint32_t* mask, const Geometry& g, const FieldsLayout& out,
+        double density, double velocityScale, double gx, double gy, double gz, double* rhoOut, double* uOut,
+        hipStream_t stream) {
+    uint64_t count = fieldsCount(g, out);
+    if (count > 0xFFFFFFFFull)
+        throw std::runtime_error("getFluidFields: the lattice with its halo exceeds 2^32 nodes");
     LBM_DISPATCH_STORAGE(storage, S, {
-        fieldsKernel<S><<<gridFor(g.n), BLOCK, 0, stream>>>((const S*) src, mask, g, density, velocityScale, gx, gy, gz,
-                rhoOut, uOut);
+        fieldsKernel<S><<<gridFor(count), BLOCK, 0, stream>>>((const S*) src, mask, g, out, (uint32_t) count, density,
+                velocityScale, gx, gy, gz, rhoOut, uOut);
     })
 }
 

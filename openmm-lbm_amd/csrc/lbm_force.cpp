@@ -60,6 +60,7 @@ extern "C" void lbm_force_defaults(lbm_force_settings* s) {
         s->face_density[f] = 0.0;
     }
     s->storage = LBM_STORAGE_FP32;
+    s->density_halo_exchange = s->velocity_halo_exchange = 0;
 }
 
 extern "C" int lbm_compute_lattice_parameters(const lbm_force_settings* s, const double box[3], double stepSize,
@@ -169,6 +170,8 @@ extern "C" int lbm_compute_lattice_parameters(const lbm_force_settings* s, const
         }
         require(s->storage >= LBM_STORAGE_FP32 && s->storage <= LBM_STORAGE_FP64, "LBMForce: unknown storage type");
         p.storage = s->storage;
+        p.density_halo_exchange = s->density_halo_exchange ? 1 : 0;     // LBMForceImpl.cpp:126-127
+        p.velocity_halo_exchange = s->velocity_halo_exchange ? 1 : 0;
         *out = p;
         lbm::setLastError("");
         return LBM_OK;

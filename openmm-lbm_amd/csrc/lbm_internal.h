@@ -129,8 +129,17 @@ void launchBoundaries(int storage, const void* src, void* dst, const BoundaryNod
         const BoundaryArgs& a, double* partials, hipStream_t stream);
 void launchGather(int storage, const void* src, double* out, const Geometry& g, hipStream_t stream);
 void launchScatter(int storage, void* dst, const double* in, const Geometry& g, hipStream_t stream);
-void launchFields(int storage, const void* src, const uint32_t* mask, const Geometry& g, double density, double velocityScale,
-        double gx, double gy, double gz, double* rhoOut, double* uOut, hipStream_t stream);
+/* The entries that launchFields writes: pad = 0, the lattice in node order; pad = 1, the lattice with a layer one node
+   thick around it (getFluidFields with halo), whose entries hold the node across the periodic boundary for the fields
+   whose halo is exchanged, NaN for the others and beyond the open faces. */
+struct FieldsLayout {
+    int pad = 0;
+    int densityHalo = 0, velocityHalo = 0;
+};
+uint64_t fieldsCount(const Geometry& g, const FieldsLayout& out);
+void launchFields(int storage, const void* src, const uint32_t* mask, const Geometry& g, const FieldsLayout& out,
+        double density, double velocityScale, double gx, double gy, double gz, double* rhoOut, double* uOut,
+        hipStream_t stream);
 void launchInitEquilibrium(int storage, void* dst, const uint32_t* mask, const Geometry& g, double ux, double uy,
         double uz, hipStream_t stream);
 void launchFill(int storage, void* dst, size_t count, hipStream_t stream);

@@ -43,6 +43,8 @@ class LBMParams(ctypes.Structure):
         ("face_velocity", _c_double3 * 6),
         ("face_density", ctypes.c_double * 6),
         ("storage", ctypes.c_int),
+        ("density_halo_exchange", ctypes.c_int),
+        ("velocity_halo_exchange", ctypes.c_int),
     ]
 
 
@@ -65,6 +67,8 @@ class LBMForceSettings(ctypes.Structure):
         ("face_velocity", _c_double3 * 6),
         ("face_density", ctypes.c_double * 6),
         ("storage", ctypes.c_int),
+        ("density_halo_exchange", ctypes.c_int),
+        ("velocity_halo_exchange", ctypes.c_int),
     ]
 
 
@@ -84,7 +88,7 @@ KERNEL_FUNCTIONS = {
     "lbm_begin_step_device": (ctypes.c_int, [_P, ctypes.c_int64, _P, _P]),
     "lbm_execute_device": (ctypes.c_int, [_P, ctypes.c_int64, ctypes.c_int, _P, _P, _P]),
     "lbm_copy_parameters": (ctypes.c_int, [_P, ctypes.POINTER(LBMParams)]),
-    "lbm_get_fluid_fields": (ctypes.c_int, [_P, _D, _D]),
+    "lbm_get_fluid_fields": (ctypes.c_int, [_P, _D, _D, ctypes.c_int]),
     "lbm_get_fluid_mach_number": (ctypes.c_int, [_P, _D]),
     "lbm_get_wall_force": (ctypes.c_int, [_P, _D]),
     "lbm_get_fluid_state": (ctypes.c_int, [_P, _D, ctypes.c_size_t]),

@@ -197,10 +197,14 @@ class Context:
         return forces
 
     # Fluid
-    def getFluidFields(self):
-        rho = np.zeros(self.numNodes)
-        u = np.zeros((self.numNodes, 3))
-        self._check(self.lib.lbm_get_fluid_fields(self.handle, _dptr(rho), _dptr(u)))
+    def getFluidFields(self, halo=False):
+        """Density and velocity of every node (CalcLBMForceKernel::getFluidFields); with halo, of the lattice with
+        a layer one node thick around it, (nx + 2)(ny + 2)(nz + 2) nodes."""
+        nx, ny, nz = self.force.getGridSize()
+        count = (nx + 2)*(ny + 2)*(nz + 2) if halo else self.numNodes
+        rho = np.zeros(count)
+        u = np.zeros((count, 3))
+        self._check(self.lib.lbm_get_fluid_fields(self.handle, _dptr(rho), _dptr(u), 1 if halo else 0))
         return rho, u
 
     def getFluidState(self):
@@ -257,6 +261,9 @@ class Context:
              "the set of coupled particles cannot be changed"),
             (sorted(force.getSolidNodes()) != self.solid, "the solid nodes cannot be changed"),
             (updated.wall_scheme != old.wall_scheme, "the wall scheme cannot be changed"),
+            ((updated.density_halo_exchange, updated.velocity_halo_exchange) !=
+             (old.density_halo_exchange, old.velocity_halo_exchange),
+             "the exchange of the halo cannot be switched on or off"),
             (list(updated.face_boundary) != list(old.face_boundary), "the boundary types of the faces cannot be changed"),
             (updated.drag_scheme != old.drag_scheme, "the drag scheme cannot be changed"),
             (updated.interpolation_stencil != old.interpolation_stencil, "the interpolation stencil cannot be changed"),

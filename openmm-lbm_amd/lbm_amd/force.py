@@ -1,7 +1,9 @@
 """LBMForce: the user-facing settings of the lattice Boltzmann force, with the names, enums, defaults and
 checks of the reference's LBMForce (openmmapi/include/LBMForce.h:41-691, openmmapi/src/LBMForce.cpp).
-The MPI-only settings (domain decomposition, halo exchange, particle-copy check) are not part of a
-one-GPU platform.  Units are OpenMM's: nm, ps, Da, K, kJ/mol."""
+The MPI-only settings (domain decomposition, particle-copy check) are kept for serialization; a decomposition other
+than one domain is refused.  The halo-exchange settings select the fields of the halo layer of getFluidFields(), which
+with one domain holds the lattice's own nodes across the periodic boundaries.  Units are OpenMM's: nm, ps, Da, K,
+kJ/mol."""
 import ctypes
 
 from . import _lib
@@ -32,8 +34,6 @@ class LBMForce:
         self._name = "LBMForce"
         self._decomposition = [1, 1, 1]
         self._copiesCheck = True
-        self._densityHalo = False
-        self._velocityHalo = False
 
     # Grid
     def getGridSize(self):
@@ -229,16 +229,27 @@ class LBMForce:
         self._copiesCheck = bool(check)
 
     def getDensityHaloExchange(self):
-        return self._densityHalo
+        return bool(self._s.density_halo_exchange)
 
     def setDensityHaloExchange(self, exchange):
-        self._densityHalo = bool(exchange)
+        self._s.density_halo_exchange = int(bool(exchange))
 
     def getVelocityHaloExchange(self):
-        return self._velocityHalo
+        return bool(self._s.velocity_halo_exchange)
 
     def setVelocityHaloExchange(self, exchange):
-        self._velocityHalo = bool(exchange)
+        self._s.velocity_halo_exchange = int(bool(exchange))
+
+    def getFluidFields(self, context, gather=False, halo=False):
+        """The density (Da/nm^3) and velocity (nm/ps) of the fluid at the nodes (LBMForce.h:517-536,
+        openmmlbm.i:449-460), as arrays: with one domain the whole lattice, node (i, j, k) at index i + nx*(j + ny*k);
+        gather changes nothing.  halo=True adds the layer one node thick around the lattice, node (i, j, k) for i from
+        -1 to nx and so on at index (i + 1) + (nx + 2)*((j + 1) + (ny + 2)*(k + 1)), holding the fields across the
+        periodic boundaries whose halo is exchanged and NaN for the others and beyond the open faces."""
+        if gather and halo:
+            raise _lib.LBMError(_lib.LBM_ERROR_INVALID, "LBMForce: getFluidFields() cannot gather the fields and add "
+                                                        "the halo in the same call")
+        return context.getFluidFields(halo=halo)
 
     def computeLatticeParameters(self, box, stepSize, masses):
         """LBMForceImpl::computeLatticeParameters (LBMForceImpl.cpp:44-214): the lattice parameters of this force

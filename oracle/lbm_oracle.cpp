@@ -37,6 +37,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <limits>
 #include <random>
 #include <sstream>
 #include <stdexcept>
@@ -1213,21 +1214,71 @@ int lbm_execute_device(lbm_context*, int64_t, int, const double*, const double*,
 }
 
 int lbm_copy_parameters(lbm_context* ctx, const lbm_params* params) {
-    ORACLE_TRY({ int storage = ctx->kernel.p.storage; ctx->kernel.p = *params; ctx->kernel.p.storage = storage; })
+    // LBMForceImpl::updateParametersInContext (LBMForceImpl.cpp:388-414): the checks, then copyParametersToContext
+    // (ReferenceLBMKernels.cpp:1345-1347).
+    ORACLE_TRY({
+        const lbm_params& o = ctx->kernel.p;
+        const lbm_params& u = *params;
+        auto refuse = [](const char* what) {
+            throw OracleError(LBM_ERROR_INVALID, std::string("updateParametersInContext: ") + what);
+        };
+        if (u.nx != o.nx || u.ny != o.ny || u.nz != o.nz)
+            refuse("the grid size cannot be changed");
+        if (u.density != o.density || u.tau != o.tau)
+            refuse("the fluid density and viscosity cannot be changed");
+        if (u.wall_scheme != o.wall_scheme)
+            refuse("the wall scheme cannot be changed");
+        if (u.density_halo_exchange != o.density_halo_exchange || u.velocity_halo_exchange != o.velocity_halo_exchange)
+            refuse("the exchange of the halo cannot be switched on or off");
+        for (int f = 0; f < 6; f++)
+            if (u.face_boundary[f] != o.face_boundary[f])
+                refuse("the boundary types of the faces cannot be changed");
+        if (u.drag_scheme != o.drag_scheme)
+            refuse("the drag scheme cannot be changed");
+        if (u.interpolation_stencil != o.interpolation_stencil)
+            refuse("the interpolation stencil cannot be changed");
+        if ((u.fluid_fluctuations != 0) != (o.fluid_fluctuations != 0))
+            refuse("the fluid fluctuations cannot be switched on or off");
+        int storage = ctx->kernel.p.storage;
+        ctx->kernel.p = *params;
+        ctx->kernel.p.storage = storage;
+    })
 }
 
-int lbm_get_fluid_fields(lbm_context* ctx, double* density, double* velocity) {
+int lbm_get_fluid_fields(lbm_context* ctx, double* density, double* velocity, int halo) {
+    // ReferenceLBMKernels.cpp:1403-1436 with the nodes of LBMDecomposition::getDomainNodes (LBMDecomposition.cpp:
+    // 183-204) for one domain: with halo, the lattice padded by one node, a padded node wrapped onto the lattice, or
+    // -1 beyond an open face; outside the lattice a field is valid only if its halo is exchanged.
     ORACLE_TRY({
         OracleKernel& k = ctx->kernel;
-        for (int node = 0; node < k.numNodes; node++) {
-            double r, u[3];
-            k.nodeFields(node, r, u);
-            if (density)
-                density[node] = r*k.p.density;
-            if (velocity)
-                for (int a = 0; a < 3; a++)
-                    velocity[3*(size_t) node + a] = u[a]*(k.p.dx/k.p.dt);
-        }
+        const double nan = std::numeric_limits<double>::quiet_NaN();
+        int pad = halo ? 1 : 0;
+        int n[3] = {k.p.nx, k.p.ny, k.p.nz};
+        size_t l = 0;
+        for (int kk = -pad; kk < n[2] + pad; kk++)
+            for (int jj = -pad; jj < n[1] + pad; jj++)
+                for (int ii = -pad; ii < n[0] + pad; ii++, l++) {
+                    int c[3] = {ii, jj, kk};
+                    bool inside = true, beyondFace = false;
+                    for (int a = 0; a < 3; a++) {
+                        inside = inside && c[a] >= 0 && c[a] < n[a];
+                        if (c[a] < 0 || c[a] >= n[a]) {
+                            beyondFace = beyondFace || k.openAxis(a);
+                            c[a] = (c[a] + n[a])%n[a];
+                        }
+                    }
+                    int node = beyondFace ? -1 : c[0] + n[0]*(c[1] + n[1]*c[2]);
+                    bool densityValid = node >= 0 && (inside || k.p.density_halo_exchange);
+                    bool velocityValid = node >= 0 && (inside || k.p.velocity_halo_exchange);
+                    double r = nan, u[3] = {nan, nan, nan};
+                    if (densityValid || velocityValid)
+                        k.nodeFields(node, r, u);
+                    if (density)
+                        density[l] = densityValid ? r*k.p.density : nan;
+                    if (velocity)
+                        for (int a = 0; a < 3; a++)
+                            velocity[3*l + a] = velocityValid ? u[a]*(k.p.dx/k.p.dt) : nan;
+                }
     })
 }
 
