@@ -385,8 +385,12 @@ def main():
     elapsed = time.perf_counter() - t0
     total_ms = ctypes.c_double()
     count = ctypes.c_int()
+    launches = (ctypes.c_float*args.steps)()
+    _lib.check(lib, lib.lbm_profile_fused_launches(ctx.handle, launches, args.steps, ctypes.byref(count)))
+    launch_ms = sorted(launches[:min(count.value, args.steps)])
     _lib.check(lib, lib.lbm_profile_fused(ctx.handle, 0, ctypes.byref(total_ms), ctypes.byref(count)))
-    print(f"bench: {args.steps} steps in {elapsed:.3f} s", file=sys.stderr, flush=True)
+    print(f"bench: {args.steps} steps in {elapsed:.3f} s; fused pass per launch (ms): "
+          f"{' '.join(f'{t:.4f}' for t in launches[:min(count.value, args.steps)])}", file=sys.stderr, flush=True)
 
     n = w.num_nodes
     steps_per_s = args.steps/elapsed
@@ -421,7 +425,9 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved/HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel": "fusedStep", "kernel_ms": round(fused_ms, 4), "bytes_per_node": bpn,
-                     "algorithmic_bytes_per_launch": round(bpn*n)},
+                     "algorithmic_bytes_per_launch": round(bpn*n),
+                     "kernel_ms_min_median_max": [round(launch_ms[0], 4), round(launch_ms[len(launch_ms)//2], 4),
+                                                  round(launch_ms[-1], 4)] if launch_ms else None},
     }
     if not args.no_cpu_baseline:
         print("bench: timing the CPU oracle baseline", file=sys.stderr, flush=True)

@@ -235,6 +235,13 @@ int lbm_synchronize(lbm_context* ctx);
  */
 int lbm_profile_fused(lbm_context* ctx, int enable, double* total_ms, int* count);
 
+/*
+ * Diagnostics with no counterpart in the reference: the duration (ms) of each fused pass recorded since recording
+ * was switched on, oldest first, without clearing them.  Writes at most capacity values and the number recorded to
+ * *count.  The oracle library does not export it.
+ */
+int lbm_profile_fused_launches(lbm_context* ctx, float* ms, int capacity, int* count);
+
 #ifdef __cplusplus
 }
 #endif

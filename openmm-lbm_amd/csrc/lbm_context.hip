@@ -131,6 +131,7 @@ struct lbm_context {
     int numParticles = 0, numSystem = 0, numSlots = 0;
     bool dense = false;                   // denseCoupling(): index order, component-plane reactions
     bool wide = false;                    // 64-bit element indices in the fused pass (populations above 4 GiB)
+    int fusedPath = FUSED_DIRECT;         // the fused-pass kernel (FusedPath)
     int* particles = nullptr;
     double* mass = nullptr;               // lattice units m/m_c
     std::vector<int> hostParticles;
@@ -374,7 +375,7 @@ void advanceFluid(lbm_context* c, const double* dPos, const double* dVel, const 
         HIP_CHECK(hipEventCreate(&ev.second));
         HIP_CHECK(hipEventRecord(ev.first, c->stream));
     }
-    launchFused(c->p.storage, c->wide, c->mask != nullptr, particles, c->fluct(), c->popA, c->popB, c->mask,
+    launchFused(c->p.storage, c->fusedPath, c->mask != nullptr, particles, c->fluct(), c->popA, c->popB, c->mask,
             c->reactionFixed, c->touched, c->ucm, a, c->stream);
     if (c->profile) {
         HIP_CHECK(hipEventRecord(ev.second, c->stream));
@@ -521,6 +522,11 @@ void create(const lbm_params* params, const int* solid, int numSolid, const int*
     // nodes with FP32 storage), and by 64-bit indices beyond.  LBM_AMD_WIDE_ADDRESSING=1 takes the 64-bit path always.
     const char* wideEnv = getenv("LBM_AMD_WIDE_ADDRESSING");
     c->wide = NQ*n*c->elem > (1ull << 32) || (wideEnv && wideEnv[0] == '1');
+    // The LDS-staged pass (fusedRows) where the row length allows it; LBM_AMD_FUSED_ROWS=0 or 1 forces the choice
+    // (where it fits).
+    const char* rowsEnv = getenv("LBM_AMD_FUSED_ROWS");
+    bool rows = rowsEnv ? rowsEnv[0] == '1' : false;
+    c->fusedPath = c->wide ? FUSED_WIDE : (rows && fusedRowsFit((uint32_t) p.nx, false) ? FUSED_ROWS : FUSED_DIRECT);
     c->seed = (uint32_t) p.random_number_seed;
     if (c->seed == 0)
         c->seed = (uint32_t) std::chrono::high_resolution_clock::now().time_since_epoch().count() | 1u;
@@ -1004,6 +1010,18 @@ int lbm_profile_fused(lbm_context* c, int enable, double* totalMs, int* count) {
             *count = (int) c->events.size();
         clearEvents(c);
         c->profile = enable != 0;
+    })
+}
+
+int lbm_profile_fused_launches(lbm_context* c, float* ms, int capacity, int* count) {
+    LBM_CHECK_CTX(c);
+    LBM_TRY({
+        HIP_CHECK(hipStreamSynchronize(c->stream));
+        int n = (int) c->events.size();
+        for (int i = 0; i < n && i < capacity; i++)
+            HIP_CHECK(hipEventElapsedTime(&ms[i], c->events[i].first, c->events[i].second));
+        if (count)
+            *count = n;
     })
 }
 

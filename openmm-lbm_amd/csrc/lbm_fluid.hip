@@ -33,6 +33,110 @@ __global__ void __launch_bounds__(THREADS) fusedStep(const S* __restrict__ src, 
     fusedNode<S, SOLIDS, REACTION, FLUCT, WIDE>(src, dst, mask, reactionFixed, touched, ucm, a, node);
 }
 
+/* The storage value v as the load type (FP32 for FP16/BF16). */
+template <typename S>
+__device__ __forceinline__ typename StorageTraits<S>::Real asReal(S v) {
+    if constexpr (std::is_same<S, __half>::value)
+        return (float) __builtin_bit_cast(_Float16, v);
+    else if constexpr (std::is_same<S, __hip_bfloat16>::value)
+        return __uint_as_float(((uint32_t) __builtin_bit_cast(uint16_t, v)) << 16);
+    else
+        return v;
+}
+
+/* c_qy + 1 and c_qz + 1 of population q, two bits each, for a q known only at run time. */
+constexpr uint64_t packedDirections(const int* c) {
+    uint64_t bits = 0;
+    for (int q = 0; q < NQ; q++)
+        bits |= (uint64_t) (c[q] + 1) << (2*q);
+    return bits;
+}
+constexpr uint64_t PACKED_CY = packedDirections(CY);
+constexpr uint64_t PACKED_CZ = packedDirections(CZ);
+
+__device__ __forceinline__ int directionOf(uint64_t packed, int q) {
+    return (int) ((packed >> (2*q)) & 3u) - 1;
+}
+
+/*
+ * The fused pass with the pulls staged through LDS.  A work-group of ROW_THREADS threads covers as many consecutive
+ * nodes, in pieces of SEG nodes along x: one piece of a row when SEG = ROW_THREADS (nx a multiple of it), else
+ * ROW_THREADS/SEG whole rows (SEG = nx).  For each population q and piece, the source run [x0 - V, x0 + SEG + V) of the
+ * row (j - c_qy, k - c_qz) is loaded in aligned 16-byte chunks of V values (wrapped along x and across the periodic
+ * faces) into LDS, and each thread takes its value at x - c_qx from there: the global loads are 16 bytes per lane and
+ * aligned, NQ*(SEG/V + 2) chunks per piece instead of NQ single-value loads per node.  Same arithmetic as fusedStep, so
+ * bit-identical results.  Requires arrays of at most 4 GiB (32-bit offsets) and nx a multiple of V.
+ */
+template <typename S, bool SOLIDS, bool REACTION, bool FLUCT, int SEG>
+__global__ void __launch_bounds__(ROW_THREADS) __attribute__((amdgpu_waves_per_eu(sizeof(S) == 8 ? 1 : 8)))
+fusedRows(const S* __restrict__ src, S* __restrict__ dst,
+        const uint32_t* __restrict__ mask, unsigned long long* __restrict__ reactionFixed, uint32_t* __restrict__ touched,
+        const double* __restrict__ ucm, StepArgs a) {
+    typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+    constexpr int T = ROW_THREADS;
+    constexpr int V = 16/(int) sizeof(S);
+    constexpr int PIECES = T/SEG;
+    constexpr int CH = SEG/V + 2;
+    constexpr int TOTAL = NQ*PIECES*CH;
+    constexpr int ITER = (TOTAL + T - 1)/T;
+    static_assert(SEG % V == 0 && T % SEG == 0, "pieces of whole chunks");
+    __shared__ u32x4 stage[TOTAL];
+    const uint32_t base = blockIdx.x*T;
+    const int nx = (int) a.g.nx, ny = (int) a.g.ny, nz = (int) a.g.nz;
+    const auto* chunks = asGlobal(reinterpret_cast<const u32x4*>(src));
+    u32x4 v[ITER];
+#pragma unroll
+    for (int it = 0; it < ITER; it++) {
+        v[it] = u32x4{0u, 0u, 0u, 0u};
+        const int c = (int) threadIdx.x + it*T;
+        if (c < TOTAL) {
+            const int q = c/(PIECES*CH);
+            const int rem = c - q*(PIECES*CH);
+            const int p = rem/CH;
+            const int ch = rem - p*CH;
+            const uint32_t first = base + (uint32_t) (p*SEG);
+            if (first < a.g.n) {
+                uint32_t i0, j, k;
+                nodeCoords(a.g, first, i0, j, k);
+                int js = (int) j - directionOf(PACKED_CY, q);
+                int ks = (int) k - directionOf(PACKED_CZ, q);
+                js = js < 0 ? js + ny : (js >= ny ? js - ny : js);
+                ks = ks < 0 ? ks + nz : (ks >= nz ? ks - nz : ks);
+                int xs = (int) i0 - V + ch*V;
+                xs = xs < 0 ? xs + nx : (xs >= nx ? xs - nx : xs);
+                const uint32_t elem = (((uint32_t) ks*(uint32_t) ny + (uint32_t) js)*NQ + (uint32_t) q)*(uint32_t) nx +
+                                      (uint32_t) xs;
+                v[it] = __builtin_nontemporal_load(chunks + elem/V);
+            }
+        }
+    }
+#pragma unroll
+    for (int it = 0; it < ITER; it++) {
+        const int c = (int) threadIdx.x + it*T;
+        if (c < TOTAL)
+            stage[c] = v[it];
+    }
+    __syncthreads();
+    const uint32_t node = base + threadIdx.x;
+    if (node >= a.g.n)
+        return;
+    const uint32_t touchedWord = takeTouchedWord<REACTION>(touched, node);
+    if constexpr (SOLIDS)
+        if (isSolid(mask, node))
+            return;
+    const int p = (int) threadIdx.x/SEG, t = (int) threadIdx.x - p*SEG;
+    const S* staged = reinterpret_cast<const S*>(stage);
+    typename StorageTraits<S>::Real fr[NQ];
+    staticFor<NQ>([&](auto I) {
+        constexpr int q = decltype(I)::value;
+        fr[q] = asReal(staged[(q*PIECES + p)*CH*V + t + V - CX[q]]);
+    });
+    uint32_t i, j, k;
+    nodeCoords(a.g, node, i, j, k);
+    collideNode<S, REACTION, FLUCT, false>(fr, dst, reactionFixed, touchedWord, ucm, a, node,
+            elementOf<uint32_t>(node, i));
+}
+
 /*
  * The same pass over every replica of a batch in one launch: work-group b covers THREADS nodes of replica
  * tileReplica[b].  The step index comes from device memory, so one captured launch serves every step.
@@ -564,27 +668,52 @@ static void launchFusedT(bool fluct, const void* src, void* dst, const uint32_t*
                 touched, ucm, a);
 }
 
-template <typename S, bool WIDE>
-static void launchFusedW(bool solids, bool reaction, bool fluct, const void* src, void* dst, const uint32_t* mask,
+template <typename S, bool SOL, bool REA, int SEG>
+static void launchRowsT(bool fluct, const void* src, void* dst, const uint32_t* mask, unsigned long long* reactionFixed,
+        uint32_t* touched, const double* ucm, const StepArgs& a, hipStream_t stream) {
+    unsigned grid = (unsigned) ((a.g.n + ROW_THREADS - 1)/ROW_THREADS);
+    if (fluct)
+        fusedRows<S, SOL, REA, true, SEG><<<grid, ROW_THREADS, 0, stream>>>((const S*) src, (S*) dst, mask,
+                reactionFixed, touched, ucm, a);
+    else
+        fusedRows<S, SOL, REA, false, SEG><<<grid, ROW_THREADS, 0, stream>>>((const S*) src, (S*) dst, mask,
+                reactionFixed, touched, ucm, a);
+}
+
+template <typename S, bool SOL, bool REA>
+static void launchPathT(int path, bool fluct, const void* src, void* dst, const uint32_t* mask,
         unsigned long long* reactionFixed, uint32_t* touched, const double* ucm, const StepArgs& a, hipStream_t stream) {
-    if (solids) {
-        if (reaction) launchFusedT<S, true, true, WIDE>(fluct, src, dst, mask, reactionFixed, touched, ucm, a, stream);
-        else launchFusedT<S, true, false, WIDE>(fluct, src, dst, mask, reactionFixed, touched, ucm, a, stream);
-    }
-    else {
-        if (reaction) launchFusedT<S, false, true, WIDE>(fluct, src, dst, mask, reactionFixed, touched, ucm, a, stream);
-        else launchFusedT<S, false, false, WIDE>(fluct, src, dst, mask, reactionFixed, touched, ucm, a, stream);
+    switch (path) {
+    case FUSED_WIDE:
+        launchFusedT<S, SOL, REA, true>(fluct, src, dst, mask, reactionFixed, touched, ucm, a, stream);
+        break;
+    case FUSED_ROWS:
+        if (a.g.nx == 128)
+            launchRowsT<S, SOL, REA, 128>(fluct, src, dst, mask, reactionFixed, touched, ucm, a, stream);
+        else
+            launchRowsT<S, SOL, REA, ROW_THREADS>(fluct, src, dst, mask, reactionFixed, touched, ucm, a, stream);
+        break;
+    default:
+        launchFusedT<S, SOL, REA, false>(fluct, src, dst, mask, reactionFixed, touched, ucm, a, stream);
     }
 }
 
-void launchFused(int storage, bool wide, bool solids, bool reaction, bool fluct, const void* src, void* dst,
+bool fusedRowsFit(uint32_t nx, bool wide) {
+    return !wide && (nx % ROW_THREADS == 0 || nx == 128);
+}
+
+void launchFused(int storage, int path, bool solids, bool reaction, bool fluct, const void* src, void* dst,
         const uint32_t* mask, unsigned long long* reactionFixed, uint32_t* touched, const double* ucm, const StepArgs& a,
         hipStream_t stream) {
     LBM_DISPATCH_STORAGE(storage, S, {
-        if (wide)
-            launchFusedW<S, true>(solids, reaction, fluct, src, dst, mask, reactionFixed, touched, ucm, a, stream);
-        else
-            launchFusedW<S, false>(solids, reaction, fluct, src, dst, mask, reactionFixed, touched, ucm, a, stream);
+        if (solids) {
+            if (reaction) launchPathT<S, true, true>(path, fluct, src, dst, mask, reactionFixed, touched, ucm, a, stream);
+            else launchPathT<S, true, false>(path, fluct, src, dst, mask, reactionFixed, touched, ucm, a, stream);
+        }
+        else {
+            if (reaction) launchPathT<S, false, true>(path, fluct, src, dst, mask, reactionFixed, touched, ucm, a, stream);
+            else launchPathT<S, false, false>(path, fluct, src, dst, mask, reactionFixed, touched, ucm, a, stream);
+        }
     })
 }
 

@@ -11,33 +11,31 @@
 namespace lbm {
 
 /*
- * One node: pull C_q(x - c_q) from src, form the moments, subtract the removal velocity, add the body force and the
- * particle reaction (consumed from the fixed-point accumulators where the touched bit is set), collide, add the
- * fluctuating part, store C_q(x) to dst at x.  Returns without writing for solid nodes.
+ * The touched-mask word of a node's wavefront (32 nodes): every lane reads it, then its first lane (always inside the
+ * lattice, since words are 32-aligned) clears it for the next step.
  */
-template <typename S, bool SOLIDS, bool REACTION, bool FLUCT, bool WIDE = false>
-__device__ __forceinline__ void fusedNode(const S* __restrict__ src, S* __restrict__ dst, const uint32_t* __restrict__ mask,
-        unsigned long long* __restrict__ reactionFixed, uint32_t* __restrict__ touched, const double* __restrict__ ucm,
-        const StepArgs& a, uint32_t node) {
-    using R = typename StorageTraits<S>::Real;
-    using C = typename StorageTraits<S>::Compute;
+template <bool REACTION>
+__device__ __forceinline__ uint32_t takeTouchedWord(uint32_t* __restrict__ touched, uint32_t node) {
     uint32_t touchedWord = 0;
     if constexpr (REACTION) {
-        // The word of the touched mask covers 32 nodes of this wave: every lane reads it, then its first lane
-        // (always inside the lattice, since words are 32-aligned) clears it for the next step.
         touchedWord = *asGlobal(touched + (node >> 5));
         if ((node & 31) == 0 && touchedWord != 0)
             *asGlobal(touched + (node >> 5)) = 0;
     }
-    if constexpr (SOLIDS)
-        if (isSolid(mask, node))
-            return;
-    uint32_t i, j, k;
-    nodeCoords(a.g, node, i, j, k);
-    using Offset = typename std::conditional<WIDE, long long, int>::type;
-    const ElementIndex<WIDE> e = elementOf<ElementIndex<WIDE>>(node, i);
-    R fr[NQ];
-    pullNodeStream<S, WIDE>(src, a.g.nx, e, elementNeighbours<Offset>(a.g, i, j, k), fr);
+    return touchedWord;
+}
+
+/*
+ * One fluid node from its pulled pre-collision deviations fr[q] = C_q(x - c_q): form the moments, subtract the removal
+ * velocity, add the body force and the particle reaction (consumed from the fixed-point accumulators where the
+ * touched bit is set), collide, add the fluctuating part, store C_q(x) to dst at element e of x.
+ */
+template <typename S, bool REACTION, bool FLUCT, bool WIDE = false>
+__device__ __forceinline__ void collideNode(const typename StorageTraits<S>::Real fr[NQ], S* __restrict__ dst,
+        unsigned long long* __restrict__ reactionFixed, uint32_t touchedWord, const double* __restrict__ ucm,
+        const StepArgs& a, uint32_t node, ElementIndex<WIDE> e) {
+    using R = typename StorageTraits<S>::Real;
+    using C = typename StorageTraits<S>::Compute;
     C f[NQ];
 #pragma unroll
     for (int q = 0; q < NQ; q++)
@@ -113,6 +111,26 @@ __device__ __forceinline__ void fusedNode(const S* __restrict__ src, S* __restri
         else
             storePopStream(atByte(dst + (size_t) q*a.g.nx, e*(uint32_t) sizeof(S)), v, r);
     }
+}
+
+/*
+ * One node: pull C_q(x - c_q) from src and collide (collideNode).  Returns without writing for solid nodes.
+ */
+template <typename S, bool SOLIDS, bool REACTION, bool FLUCT, bool WIDE = false>
+__device__ __forceinline__ void fusedNode(const S* __restrict__ src, S* __restrict__ dst, const uint32_t* __restrict__ mask,
+        unsigned long long* __restrict__ reactionFixed, uint32_t* __restrict__ touched, const double* __restrict__ ucm,
+        const StepArgs& a, uint32_t node) {
+    uint32_t touchedWord = takeTouchedWord<REACTION>(touched, node);
+    if constexpr (SOLIDS)
+        if (isSolid(mask, node))
+            return;
+    uint32_t i, j, k;
+    nodeCoords(a.g, node, i, j, k);
+    using Offset = typename std::conditional<WIDE, long long, int>::type;
+    const ElementIndex<WIDE> e = elementOf<ElementIndex<WIDE>>(node, i);
+    typename StorageTraits<S>::Real fr[NQ];
+    pullNodeStream<S, WIDE>(src, a.g.nx, e, elementNeighbours<Offset>(a.g, i, j, k), fr);
+    collideNode<S, REACTION, FLUCT, WIDE>(fr, dst, reactionFixed, touchedWord, ucm, a, node, e);
 }
 
 }  // namespace lbm

@@ -14,6 +14,7 @@ namespace lbm {
 constexpr int BLOCK = 256;
 constexpr int FUSED_THREADS = 512;         // work-group size of the fused pass (measured best on MI355X)
 constexpr int FUSED_THREADS_REDUCED = 256; // the same with FP16/BF16 storage
+constexpr int ROW_THREADS = 256;           // work-group size of the LDS-staged fused pass (fusedRows)
 constexpr int REDUCE_BLOCKS = 1024;       // fixed grid of the FP64 reductions: fixed summation order
 // Fixed-point scale of the particle reactions, 2^50: each contribution is rounded to 8.9e-16 lattice units, and a node
 // holds up to +-8192 lattice units per step (a bead of 6,000 cell masses at gamma dt = 2 and a velocity difference of
@@ -117,7 +118,10 @@ void launchCoupleBatch(int stencil, const ReplicaDesc* table, const uint16_t* pa
 int coupleLanes(int stencil);
 void launchVerletBatch(const ReplicaDesc* table, const uint16_t* particleReplica, int numParticles, double* positions,
         double* velocities, const double* forces, const double* invMass, int64_t* stepCounter, hipStream_t stream);
-void launchFused(int storage, bool wide, bool solids, bool reaction, bool fluct, const void* src, void* dst,
+/* Which fused-pass kernel a context runs: fusedStep with 32-bit or 64-bit offsets, or fusedRows (LDS-staged pulls). */
+enum FusedPath { FUSED_DIRECT = 0, FUSED_WIDE = 1, FUSED_ROWS = 2 };
+bool fusedRowsFit(uint32_t nx, bool wide);
+void launchFused(int storage, int path, bool solids, bool reaction, bool fluct, const void* src, void* dst,
         const uint32_t* mask, unsigned long long* reactionFixed, uint32_t* touched, const double* ucm, const StepArgs& a,
         hipStream_t stream);
 void launchMomentReduce(int storage, bool solids, const void* src, const uint32_t* mask, const Geometry& g,

@@ -107,6 +107,8 @@ PRODUCT_FUNCTIONS = {
     "lbm_compute_lattice_parameters": (ctypes.c_int, [ctypes.POINTER(LBMForceSettings), _D, ctypes.c_double, _I,
                                                       ctypes.c_int, _I, ctypes.c_int, _D, ctypes.c_int,
                                                       ctypes.POINTER(LBMParams)]),
+    "lbm_profile_fused_launches": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_float), ctypes.c_int,
+                                                  ctypes.POINTER(ctypes.c_int)]),
 }
 
 

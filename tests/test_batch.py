@@ -137,3 +137,57 @@ def test_batch_rejects_what_it_does_not_support(change, message):
     change(replicas[1].force)
     with pytest.raises(LBMError, match=message):
         Batch(replicas)
+
+
+def oracle_context(replica):
+    """One replica as a Context on the FP64 oracle: the same force, box, step and particles; the coupling force is
+    the only force, integrated with the same leapfrog Verlet."""
+    import oracle_lib
+    system = System(replica.box)
+    for m in replica.masses:
+        system.addParticle(m)
+    ctx = Context(system, VerletIntegrator(replica.stepSize), replica.force, library=oracle_lib.load_oracle())
+    ctx.setPositions(replica.positions)
+    ctx.setVelocities(replica.velocities)
+    return ctx
+
+
+@pytest.mark.gpu
+def test_config4_full_size():
+    """The batching form at its full size (workloads.config4: 64 periodic replicas, edges {32, 48, 64, 96}, tau in
+    [0.55, 1], 50-2000 trilinear beads, fluctuations on with per-replica seeds and kT), stepped together.  The two
+    smallest replicas run in lockstep with the FP64 oracle for 10 steps: density, velocity and bead positions agree
+    to 1e-4 relative (the fluctuations use the same Philox stream; the normals come from FP32 hardware
+    transcendentals on the GPU and FP64 ones in the oracle).  After 300 steps every replica has kept its mass to
+    1e-6 and its fluid temperature, the velocity variance of a node, is kT/(rho dx^3) within 5%."""
+    from lbm_amd import workloads
+    replicas, rows, _ = workloads.config4()
+    batch = Batch(replicas)
+    masses0 = []
+    for r, row in enumerate(rows):
+        rho, _ = batch.getFluidFields(r)
+        masses0.append(rho.sum())
+    order = np.argsort([row["edge"]*10000 + row["beads"] for row in rows])[:2]
+    oracles = {int(r): oracle_context(replicas[r]) for r in order}
+    batch.step(10)
+    particles = batch.getParticles()
+    for r, ctx in oracles.items():
+        ctx.step(10)
+        rho0, u0 = ctx.getFluidFields()
+        rho1, u1 = batch.getFluidFields(r)
+        pos1 = particles[r][0]
+        d_rho = np.abs(rho1 - rho0).max()/np.abs(rho0).max()
+        d_u = np.abs(u1 - u0).max()/np.abs(u0).max()
+        d_x = np.abs(pos1 - ctx.positions[:len(pos1)]).max()
+        print(f"replica {r} edge {rows[r]['edge']} beads {rows[r]['beads']}: density {d_rho:.1e} velocity {d_u:.1e} "
+              f"positions {d_x:.1e} nm", flush=True)
+        assert d_rho < 1e-4 and d_u < 1e-4 and d_x < 1e-4
+    batch.step(290)
+    for r, row in enumerate(rows):
+        rho, u = batch.getFluidFields(r)
+        drift = abs(rho.sum() - masses0[r])/masses0[r]
+        expected = row["kT"]/602.214
+        measured = np.var(u - u.mean(axis=0), axis=0).mean()
+        assert drift < 1e-6, (r, drift)
+        assert abs(measured/expected - 1) < 0.05, (r, measured/expected)
+    print("all 64 replicas: mass kept to 1e-6 and fluid temperature within 5% of kT", flush=True)

@@ -224,3 +224,93 @@ def test_training_learns_the_keys_correction():
           f"nearest only {err_nearest:.3e} ({err_net/err_nearest:.3f})")
     assert losses[-1] < 0.5*losses[0]
     assert err_net < 0.6*err_nearest
+
+
+def test_config5_full_size_keys_run():
+    """The learned form's workload at its full size (workloads.config5: 128^3, 2,000 harmonic chains of 50 beads,
+    fluctuating fluid and particle noise at 300 K, Keys coupling), 300 device-resident steps: the bonds keep their
+    length (mean within 2% of r0 = 0.38 nm, none beyond 2 r0), the beads are at the temperature of the explicit drag's
+    leapfrog half-step velocities, T/(1 - gamma dt/2), within 5%, and the fluid's node velocity variance is kT/(rho
+    dx^3) within 5%.  Then the closure (a network with every particle out of distribution) takes the Keys coupling
+    for all of them: 100,000 fallbacks per step and the same state bit for bit as the Keys run itself."""
+    from lbm_amd import workloads
+    w = workloads.config5()
+    runs = []
+    for with_closure in (False, True):
+        c = w.build()
+        dp = w.device_particles(c)
+        net = None
+        if with_closure:
+            net = MLPClosure(seed=1)
+            net.setNormalization(u_scale=1.0, v_scale=1.0, out_scale=1.0, u_bound=0.0, v_bound=0.0)
+            net.attach(c)
+        dp.step(300)
+        pos, vel = dp.pull()
+        rho, u = c.getFluidFields()
+        counts = stats(c) if with_closure else None
+        runs.append((pos, vel, rho, u, counts))
+        del dp, c
+    pos, vel, rho, u, _ = runs[0]
+    pairs_ = w.bonds[0].reshape(-1, 2)
+    box = np.array(w.grid, dtype=np.float64)*w.dx
+    d = pos[pairs_[:, 1]] - pos[pairs_[:, 0]]
+    d -= box*np.round(d/box)
+    bond = np.linalg.norm(d, axis=1)
+    kT = BOLTZ*300.0
+    T_beads = 100.0*np.mean(vel**2)/kT
+    T_expected = 1.0/(1.0 - 10.0*w.dt/2)
+    fluid = np.var(u - u.mean(axis=0), axis=0).mean()/(kT/602.214)
+    print(f"bond length mean {bond.mean():.4f} max {bond.max():.4f} nm; bead temperature {T_beads:.4f} kT "
+          f"(expected {T_expected:.4f}); fluid temperature {fluid:.4f} kT", flush=True)
+    assert abs(bond.mean()/0.38 - 1) < 0.02 and bond.max() < 0.76
+    assert abs(T_beads/T_expected - 1) < 0.05
+    assert abs(fluid - 1) < 0.05
+    c_pos, c_vel, c_rho, c_u, counts = runs[1]
+    assert counts == (0, 300*len(w.masses))
+    assert np.array_equal(c_pos, pos) and np.array_equal(c_vel, vel)
+    assert np.array_equal(c_rho, rho) and np.array_equal(c_u, u)
+
+
+def test_closure_fallback_against_the_oracle_keys():
+    """Chains on a small lattice (16^3, 12 chains of 10 beads, fluctuating fluid and particles at 300 K): the closure
+    with every particle out of distribution couples through its Keys fallback, in lockstep with the FP64 oracle's Keys
+    coupling on the same Philox streams, for 20 steps: density, velocity, bead forces and positions within 1e-4."""
+    from lbm_amd import workloads
+    from parity import rel_err
+    import oracle_lib
+    w = workloads.config5(n=16, num_chains=12, chain_length=10, seed=99)
+    bonds = w.bonds
+
+    def bond_forces(pos):
+        pairs_ = bonds[0].reshape(-1, 2)
+        box = np.array(w.grid, dtype=np.float64)*w.dx
+        d = pos[pairs_[:, 1]] - pos[pairs_[:, 0]]
+        d -= box*np.round(d/box)
+        r = np.linalg.norm(d, axis=1)
+        f = (bonds[1]*(r - bonds[2])/r)[:, None]*d
+        out = np.zeros_like(pos)
+        np.add.at(out, pairs_[:, 0], f)
+        np.add.at(out, pairs_[:, 1], -f)
+        return out
+    w.bonds = None
+    ref = w.build(oracle_lib.load_oracle())
+    gpu = w.build()
+    for c in (ref, gpu):
+        c.extraForces.append(bond_forces)
+    net = MLPClosure(seed=1)
+    net.setNormalization(u_scale=1.0, v_scale=1.0, out_scale=1.0, u_bound=0.0, v_bound=0.0)
+    net.attach(gpu)
+    worst = {"density": 0.0, "velocity": 0.0, "force": 0.0, "position": 0.0}
+    for step in range(20):
+        ref.step(1)
+        gpu.step(1)
+        r0, u0 = ref.getFluidFields()
+        r1, u1 = gpu.getFluidFields()
+        worst["density"] = max(worst["density"], rel_err(r1, r0))
+        worst["velocity"] = max(worst["velocity"], rel_err(u1, u0))
+        worst["force"] = max(worst["force"], rel_err(gpu.getForces(), ref.getForces()))
+        worst["position"] = max(worst["position"], rel_err(gpu.positions, ref.positions, scale=w.dx))
+    network, fallback = stats(gpu)
+    print("closure fallback vs oracle Keys:", worst, "couplings", network, fallback)
+    assert network == 0 and fallback >= 20*len(w.masses)
+    assert max(worst.values()) < 1e-4

@@ -326,6 +326,42 @@ def test_wide_addressing_is_bit_identical(storage, monkeypatch):
     assert np.array_equal(runs[0][2], runs[1][2])
 
 
+@pytest.mark.parametrize("storage", [LBMForce.FP32, LBMForce.FP16, LBMForce.BF16, LBMForce.FP64],
+                         ids=["fp32", "fp16", "bf16", "fp64"])
+@pytest.mark.parametrize("grid", [(256, 3, 4), (128, 5, 3), (128, 3, 3)], ids=["rows256", "rows128", "rows128-tail"])
+def test_lds_staged_pass_is_bit_identical(storage, grid, monkeypatch):
+    """The fused pass with its pulls staged through LDS in aligned 16-byte chunks (LBM_AMD_FUSED_ROWS=1; rows of 128
+    nodes or a multiple of 256) and the pass with direct pulls (=0) give the same run bit for bit: walls along y and
+    z with bounce-back, fluctuations, a body force and Keys beads whose stencils wrap across the periodic x faces.
+    128 x 3 x 3 leaves a partial last work-group."""
+    nx, ny, nz = grid
+    s = Setup((nx, ny, nz), dx=1.0, dt=0.01, tau=0.7)
+    rng = np.random.default_rng(8)
+    pos = rng.uniform([0, 1, 1], [nx, ny - 1, nz - 1], size=(24, 3))
+    pos[:4, 0] = [0.2, nx - 0.3, 1.1, nx - 1.6]
+    s.add_particles(pos, mass=150.0, velocities=rng.normal(scale=0.5, size=(24, 3)))
+
+    def conf(f):
+        f.setSolidNodes([i + nx*ny*(nz - 1) for i in range(nx*ny)] + [i for i in range(nx)])
+        f.setWallScheme(LBMForce.BounceBack)
+        f.setBodyAcceleration((2e-5/1e-4, 0, 0))
+        f.setFluidFluctuations(True)
+        f.setTemperature(300.0)
+        f.setRandomNumberSeed(23)
+        f.setFriction(8.0)
+        f.setInterpolationStencil(LBMForce.Keys)
+    s.configure.append(conf)
+    runs = []
+    for rows in ("0", "1"):
+        monkeypatch.setenv("LBM_AMD_FUSED_ROWS", rows)
+        c = s.build(None, storage=storage)
+        c.step(9)
+        runs.append((c.getFluidState(), c.positions.copy(), c.getWallForce()))
+    assert np.array_equal(runs[0][0], runs[1][0])
+    assert np.array_equal(runs[0][1], runs[1][1])
+    assert np.array_equal(runs[0][2], runs[1][2])
+
+
 def test_lattice_beyond_4_gib():
     """A 400^3 lattice in FP32 (4.9 GB per population array, so the fused pass takes 64-bit indices): beads near the
     top of the array (element offsets above 2^32 bytes) exchange momentum with the fluid, and the total momentum of
