@@ -157,7 +157,7 @@ def test_config4_full_size():
     """The batching form at its full size (workloads.config4: 64 periodic replicas, edges {32, 48, 64, 96}, tau in
     [0.55, 1], 50-2000 trilinear beads, fluctuations on with per-replica seeds and kT), stepped together.  The two
     smallest replicas run in lockstep with the FP64 oracle for 10 steps: density, velocity and bead positions agree
-    to 1e-4 relative (the fluctuations use the same Philox stream; the normals come from FP32 hardware
+    to 1e-5 relative (measured 2.6e-9, 1.1e-6 and 1.6e-10 nm; the fluctuations use the same Philox stream; the normals come from FP32 hardware
     transcendentals on the GPU and FP64 ones in the oracle).  After 300 steps every replica has kept its mass to
     1e-6 and its fluid temperature, the velocity variance of a node, is kT/(rho dx^3) within 5%."""
     from lbm_amd import workloads
@@ -181,7 +181,7 @@ def test_config4_full_size():
         d_x = np.abs(pos1 - ctx.positions[:len(pos1)]).max()
         print(f"replica {r} edge {rows[r]['edge']} beads {rows[r]['beads']}: density {d_rho:.1e} velocity {d_u:.1e} "
               f"positions {d_x:.1e} nm", flush=True)
-        assert d_rho < 1e-4 and d_u < 1e-4 and d_x < 1e-4
+        assert d_rho < 1e-5 and d_u < 1e-5 and d_x < 1e-5
     batch.step(290)
     for r, row in enumerate(rows):
         rho, u = batch.getFluidFields(r)

@@ -229,8 +229,9 @@ def test_training_learns_the_keys_correction():
 def test_config5_full_size_keys_run():
     """The learned form's workload at its full size (workloads.config5: 128^3, 2,000 harmonic chains of 50 beads,
     fluctuating fluid and particle noise at 300 K, Keys coupling), 300 device-resident steps: the bonds keep their
-    length (mean within 2% of r0 = 0.38 nm, none beyond 2 r0), the beads are at the temperature of the explicit drag's
-    leapfrog half-step velocities, T/(1 - gamma dt/2), within 5%, and the fluid's node velocity variance is kT/(rho
+    length (mean within 1% of the thermal mean of a harmonic bond, <r> = int r^3 exp(-k (r - r0)^2/2kT) dr /
+    int r^2 exp(-k (r - r0)^2/2kT) dr = 0.3929 nm for r0 = 0.38 nm, none beyond 2 r0), the beads are at the temperature of the explicit drag's
+    leapfrog half-step velocities, T/(1 - gamma dt/2), within 2% (the explicit drag heats them by about 0.45%), and the fluid's node velocity variance is kT/(rho
     dx^3) within 5%.  Then the closure (a network with every particle out of distribution) takes the Keys coupling
     for all of them: 100,000 fallbacks per step and the same state bit for bit as the Keys run itself."""
     from lbm_amd import workloads
@@ -240,11 +241,14 @@ def test_config5_full_size_keys_run():
         c = w.build()
         dp = w.device_particles(c)
         net = None
+        # The first step starts from rest (every input zero, so in distribution even with zero bounds): the closure
+        # is attached after it.
+        dp.step(1)
         if with_closure:
             net = MLPClosure(seed=1)
             net.setNormalization(u_scale=1.0, v_scale=1.0, out_scale=1.0, u_bound=0.0, v_bound=0.0)
             net.attach(c)
-        dp.step(300)
+        dp.step(299)
         pos, vel = dp.pull()
         rho, u = c.getFluidFields()
         counts = stats(c) if with_closure else None
@@ -262,11 +266,14 @@ def test_config5_full_size_keys_run():
     fluid = np.var(u - u.mean(axis=0), axis=0).mean()/(kT/602.214)
     print(f"bond length mean {bond.mean():.4f} max {bond.max():.4f} nm; bead temperature {T_beads:.4f} kT "
           f"(expected {T_expected:.4f}); fluid temperature {fluid:.4f} kT", flush=True)
-    assert abs(bond.mean()/0.38 - 1) < 0.02 and bond.max() < 0.76
-    assert abs(T_beads/T_expected - 1) < 0.05
+    r = np.linspace(1e-4, 1.2, 200001)
+    weight = r**2*np.exp(-1000.0*(r - 0.38)**2/(2*kT))
+    thermal_mean = (r*weight).sum()/weight.sum()
+    assert abs(bond.mean()/thermal_mean - 1) < 0.01 and bond.max() < 0.76
+    assert abs(T_beads/T_expected - 1) < 0.02
     assert abs(fluid - 1) < 0.05
     c_pos, c_vel, c_rho, c_u, counts = runs[1]
-    assert counts == (0, 300*len(w.masses))
+    assert counts == (0, 299*len(w.masses))
     assert np.array_equal(c_pos, pos) and np.array_equal(c_vel, vel)
     assert np.array_equal(c_rho, rho) and np.array_equal(c_u, u)
 
@@ -274,7 +281,8 @@ def test_config5_full_size_keys_run():
 def test_closure_fallback_against_the_oracle_keys():
     """Chains on a small lattice (16^3, 12 chains of 10 beads, fluctuating fluid and particles at 300 K): the closure
     with every particle out of distribution couples through its Keys fallback, in lockstep with the FP64 oracle's Keys
-    coupling on the same Philox streams, for 20 steps: density, velocity, bead forces and positions within 1e-4."""
+    coupling on the same Philox streams, for 20 steps: density, velocity, bead forces and positions within 1e-5
+    (measured 1.1e-9, 3.1e-7, 1.1e-8 and 7.8e-10)."""
     from lbm_amd import workloads
     from parity import rel_err
     import oracle_lib
@@ -299,11 +307,12 @@ def test_closure_fallback_against_the_oracle_keys():
         c.extraForces.append(bond_forces)
     net = MLPClosure(seed=1)
     net.setNormalization(u_scale=1.0, v_scale=1.0, out_scale=1.0, u_bound=0.0, v_bound=0.0)
-    net.attach(gpu)
     worst = {"density": 0.0, "velocity": 0.0, "force": 0.0, "position": 0.0}
     for step in range(20):
         ref.step(1)
         gpu.step(1)
+        if step == 0:
+            net.attach(gpu)        # after the first step from rest, whose zero inputs are in distribution
         r0, u0 = ref.getFluidFields()
         r1, u1 = gpu.getFluidFields()
         worst["density"] = max(worst["density"], rel_err(r1, r0))
@@ -312,5 +321,5 @@ def test_closure_fallback_against_the_oracle_keys():
         worst["position"] = max(worst["position"], rel_err(gpu.positions, ref.positions, scale=w.dx))
     network, fallback = stats(gpu)
     print("closure fallback vs oracle Keys:", worst, "couplings", network, fallback)
-    assert network == 0 and fallback >= 20*len(w.masses)
-    assert max(worst.values()) < 1e-4
+    assert network == 0 and fallback >= 19*len(w.masses)
+    assert max(worst.values()) < 1e-5
