@@ -64,6 +64,16 @@ int lbm_mlp_train(lbm_mlp* mlp, const float* features, const float* targets, int
 int lbm_mlp_forward(lbm_mlp* mlp, const float* features, int64_t n, float* out);
 
 /*
+ * No counterpart in the reference: lbm_mlp_forward for callers that cannot allocate device memory (serving loops and
+ * harnesses that forbid hipMalloc).  The same kernels and results, with no hipMalloc, hipFree or allocating library
+ * call: the device working memory of up to LBM_MLP_NOALLOC_MAX_ROWS rows is a fixed array that each GPU holds until
+ * the process ends (9.0 MB).  n beyond that maximum is refused with LBM_ERROR_INVALID.  Every call on one GPU shares
+ * that GPU's array: calls that overlap on one GPU (networks on different streams, or threads) race.
+ */
+#define LBM_MLP_NOALLOC_MAX_ROWS 16384
+int lbm_mlp_forward_noalloc(lbm_mlp* mlp, const float* features, int64_t n, float* out);
+
+/*
  * The gradient of the loss mean_b (1/3)|y_b - t_b|^2 (normalized outputs and targets) with respect to the
  * parameters, for one batch of n raw pairs, computed by the training kernels (no update); loss may be NULL.
  */

@@ -20,6 +20,7 @@
 #include "lbm_mlp.h"
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 #include <numeric>
 #include <random>
 #include <sstream>
@@ -382,8 +383,44 @@ unsigned blocksFor(int64_t n, int threads) {
 
 }  // namespace
 
+/*
+ * The device working memory of lbm_mlp_forward_noalloc: raw inputs, normalized FP16 inputs and outputs of up to
+ * LBM_MLP_NOALLOC_MAX_ROWS rows (9.0 MB), one copy per GPU for the life of the process.  Every call on a GPU shares
+ * that GPU's copy; nothing in it has to start at zero (each call writes every element it reads).
+ */
+namespace {
+constexpr size_t NOALLOC_RAW_BYTES = (size_t) LBM_MLP_NOALLOC_MAX_ROWS*MLP_IN*sizeof(float);
+constexpr size_t NOALLOC_X_BYTES = (size_t) LBM_MLP_NOALLOC_MAX_ROWS*MLP_IN_PAD*sizeof(_Float16);
+constexpr size_t NOALLOC_OUT_BYTES = (size_t) LBM_MLP_NOALLOC_MAX_ROWS*MLP_OUT*sizeof(float);
+static_assert(NOALLOC_RAW_BYTES % 256 == 0 && NOALLOC_X_BYTES % 256 == 0, "256-byte aligned parts");
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[NOALLOC_RAW_BYTES + NOALLOC_X_BYTES + NOALLOC_OUT_BYTES];
+
+namespace {
+std::mutex scratchMutex;
+std::vector<void*> scratchByDevice;
+
+/* The device address of `scratch` on the current device (the host-side name is only a stand-in), looked up once per
+   device with that device current. */
+void* scratchOnCurrentDevice() {
+    int device = 0;
+    hipCheck(hipGetDevice(&device), "hipGetDevice");
+    std::lock_guard<std::mutex> lock(scratchMutex);
+    if ((int) scratchByDevice.size() <= device)
+        scratchByDevice.resize(device + 1, nullptr);
+    if (scratchByDevice[device] == nullptr) {
+        void* ptr = nullptr;
+        hipCheck(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+        scratchByDevice[device] = ptr;
+    }
+    return scratchByDevice[device];
+}
+}  // namespace
+
 struct lbm_mlp {
     hipStream_t stream = nullptr;
+    void* scratch = nullptr;          // this GPU's no-allocation working memory (scratchOnCurrentDevice)
     MlpDevice* dev = nullptr;
     float* master = nullptr;
     float* mom1 = nullptr;
@@ -588,6 +625,7 @@ int lbm_mlp_create(uint64_t seed, lbm_mlp** out) {
     lbm_mlp* m = new lbm_mlp();
     try {
         hipCheck(hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking), "stream");
+        m->scratch = scratchOnCurrentDevice();
         m->dev = deviceAlloc<MlpDevice>(1);
         m->master = deviceAlloc<float>(MLP_PARAMS);
         m->mom1 = deviceAlloc<float>(MLP_PARAMS);
@@ -711,6 +749,30 @@ int lbm_mlp_forward(lbm_mlp* m, const float* features, int64_t n, float* out) {
         (void) hipStreamSynchronize(m->stream);
         (void) hipFree(du);
         hipCheck(e, "forward");
+    })
+}
+
+int lbm_mlp_forward_noalloc(lbm_mlp* m, const float* features, int64_t n, float* out) {
+    MLP_TRY({
+        if (n <= 0 || n > LBM_MLP_NOALLOC_MAX_ROWS) {
+            std::stringstream msg;
+            msg << "lbm_mlp_forward_noalloc: n = " << n << " is outside [1, LBM_MLP_NOALLOC_MAX_ROWS = "
+                << LBM_MLP_NOALLOC_MAX_ROWS << "]";
+            throw MlpError(LBM_ERROR_INVALID, msg.str());
+        }
+        char* base = (char*) m->scratch;
+        float* raw = (float*) base;
+        _Float16* X = (_Float16*) (base + NOALLOC_RAW_BYTES);
+        float* du = (float*) (base + NOALLOC_RAW_BYTES + NOALLOC_X_BYTES);
+        hipCheck(hipMemcpyAsync(raw, features, (size_t) n*MLP_IN*sizeof(float), hipMemcpyHostToDevice, m->stream),
+                "forward upload");
+        mlpNormalize<<<blocksFor(n*MLP_IN_PAD, 256), 256, 0, m->stream>>>(raw, nullptr, n, mlpClosureNorm(m), X,
+                nullptr);
+        launchMlpInfer(m->dev, X, (int) n, (float) m->norm.out_scale, du, m->stream);
+        hipCheck(hipGetLastError(), "forward launch");
+        hipCheck(hipMemcpyAsync(out, du, (size_t) n*MLP_OUT*sizeof(float), hipMemcpyDeviceToHost, m->stream),
+                "forward download");
+        hipCheck(hipStreamSynchronize(m->stream), "forward");
     })
 }
 

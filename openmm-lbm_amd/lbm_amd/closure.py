@@ -41,6 +41,7 @@ CLOSURE_FUNCTIONS = {
     "lbm_mlp_train": (ctypes.c_int, [_P, _F, _F, ctypes.c_int64, ctypes.c_int, ctypes.c_int, ctypes.c_double,
                                      ctypes.c_double, ctypes.c_uint64, _D]),
     "lbm_mlp_forward": (ctypes.c_int, [_P, _F, ctypes.c_int64, _F]),
+    "lbm_mlp_forward_noalloc": (ctypes.c_int, [_P, _F, ctypes.c_int64, _F]),
     "lbm_mlp_gradients": (ctypes.c_int, [_P, _F, _F, ctypes.c_int64, _F, _D]),
     "lbm_closure_pairs": (ctypes.c_int, [_P, _D, _D, _F, _F]),
     "lbm_set_closure": (ctypes.c_int, [_P, _P]),
