@@ -56,22 +56,72 @@ def make_workload(name):
     raise SystemExit(f"unknown workload {name}")
 
 
-def cpu_baseline(workload_name, budget_steps=1):
-    """The FP64 CPU oracle (one core, the reference's serial loops restated) on the same workload, timed over a
-    bounded number of lattice steps after setup."""
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or platform.machine()
+
+
+def host_threads():
+    """The host cores this process may use (its affinity), at most 16 (the CPU share of a one-GPU box)."""
+    try:
+        count = len(os.sched_getaffinity(0))
+    except AttributeError:
+        count = os.cpu_count() or 1
+    return max(1, min(16, count))
+
+
+def cpu_baseline(workload_name, warmup_steps=2, timed_steps=10):
+    """The FP64 CPU oracle (the reference's serial loops restated; its node loops shared among OpenMP threads) on the
+    same workload: all host cores over timed_steps lattice steps after warmup_steps, then one thread over one step."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib
+    lib = oracle_lib.load_oracle()
+    lib.lbm_oracle_threads.restype = ctypes.c_int
+    lib.lbm_oracle_threads.argtypes = [ctypes.c_int]
     w = make_workload(workload_name)
-    ctx = w.build(oracle_lib.load_oracle())
-    ctx.step(1)      # warm-up (first touch of the arrays)
+    ctx = w.build(lib)
+    threads = lib.lbm_oracle_threads(host_threads())
+    ctx.step(warmup_steps)
     t0 = time.perf_counter()
-    ctx.step(budget_steps)
+    ctx.step(timed_steps)
     elapsed = time.perf_counter() - t0
-    mlups = w.num_nodes*budget_steps/elapsed/1e6
-    return {"value": round(mlups, 3), "unit": "MLUPS", "cores": 1, "kind": "port",
-            "sample": f"{budget_steps} lattice step(s) of the same {w.grid[0]}^3 workload after 1 warm-up step, "
-                      f"{elapsed:.1f} s, FP64 oracle (oracle/lbm_oracle.cpp), one thread; "
-                      f"host {platform.processor() or platform.machine()}, nproc {os.cpu_count()}"}
+    lib.lbm_oracle_threads(1)
+    t1 = time.perf_counter()
+    ctx.step(1)
+    serial = time.perf_counter() - t1
+    n = w.num_nodes
+    model = cpu_model()
+    return {"value": round(n*timed_steps/elapsed/1e6, 3), "unit": "MLUPS", "cores": threads, "kind": "port",
+            "sample": f"{timed_steps} coupled lattice steps of the same {w.grid[0]}^3 workload after {warmup_steps} "
+                      f"warm-up steps, {elapsed:.1f} s, FP64 oracle (oracle/lbm_oracle.cpp) with its node loops on "
+                      f"{threads} OpenMP threads; host {model}, nproc {os.cpu_count()}",
+            "cpu_model": model,
+            "single_thread": {"value": round(n/serial/1e6, 3), "unit": "MLUPS", "cores": 1,
+                              "sample": f"1 further step on one thread, {serial:.1f} s"}}
+
+
+WARMUP_SECONDS = 0.25
+
+
+def warm_up(stepper, sync, steps):
+    """At least `steps` untimed steps, and more until WARMUP_SECONDS of GPU work have run: an idle MI355X starts in a
+    low clock state (sclk level S) and its first ~25 fused passes of config2 run 0.58 down to 0.45 ms before it settles,
+    so a timed region that begins after a few steps would time the clock ramp rather than the step.  Returns the number
+    of warm-up steps run."""
+    stepper.step(steps)
+    sync()
+    done = steps
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < WARMUP_SECONDS and done < steps + 2000:
+        stepper.step(10)
+        sync()
+        done += 10
+    return done
 
 
 def copy_calibration(mib=1024, repeats=5):
@@ -90,7 +140,7 @@ def copy_calibration(mib=1024, repeats=5):
     print(json.dumps({"copy_calibration_bytes": nbytes, "repeats": repeats}), flush=True)
 
 
-def precision_study(steps, n=256, num_beads=20000):
+def precision_study(steps, n=256, num_beads=20000, warmup=300):
     """The precision form: config2 with fluctuations off, run with FP64, FP32, FP16 and BF16 storage of the
     populations.  Reports, against the FP64 run (which agrees with the FP64 CPU oracle to 1e-12), the error of the mean
     velocity profile u_x(z) (plane averages, max over z relative to the max of the profile), of the bead forces (max
@@ -106,6 +156,8 @@ def precision_study(steps, n=256, num_beads=20000):
         ctx = w.build(storage=st)
         dp = w.device_particles(ctx)
         m0, p0 = ctx.getFluidTotals()
+        dp.step(warmup)             # the same number of untimed steps for every storage, so the runs stay comparable
+        _lib.check(ctx.lib, ctx.lib.lbm_synchronize(ctx.handle))
         _lib.check(ctx.lib, ctx.lib.lbm_profile_fused(ctx.handle, 1, None, None))
         t0 = time.perf_counter()
         dp.step(steps)
@@ -327,6 +379,11 @@ def closure_study(steps, warmup, snapshots=10, interval=20, epochs=40, batch=204
                  "mlups": round(n/keys_s/1e6, 1)},
         "closure": {"ms_per_step": round(1e3*closure_s, 4), "coupled_steps_per_s": round(1/closure_s, 2),
                     "mlups": round(n/closure_s/1e6, 1), "network_couplings": network, "fallback_couplings": fallback},
+        # The closure's accuracy is stated on the drag it approximates, gamma m (u_Keys - v): at 300 K the total coupling
+        # force is dominated by the random force, identical in both runs, which would deflate the ratio.
+        "bead_force_error_vs_keys": round(rms(df)/rms(drag), 5),
+        "bead_force_error_definition": "rms |F_closure - F_Keys| / rms |gamma m (u_Keys - v)| over the beads at one "
+                                       "state (same random forces); target <= 0.02",
         "force_error_vs_keys": {
             "rms_rel_total_force": round(rms(df)/rms(f_keys), 5),
             "rms_rel_drag_force": round(rms(df)/rms(drag), 5),
@@ -338,6 +395,32 @@ def closure_study(steps, warmup, snapshots=10, interval=20, epochs=40, batch=204
     }
 
 
+def extra_configs():
+    """Bounded runs of the other benchmark configurations, summarized as extra keys of the default line: the precision
+    form (config3: FP16/BF16 against FP32 and FP64 storage), the batched replicas (config4) and the learned closure
+    (config5, against the Keys stencil)."""
+    out = {}
+    print("bench: config3 (precision form, 200 timed steps per storage)", file=sys.stderr, flush=True)
+    c3 = precision_study(200)
+    st = c3["storages"]
+    out["config3"] = {
+        "workload": c3["config"]["workload"], "steps": c3["steps"],
+        "metric": "coupled steps/s of FP16/BF16 storage over FP32; u_x(z) profile and bead-force errors vs the FP64 run",
+        "fp16_speedup_vs_fp32": round(st["fp16"]["speedup_vs_fp32"], 3),
+        "bf16_speedup_vs_fp32": round(st["bf16"]["speedup_vs_fp32"], 3),
+        "fp16_profile_rel_err": st["fp16"]["profile_rel_err"], "bf16_profile_rel_err": st["bf16"]["profile_rel_err"],
+        "storages": st}
+    print("bench: config4 (batched replicas, 200 timed steps)", file=sys.stderr, flush=True)
+    c4 = batch_study(200, 20, sequential_steps=10)
+    out["config4"] = {k: c4[k] for k in ("value", "unit", "ms_per_step", "coupled_steps_per_s", "config", "roofline",
+                                         "sequential_single_lattice")}
+    print("bench: config5 (learned closure vs Keys, 100 timed steps)", file=sys.stderr, flush=True)
+    c5 = closure_study(100, 20)
+    out["config5"] = {k: c5[k] for k in ("value", "unit", "config", "keys", "closure", "bead_force_error_vs_keys",
+                                         "bead_force_error_definition", "force_error_vs_keys", "training")}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -346,6 +429,8 @@ def main():
     ap.add_argument("--workload", default="config2")
     ap.add_argument("--storage", default="fp32")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the bounded config3/config4/config5 runs that the default config2 line carries")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device copy (for the counter scale of rocprofv3 --pmc)")
     args = ap.parse_args()
@@ -376,8 +461,7 @@ def main():
     ctx = w.build(storage=storage_code(args.storage))
     dp = w.device_particles(ctx)
     lib = ctx.lib
-    dp.step(args.warmup)
-    _lib.check(lib, lib.lbm_synchronize(ctx.handle))
+    warmup_run = warm_up(dp, lambda: _lib.check(lib, lib.lbm_synchronize(ctx.handle)), args.warmup)
     _lib.check(lib, lib.lbm_profile_fused(ctx.handle, 1, None, None))
     t0 = time.perf_counter()
     dp.step(args.steps)
@@ -414,6 +498,7 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
+        "warmup_run": warmup_run,
         "ms_per_step": round(1e3*elapsed/args.steps, 4),
         "coupled_steps_per_s": round(steps_per_s, 2),
         "higher_is_better": True,
@@ -427,8 +512,14 @@ def main():
                      "kernel": "fusedStep", "kernel_ms": round(fused_ms, 4), "bytes_per_node": bpn,
                      "algorithmic_bytes_per_launch": round(bpn*n),
                      "kernel_ms_min_median_max": [round(launch_ms[0], 4), round(launch_ms[len(launch_ms)//2], 4),
-                                                  round(launch_ms[-1], 4)] if launch_ms else None},
+                                                  round(launch_ms[-1], 4)] if launch_ms else None,
+                     "traffic_source": (f"profiles/fusedStep_{args.workload}_{args.storage}.traffic.json: rocprofv3 "
+                                        f"--pmc FETCH_SIZE and WRITE_SIZE of the same workload, scaled by a copy of "
+                                        f"known size") if traffic is not None else None},
     }
+    del dp, ctx
+    if args.workload == "config2" and args.storage == "fp32" and not args.no_extras:
+        result.update(extra_configs())
     if not args.no_cpu_baseline:
         print("bench: timing the CPU oracle baseline", file=sys.stderr, flush=True)
         result["cpu_baseline"] = cpu_baseline(args.workload)

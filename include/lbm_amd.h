@@ -150,6 +150,8 @@ int lbm_compute_lattice_parameters(const lbm_force_settings* settings, const dou
  * solid_nodes: node indices i + nx*(j + ny*k), sorted, without repetitions.
  * At most 2^30 nodes (LBM_ERROR_INVALID beyond).  LBM_AMD_WIDE_ADDRESSING=1 in the environment makes the lattice
  * pass use 64-bit indices at any size (it does by itself where an array of populations exceeds 4 GiB).
+ * LBM_AMD_FUSED_ROWS=1 makes it stage its pulls through LDS where rows hold 128 or a multiple of 256 nodes (the same
+ * results; slower on MI355X, so off by default).
  */
 int lbm_create(const lbm_params* params, const int* solid_nodes, int num_solid, const int* particles,
         const double* particle_masses, int num_particles, int num_system_particles, lbm_context** out);
@@ -200,6 +202,13 @@ int lbm_copy_parameters(lbm_context* ctx, const lbm_params* params);
  * (density_halo_exchange, velocity_halo_exchange), and NaN for the other fields and beyond open faces.
  */
 int lbm_get_fluid_fields(lbm_context* ctx, double* density, double* velocity, int halo);
+
+/*
+ * Diagnostics with no counterpart in the reference: the nearest node (i + nx (j + ny k)) of every coupled particle,
+ * in coupling order, for the positions (nm) of all particles of the System, by the rule of nearestNode
+ * (ReferenceLBMKernels.cpp:973-983) as the coupling kernels apply it.
+ */
+int lbm_get_particle_nodes(lbm_context* ctx, const double* positions, int* nodes);
 
 /* Replaces getFluidMachNumber (LBMKernels.h:149; ReferenceLBMKernels.cpp:1449-1468). */
 int lbm_get_fluid_mach_number(lbm_context* ctx, double* mach);

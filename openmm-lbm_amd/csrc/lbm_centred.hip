@@ -76,6 +76,8 @@ __device__ __forceinline__ bool addReaction(uint32_t node, bool solid, const dou
     }
     bool any = false;
     for (int k = 0; k < 3; k++) {
+        if (!reactionInRange(r[k], a.overflow))
+            continue;
         long long fixed = llrint(r[k]*FIXED_SCALE);
         if (fixed != 0) {
             atomicAdd(reactionFixed + reactionIndex(node, k, a.g.n, a.reactionPlanes), (unsigned long long) fixed);

@@ -122,6 +122,7 @@ struct lbm_context {
     double* reducePartials = nullptr;
     double* totals = nullptr;
     double* ucm = nullptr;
+    unsigned int* overflow = nullptr;     // set by a particle reaction beyond REACTION_LIMIT
     double* wallPartials = nullptr;
     int wallBounceOffset = 0, wallBoundaryOffset = 0, wallCoupleOffset = 0, wallReflectOffset = 0, wallPartialCount = 0;
     bool wallStepDone = false;
@@ -315,6 +316,7 @@ ParticleArgs particleArgs(lbm_context* c, int64_t step, bool isStep) {
     a.step = step;
     a.hasSolids = c->mask != nullptr;
     a.reactionPlanes = c->dense ? 1 : 0;
+    a.overflow = c->overflow;
     return a;
 }
 
@@ -411,7 +413,21 @@ void advanceFluid(lbm_context* c, const double* dPos, const double* dVel, const 
     c->wallStepDone = true;
     c->stepForcesCurrent = true;
     c->stepIndex++;
-    if (c->p.mach_check_frequency > 0 && c->stepIndex % c->p.mach_check_frequency == 0) {
+    bool machDue = c->p.mach_check_frequency > 0 && c->stepIndex % c->p.mach_check_frequency == 0;
+    if (particles && (machDue || c->stepIndex % 100 == 0)) {
+        // A particle reaction beyond the range of the fixed-point accumulators was dropped: the run has blown up.
+        unsigned int flag = 0;
+        HIP_CHECK(hipMemcpyAsync(&flag, c->overflow, sizeof(flag), hipMemcpyDeviceToHost, c->stream));
+        HIP_CHECK(hipStreamSynchronize(c->stream));
+        if (flag != 0) {
+            std::stringstream msg;
+            msg << "LBMForce: a particle reaction beyond " << REACTION_LIMIT << " lattice units (or not a number) by "
+                << "lattice step " << c->stepIndex << ": the coupling forces have diverged.  Reduce the time step, the "
+                << "friction or the forces on the particles.";
+            throw Error(LBM_ERROR_RUNTIME, msg.str());
+        }
+    }
+    if (machDue) {
         reduceMoments(c);
         double totals[5];
         HIP_CHECK(hipMemcpyAsync(totals, c->totals, sizeof(totals), hipMemcpyDeviceToHost, c->stream));
@@ -473,6 +489,7 @@ void destroy(lbm_context* c) {
     centredDestroy(c->centred);
     coupleOrderDestroy(c->order);
     void* buffers[] = {c->popA, c->popB, c->mask, c->wallNodes, c->wallLinks, c->boundary, c->reducePartials, c->totals,
+                       c->overflow,
                        c->ucm, c->wallPartials, c->particles, c->mass, c->particleForce, c->reactionFixed,
                        c->touched, c->stagePos, c->stageVel, c->closureX, c->closureNode, c->closureU, c->closureDu,
                        c->closureOod, c->closureCounters, c->closureField, c->stageForce};
@@ -566,6 +583,8 @@ void create(const lbm_params* params, const int* solid, int numSolid, const int*
     // Reductions.
     c->reducePartials = deviceAlloc<double>(5*REDUCE_BLOCKS);
     c->totals = deviceAlloc<double>(8);
+    c->overflow = deviceAlloc<unsigned int>(1);
+    HIP_CHECK(hipMemsetAsync(c->overflow, 0, sizeof(unsigned int), c->stream));
     c->ucm = deviceAlloc<double>(4);
     HIP_CHECK(hipMemsetAsync(c->ucm, 0, 4*sizeof(double), c->stream));
     // Particles.
@@ -872,6 +891,27 @@ int lbm_get_fluid_fields(lbm_context* c, double* density, double* velocity, int 
             throw;
         }
         HIP_CHECK(hipFree(stage));
+    })
+}
+
+int lbm_get_particle_nodes(lbm_context* c, const double* positions, int* nodes) {
+    LBM_CHECK_CTX(c);
+    LBM_TRY({
+        if (c->numParticles == 0)
+            return LBM_OK;
+        size_t bytes = 3*sizeof(double)*(size_t) c->numSystem;
+        HIP_CHECK(hipMemcpyAsync(c->stagePos, positions, bytes, hipMemcpyHostToDevice, c->stream));
+        int* dNodes = deviceAlloc<int>(c->numParticles);
+        ParticleArgs a = particleArgs(c, c->stepIndex, false);
+        launchParticleNodes(c->stagePos, c->particles, a, dNodes, c->stream);
+        hipError_t e = hipGetLastError();
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(nodes, dNodes, sizeof(int)*(size_t) c->numParticles, hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess)
+            e = hipStreamSynchronize(c->stream);
+        (void) hipStreamSynchronize(c->stream);
+        (void) hipFree(dNodes);
+        HIP_CHECK(e);
     })
 }
 

@@ -275,7 +275,7 @@ __device__ __forceinline__ bool spreadReaction(const StencilSlot& sl, const doub
         double r = -sl.weight*f[k];
         if (sl.solid)
             wallAcc[k] = r;
-        else {
+        else if (reactionInRange(r, a.overflow)) {
             long long fixed = llrint(r*FIXED_SCALE);
             if (fixed != 0) {
                 __hip_atomic_fetch_add(asGlobal(reactionFixed + reactionIndex(sl.node, k, a.g.n, a.reactionPlanes)),

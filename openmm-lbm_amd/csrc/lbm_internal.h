@@ -73,7 +73,24 @@ struct ParticleArgs {
     int64_t step;
     int hasSolids;
     int reactionPlanes;     // layout of the reaction accumulators (reactionIndex)
+    unsigned int* overflow = nullptr;   // set to 1 by a reaction beyond REACTION_LIMIT (reactionInRange)
 };
+
+/*
+ * The largest particle reaction one contribution may carry, in lattice units.  The fixed-point accumulators of a node
+ * hold +-8192 (2^63 of 2^-50); a single contribution of 64 corresponds to a force of about 4e8 kJ/mol/nm at dx = 1 nm,
+ * dt = 0.01 ps and the density of water, so one beyond it means a run that has blown up (or a NaN): it is dropped and
+ * the flag set, and the host reports LBM_ERROR_RUNTIME at its next check.
+ */
+constexpr double REACTION_LIMIT = 64.0;
+
+__device__ __forceinline__ bool reactionInRange(double r, unsigned int* overflow) {
+    if (fabs(r) < REACTION_LIMIT)
+        return true;
+    if (overflow != nullptr)
+        atomicOr(overflow, 1u);
+    return false;
+}
 
 /*
  * Index of component c of a node's three reaction accumulators.  Sparse particles use node-major [node][3]: a stencil
@@ -173,6 +190,8 @@ void coupleOrderDestroy(CoupleOrder* order);
 const int* coupleOrderAt(CoupleOrder* order, int64_t step, const double* positions, const int* particles,
         const ParticleArgs& a, hipStream_t stream);
 void launchAddForces(const double* particleForce, const int* particles, int count, double* forces, hipStream_t stream);
+void launchParticleNodes(const double* positions, const int* particles, const ParticleArgs& a, int* nodes,
+        hipStream_t stream);
 
 /* The centred drag (lbm_centred.hip): its buffers for numParticles coupled particles, and one coupling.  otherForces
    (kJ/mol/nm, all System particles, may be null) are the forces on the particles before the coupling; g is the body

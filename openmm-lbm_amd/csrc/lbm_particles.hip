@@ -305,6 +305,20 @@ void launchAddForces(const double* particleForce, const int* particles, int coun
     addForces<<<gridFor(count), BLOCK, 0, stream>>>(particleForce, particles, count, forces);
 }
 
+/* The nearest node of every coupled particle, as the coupling kernels compute it (nearestNodeOf). */
+__global__ void __launch_bounds__(BLOCK) particleNodes(const double* __restrict__ positions,
+        const int* __restrict__ particles, ParticleArgs a, int* __restrict__ nodes) {
+    int i = blockIdx.x*BLOCK + threadIdx.x;
+    if (i < a.numParticles)
+        nodes[i] = (int) nearestNodeOf(positions + 3*(size_t) particles[i], a);
+}
+
+void launchParticleNodes(const double* positions, const int* particles, const ParticleArgs& a, int* nodes,
+        hipStream_t stream) {
+    if (a.numParticles > 0)
+        particleNodes<<<gridFor(a.numParticles), BLOCK, 0, stream>>>(positions, particles, a, nodes);
+}
+
 
 /* ------------------------------------------------------------------------------------------ */
 /* Processing order of the coupling.                                                           */

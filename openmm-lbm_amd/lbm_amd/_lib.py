@@ -90,6 +90,7 @@ KERNEL_FUNCTIONS = {
     "lbm_copy_parameters": (ctypes.c_int, [_P, ctypes.POINTER(LBMParams)]),
     "lbm_get_fluid_fields": (ctypes.c_int, [_P, _D, _D, ctypes.c_int]),
     "lbm_get_fluid_mach_number": (ctypes.c_int, [_P, _D]),
+    "lbm_get_particle_nodes": (ctypes.c_int, [_P, _D, _I]),
     "lbm_get_wall_force": (ctypes.c_int, [_P, _D]),
     "lbm_get_fluid_state": (ctypes.c_int, [_P, _D, ctypes.c_size_t]),
     "lbm_set_fluid_state": (ctypes.c_int, [_P, _D, ctypes.c_size_t]),

@@ -216,6 +216,16 @@ class Context:
         state = np.ascontiguousarray(state, dtype=np.float64)
         self._check(self.lib.lbm_set_fluid_state(self.handle, _dptr(state), state.size))
 
+    def getParticleNodes(self, positions=None):
+        """The nearest node of every coupled particle (lbm_get_particle_nodes), for the current positions or the given
+        ones."""
+        pos = np.ascontiguousarray(self.positions if positions is None else positions, dtype=np.float64)
+        count = len(self.coupled)
+        nodes = np.zeros(max(count, 1), dtype=np.int32)
+        self._check(self.lib.lbm_get_particle_nodes(self.handle, _dptr(pos),
+                                                    nodes.ctypes.data_as(ctypes.POINTER(ctypes.c_int))))
+        return nodes[:count]
+
     def getFluidMachNumber(self):
         m = ctypes.c_double()
         self._check(self.lib.lbm_get_fluid_mach_number(self.handle, ctypes.byref(m)))

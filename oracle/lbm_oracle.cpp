@@ -36,6 +36,9 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 #include <cstring>
 #include <limits>
 #include <random>
@@ -43,6 +46,9 @@
 #include <stdexcept>
 #include <string>
 #include <vector>
+
+/* Threads of the node loops (lbm_oracle_threads): one by default, the serial loops of the reference. */
+static int oracleThreads = 1;
 
 namespace {
 
@@ -550,9 +556,11 @@ public:
     }
 
     void computeMoments() {
-        // ReferenceLBMKernels.cpp:430-469.
-        double f[NQ];
+        // ReferenceLBMKernels.cpp:430-469.  Nodes are independent: with OpenMP the loop is shared among threads,
+        // with results independent of their number.
+#pragma omp parallel for schedule(static) num_threads(oracleThreads)
         for (int node = 0; node < numNodes; node++) {
+            double f[NQ];
             if (!fluid(node)) {
                 rho[node] = dev[node] = 0.0;
                 for (int k = 0; k < 3; k++)
@@ -929,9 +937,12 @@ public:
         bool fluct = p.fluid_fluctuations && p.fluid_kT > 0;
         double mu = 3.0*p.fluid_kT*p.dt*p.dt/(cellMass()*p.dx*p.dx);
         double omega = p.omega;
-        double eq[NQ], neq[NQ], src[NQ], xi[NQ], normal[16];
         int nx = p.nx, ny = p.ny, nz = p.nz;
+        // Each population slot is written by exactly one node and none is read (the moments are stored): the nodes
+        // are shared among OpenMP threads, with results independent of their number.
+#pragma omp parallel for schedule(static) num_threads(oracleThreads)
         for (int node = 0; node < numNodes; node++) {
+            double eq[NQ], neq[NQ], src[NQ], xi[NQ], normal[16];
             if (!fluid(node))
                 continue;
             int i = node % nx, j = (node/nx) % ny, k = node/(nx*ny);
@@ -1279,6 +1290,25 @@ int lbm_get_fluid_fields(lbm_context* ctx, double* density, double* velocity, in
                         for (int a = 0; a < 3; a++)
                             velocity[3*l + a] = velocityValid ? u[a]*(k.p.dx/k.p.dt) : nan;
                 }
+    })
+}
+
+/* Threads of the oracle's node loops (OpenMP), 1 unless set; n <= 0 leaves them.  Returns the number in use. */
+int lbm_oracle_threads(int n) {
+#ifdef _OPENMP
+    if (n > 0)
+        oracleThreads = n;
+    return oracleThreads;
+#else
+    (void) n;
+    return 1;
+#endif
+}
+
+int lbm_get_particle_nodes(lbm_context* ctx, const double* positions, int* nodes) {
+    ORACLE_TRY({
+        for (size_t i = 0; i < ctx->kernel.particles.size(); i++)
+            nodes[i] = ctx->kernel.nearestNode(positions + 3*(size_t) ctx->kernel.particles[i]);
     })
 }
 

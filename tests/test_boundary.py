@@ -170,3 +170,94 @@ def test_copy_parameters_refuses_fixed_settings(backend):
     c.step(1)
     twin.step(1)
     assert not np.array_equal(c.getFluidState(), twin.getFluidState())
+
+
+# ------------------------------------------------------------------------------------------------------------------
+# The time filter of the Density faces (ReferenceLBMKernels.cpp:1264-1272; docs/theory.md:366-382), pinned on one step
+# away from steady state, where the filter weight shows.
+# ------------------------------------------------------------------------------------------------------------------
+CXA = np.array([0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 0, 0, 0, 0, 1, -1, -1, 1])
+CYA = np.array([0, 0, 0, 1, -1, 0, 0, 1, -1, -1, 1, 1, -1, 1, -1, 0, 0, 0, 0])
+CZA = np.array([0, 0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 1, -1, -1, 1, 1, -1, 1, -1])
+WA = np.array([1/3] + [1/18]*6 + [1/36]*12)
+OPPOSITE = np.array([0, 2, 1, 4, 3, 6, 5, 8, 7, 10, 9, 12, 11, 14, 13, 16, 15, 18, 17])
+
+
+def equilibrium_deviation(drho, u):
+    """feq_q - w_q = w_q [drho + rho (3 c.u + 9/2 (c.u)^2 - 3/2 u.u)] (the second-order equilibrium, cs^2 = 1/3)."""
+    cu = CXA*u[0] + CYA*u[1] + CZA*u[2]
+    return WA*(drho + (1 + drho)*(3*cu + 4.5*cu*cu - 1.5*np.dot(u, u)))
+
+
+def hermite2():
+    """H2(c_q) = c c - I/3 as xx, yy, zz, xy, xz, yz (rows q)."""
+    return np.stack([CXA*CXA - 1/3, CYA*CYA - 1/3, CZA*CZA - 1/3, CXA*CYA, CXA*CZA, CYA*CZA], axis=1)
+
+
+def test_density_filter_helpers_match_golden_vectors():
+    """The helpers of the filter test against the reference's D3Q19 algebra (golden vectors from its header)."""
+    import os
+    g = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "d3q19_vectors.npz"))
+    H = hermite2()
+    for k in range(len(g["dr"])):
+        assert np.abs(equilibrium_deviation(g["dr"][k], g["u"][k]) - g["equilibrium"][k]).max() < 1e-15
+        # Pi_neq of the deviations df at the momentum (1 + dr) u, then the regularized non-equilibrium part
+        pi = H.T @ (g["df"][k] - equilibrium_deviation(g["dr"][k], g["u"][k]))
+        assert np.abs(pi - g["pi_neq"][k]).max() < 1e-15
+        fneq = 4.5*WA*(H[:, :3] @ g["pi_neq"][k][:3] + 2*(H[:, 3:] @ g["pi_neq"][k][3:]))
+        assert np.abs(fneq - g["regularized_neq"][k]).max() < 1e-15
+
+
+@pytest.mark.parametrize("backend", BACKENDS[:2])
+def test_density_face_time_filter(backend):
+    """One step from a state away from steady flow (random velocities and non-equilibrium parts): the populations that
+    a node of a Density face (ZMin, density 1.004) rebuilds, its unknown directions c_z = +1, are the equilibrium at
+    the face density and the velocity u_b, plus (1 - omega) times the regularized non-equilibrium part of the node.
+    u_b keeps the node's velocity v_x = j/rho along the face; across it, u_b is (v_ZH + v_x)/2, where v_ZH is the
+    velocity that gives the node the density of the face from the populations that arrived, the unknown ones replaced
+    by the bounce-back of their opposites (Zou and He 1997: rho_face = rho_0 + 2 rho_out + rho_face v_ZH).  The test
+    takes the post-streaming populations and the node's moments from the states before and after the step, and checks
+    that the weight 0 or 1 (v_ZH alone, v_x alone) would miss by more than 1e4 times the tolerance."""
+    nx, ny, nz = 3, 3, 4
+    n = nx*ny*nz
+    rng = np.random.default_rng(41)
+    s = Setup((nx, ny, nz), dx=1.0, dt=0.01, tau=0.8)
+    u = rng.normal(scale=0.02, size=(n, 3))
+    rho = 1.0 + rng.normal(scale=0.003, size=n)
+    state = equilibrium_state(n, rho, u).reshape(19, n) + rng.normal(scale=4e-4, size=(19, n))*WA[:, None]
+    s.state = state.reshape(-1)
+    face_density = 1.004
+
+    def conf(f):
+        f.setFaceBoundary(LBMForce.ZMin, LBMForce.Density)
+        f.setFaceBoundary(LBMForce.ZMax, LBMForce.Density)
+        f.setFaceDensity(LBMForce.ZMin, face_density*602.214)
+        f.setFaceDensity(LBMForce.ZMax, 602.214)
+    s.configure.append(conf)
+    c = build(s, backend)
+    c.step(1)
+    after = c.getFluidState().reshape(19, n)
+    omega = 1/0.8
+    node = 1 + nx*(1 + ny*0)
+    f0 = state[:, node]
+    drho = f0.sum()
+    j = np.array([(CXA*f0).sum(), (CYA*f0).sum(), (CZA*f0).sum()])
+    v_x = j/(1 + drho)
+    pi = hermite2().T @ (f0 - equilibrium_deviation(drho, v_x))
+    H = hermite2()
+    fneq = 4.5*WA*(H[:, :3] @ pi[:3] + 2*(H[:, 3:] @ pi[3:]))
+    unknown = CZA == 1
+    arrived = np.where(unknown, after[OPPOSITE, node], after[:, node])
+    v_zh = ((face_density - 1.0) - arrived.sum())/face_density          # inward = +z on ZMin
+
+    def rebuilt(beta):
+        u_b = np.array([v_x[0], v_x[1], beta*v_zh + (1 - beta)*v_x[2]])
+        return (equilibrium_deviation(face_density - 1.0, u_b) + (1 - omega)*fneq)[unknown]
+
+    got = after[unknown, node]
+    err = np.abs(got - rebuilt(0.5)).max()
+    miss = min(np.abs(got - rebuilt(0.0)).max(), np.abs(got - rebuilt(1.0)).max())
+    print(f"v_ZH {v_zh:.3e} v_x {v_x[2]:.3e}: rebuilt populations within {err:.1e} of the weight 1/2; "
+          f"weights 0 and 1 miss by {miss:.1e}")
+    assert err < 1e-13
+    assert miss > 1e4*1e-13

@@ -215,7 +215,8 @@ def test_checkpoint_refused_by_the_other_drag(stencil):
 @pytest.mark.parametrize("stencil", [LBMForce.NearestNode, LBMForce.Keys])
 def test_centred_parity_with_fluctuations(stencil):
     """The GPU against the oracle, 200 steps, dense particles sharing nodes and stencils, fluctuating fluid and particle
-    noise on the same Philox streams: fields, forces and positions to 1e-5 (the FP32 criterion)."""
+    noise on the same Philox streams: fields, forces and positions to 1e-5 (the FP32 criterion; measured 7e-9, 4.4e-7,
+    1.2e-7 and 3.2e-8)."""
     rng = np.random.default_rng(31)
     n = 10
     pos = rng.uniform(0, n*DX, size=(150, 3))
@@ -223,8 +224,8 @@ def test_centred_parity_with_fluctuations(stencil):
     s = centred_setup([100.0]*150, pos, v0, 20.0, stencil=stencil, n=n, temperature=300.0, fluct=True, seed=9)
     worst, _, _ = lockstep(s, 200, check_every=10)
     print("worst relative errors", worst)
-    for key in ("density", "velocity", "force"):
-        assert worst[key] < 1e-4, (key, worst)
+    for key in ("density", "velocity", "force", "position"):
+        assert worst[key] < 1e-5, (key, worst)
 
 
 SOLVE_BACKENDS = [pytest.param("oracle", id="oracle"),

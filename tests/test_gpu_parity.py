@@ -6,6 +6,7 @@ velocity and coupled-particle forces at every checked step; FP64 storage 1e-11.
 import numpy as np
 import pytest
 
+import oracle_lib
 from lbm_amd import LBMForce
 from parity import Setup, equilibrium_state, lockstep, rel_err, total_moments
 
@@ -62,10 +63,28 @@ def test_stencil_coupling_parity(stencil):
 
 
 def test_node_assignment_bit_exact():
-    """Particle-to-node assignment at ties: positions on half-integer planes go to the same node as on the CPU."""
+    """Particle-to-node assignment (nearestNode, ReferenceLBMKernels.cpp:973-983) compared as integer node indices,
+    GPU against the oracle, at ties and near them: positions on half-integer planes, one ulp either side of them,
+    outside the box by whole and half periods, negative, on the lattice nodes, and 2000 random ones; then the
+    couplings of three steps agree."""
     s = config1_setup(n=8, num_beads=0, stencil=LBMForce.NearestNode)
-    pos = np.array([[0.5, 1.5, 2.5], [7.5, 7.4999999999, 0.5000000001], [-0.5, 8.5, 16.5], [3.0, 3.0, 3.0]])
-    s.add_particles(pos, mass=100.0, velocities=np.ones((4, 3))*0.1)
+    pos = [[0.5, 1.5, 2.5], [7.5, 7.4999999999, 0.5000000001], [-0.5, 8.5, 16.5], [3.0, 3.0, 3.0],
+           [-8.5, -7.5, 23.5], [1e-300, 7.999999999999999, -1e-12]]
+    for h in (0.5, 3.5, 7.5):
+        pos.append([np.nextafter(h, 0), np.nextafter(h, 9), h])
+    rng = np.random.default_rng(3)
+    ties = np.floor(rng.uniform(-20, 28, size=(1000, 3))) + 0.5
+    pos = np.concatenate([np.array(pos), ties, rng.uniform(-20, 28, size=(1000, 3))])
+    s.add_particles(pos, mass=100.0, velocities=np.ones((len(pos), 3))*0.1)
+    ref = s.build(oracle_lib.load_oracle())
+    gpu = s.build(None)
+    for trial in range(2):
+        nodes_ref = ref.getParticleNodes()
+        nodes_gpu = gpu.getParticleNodes()
+        assert nodes_ref.dtype == np.int32 and len(nodes_ref) == len(pos)
+        assert np.array_equal(nodes_gpu, nodes_ref)
+        ref.step(1)
+        gpu.step(1)
     worst, ref, gpu = lockstep(s, 3, check_every=1)
     assert worst["force"] < TOL32, worst
 
@@ -412,3 +431,18 @@ def test_config2_full_size_properties():
     z = pos[:, 2]/w.dx
     assert z.min() > 0.0 and z.max() < 255.5          # none has crossed a wall (positions are not wrapped)
     assert p1[0] > p0[0] and abs(p1[1]) < 0.1*p1[0] and abs(p1[2]) < 0.1*p1[0]
+
+
+def test_reaction_beyond_fixed_point_range_is_reported():
+    """A particle reaction beyond the range of the fixed-point accumulators (a diverged coupling: a 10^6 Da bead at 1
+    lattice unit per step with gamma dt = 0.5) is dropped on the device and reported at the next check as
+    LBM_ERROR_RUNTIME, instead of wrapping around in the sums."""
+    from lbm_amd import LBMError, _lib
+    s = Setup((8, 8, 8), dx=1.0, dt=0.01, tau=0.8)
+    s.add_particles([[4.2, 3.7, 4.4]], mass=1e6, velocities=[[100.0, 0.0, 0.0]])
+    s.configure.append(lambda f: (f.setFriction(50.0), f.setTemperature(0.0), f.setInterpolationStencil(LBMForce.Keys),
+                                  f.setMachCheckFrequency(1)))
+    c = s.build(None)
+    with pytest.raises(LBMError, match="diverged") as err:
+        c.step(1)
+    assert err.value.status == _lib.LBM_ERROR_RUNTIME
