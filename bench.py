@@ -207,7 +207,7 @@ def batch_study(steps, warmup, sequential_steps=20):
           f"{description['total_beads']} beads)", file=sys.stderr, flush=True)
     batch = Batch(replicas)
     tune, best = batch.autotune(20)
-    print(f"bench: autotune ms/step {tune} -> {best} threads", file=sys.stderr, flush=True)
+    print(f"bench: autotune ms/step {tune} -> {best} (fused, coupling) threads", file=sys.stderr, flush=True)
     batch.step(warmup)
     batch.synchronize()
     t0 = time.perf_counter()
@@ -257,8 +257,8 @@ def batch_study(steps, warmup, sequential_steps=20):
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": dict(description, autotune_ms_per_step={str(k): round(v, 4) for k, v in tune.items()},
-                       threads=best),
+        "config": dict(description, autotune_ms_per_step={f"{f}x{c}": round(v, 4) for (f, c), v in tune.items()},
+                       fused_threads=best[0], coupling_threads=best[1]),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved/HBM_PEAK_GBS, 4), "traffic": None, "kernel": "fusedBatch",
                      "kernel_ms": round(fused_ms, 4), "bytes_per_node": bpn,

@@ -29,10 +29,12 @@ typedef struct lbm_batch lbm_batch;
 int lbm_batch_create(int num_replicas, const lbm_params* params, const int* num_particles, const double* masses,
         const double* positions, const double* velocities, lbm_batch** out);
 
-/* Time each work-group size of the fused pass in three interleaved rounds of steps_per_trial steps (the state
-   advances) and keep the fastest.  times_ms (3 entries, may be NULL) receives the best round's time per step of 256,
-   512 and 1024 threads; *best the choice. */
-int lbm_batch_autotune(lbm_batch* batch, int steps_per_trial, double* times_ms, int* best);
+/* Time each tile shape, the nodes per work-group of the lattice pass (256, 512, 1024) times the threads per
+   work-group of the coupling (64, 128, 256: particles per tile = threads/lanes, 1, 8, 27 or 64 lanes per particle by
+   the stencil), in three interleaved rounds of steps_per_trial steps (the state advances), and keep the fastest.
+   times_ms (9 entries, may be NULL) receives the best round's time per step, lattice-pass size major; *best_fused and
+   *best_couple (may be NULL) the choice. */
+int lbm_batch_autotune(lbm_batch* batch, int steps_per_trial, double* times_ms, int* best_fused, int* best_couple);
 
 /* Advance every replica by num_steps steps (asynchronous; replays the captured graph). */
 int lbm_batch_step(lbm_batch* batch, int num_steps);

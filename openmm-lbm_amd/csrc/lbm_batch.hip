@@ -66,6 +66,7 @@ struct lbm_batch {
     uint16_t* tileReplica = nullptr;
     uint32_t numTiles = 0;
     int threads = FUSED_THREADS;
+    int coupleThreads = BLOCK;              // work-group size of the coupling (particles per tile: coupleThreads/lanes)
     int numParticles = 0;
     uint16_t* particleReplica = nullptr;
     uint16_t* particleTiles = nullptr;      // replica of each work-group of the coupling
@@ -86,6 +87,27 @@ struct lbm_batch {
 
 namespace {
 
+/* The coupling's tile table for work-groups of `threads` threads: threads/lanes particles of one replica each. */
+void buildParticleTiles(lbm_batch* b, int threads) {
+    std::vector<uint16_t> tiles;
+    int perTile = threads/coupleLanes(b->stencil);
+    for (int r = 0; r < b->numReplicas; r++) {
+        b->desc[r].particleTileBegin = (uint32_t) tiles.size();
+        int count = (b->desc[r].part.numParticles + perTile - 1)/perTile;
+        tiles.insert(tiles.end(), count, (uint16_t) r);
+    }
+    if (b->particleTiles)
+        hipCheck(hipFree(b->particleTiles), "hipFree");
+    b->particleTiles = nullptr;
+    b->numParticleTiles = (int) tiles.size();
+    if (!tiles.empty()) {
+        b->particleTiles = alloc<uint16_t>(tiles.size(), "particle tiles");
+        hipCheck(hipMemcpy(b->particleTiles, tiles.data(), tiles.size()*sizeof(uint16_t), hipMemcpyHostToDevice),
+                "upload particle tiles");
+    }
+    b->coupleThreads = threads;
+}
+
 void buildTiles(lbm_batch* b, int threads) {
     std::vector<uint16_t> tiles;
     for (int r = 0; r < b->numReplicas; r++) {
@@ -104,8 +126,8 @@ void buildTiles(lbm_batch* b, int threads) {
 }
 
 void launchStep(lbm_batch* b, bool timed = false) {
-    launchCoupleBatch(b->stencil, b->dDesc, b->particleTiles, b->numParticleTiles, b->pos, b->vel, b->mass, b->force,
-            b->stepCounter, b->stream);
+    launchCoupleBatch(b->stencil, b->coupleThreads, b->dDesc, b->particleTiles, b->numParticleTiles, b->pos, b->vel,
+            b->mass, b->force, b->stepCounter, b->stream);
     std::pair<hipEvent_t, hipEvent_t> ev{nullptr, nullptr};
     if (timed) {
         hipCheck(hipEventCreate(&ev.first), "event");
@@ -212,7 +234,7 @@ void createBatch(lbm_batch* b, int R, const lbm_params* params, const int* numPa
     hipCheck(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking), "stream");
     b->desc.resize(R);
     int particleBegin = 0;
-    std::vector<uint16_t> particleReplica, particleTiles;
+    std::vector<uint16_t> particleReplica;
     std::vector<double> latticeMass, invMass;
     for (int r = 0; r < R; r++) {
         const lbm_params& p = params[r];
@@ -280,20 +302,10 @@ void createBatch(lbm_batch* b, int R, const lbm_params* params, const int* numPa
             latticeMass.push_back(m/cellMass);
             invMass.push_back(1.0/m);
         }
-        // Coupling work-groups of this replica: BLOCK/lanes particles each.
-        int perTile = BLOCK/coupleLanes(b->stencil);
-        int tiles = (numParticles[r] + perTile - 1)/perTile;
-        d.particleTileBegin = (uint32_t) particleTiles.size();
-        particleTiles.insert(particleTiles.end(), tiles, (uint16_t) r);
         particleBegin += numParticles[r];
         b->totalNodes += (int64_t) n;
     }
     b->numParticles = particleBegin;
-    b->numParticleTiles = (int) particleTiles.size();
-    b->particleTiles = alloc<uint16_t>(particleTiles.size(), "particle tiles");
-    if (!particleTiles.empty())
-        hipCheck(hipMemcpyAsync(b->particleTiles, particleTiles.data(), particleTiles.size()*sizeof(uint16_t),
-                hipMemcpyHostToDevice, b->stream), "upload");
     size_t np = (size_t) b->numParticles;
     b->dDesc = alloc<ReplicaDesc>(R, "replica table");
     b->particleReplica = alloc<uint16_t>(np, "particle table");
@@ -312,6 +324,7 @@ void createBatch(lbm_batch* b, int R, const lbm_params* params, const int* numPa
     }
     hipCheck(hipMemsetAsync(b->stepCounter, 0, sizeof(int64_t), b->stream), "memset");
     hipCheck(hipStreamSynchronize(b->stream), "sync");
+    buildParticleTiles(b, BLOCK);
     buildTiles(b, FUSED_THREADS);
     buildGraph(b);
 }
@@ -353,20 +366,26 @@ int lbm_batch_create(int R, const lbm_params* params, const int* numParticles, c
     return LBM_OK;
 }
 
-int lbm_batch_autotune(lbm_batch* b, int stepsPerTrial, double* timesMs, int* best) {
+int lbm_batch_autotune(lbm_batch* b, int stepsPerTrial, double* timesMs, int* bestFused, int* bestCouple) {
     BATCH_TRY({
-        // The candidates are timed in interleaved rounds and each keeps its best round, so that drift of the clocks
-        // or of the memory system during the trials does not favour one of them.
-        const int candidates[3] = {256, 512, 1024};
+        // Tile shapes: nodes per work-group of the lattice pass (256, 512, 1024 threads, one node each) times
+        // particles per work-group of the coupling (64, 128, 256 threads: a particle per 1, 8, 27 or 64 lanes by the
+        // stencil).  The candidates are timed in interleaved rounds and each keeps its best round, so that drift of
+        // the clocks or of the memory system during the trials does not favour one of them.
+        const int fused[3] = {256, 512, 1024};
+        const int couple[3] = {64, 128, BLOCK};
         const int rounds = 3;
-        double times[3] = {1e300, 1e300, 1e300};
+        double times[9];
+        for (double& t : times)
+            t = 1e300;
         hipEvent_t e0, e1;
         hipCheck(hipEventCreate(&e0), "event");
         hipCheck(hipEventCreate(&e1), "event");
         int trials = std::max(stepsPerTrial, STEPS_PER_GRAPH);
         for (int r = 0; r < rounds; r++)
-            for (int c = 0; c < 3; c++) {
-                buildTiles(b, candidates[c]);
+            for (int c = 0; c < 9; c++) {
+                buildParticleTiles(b, couple[c % 3]);
+                buildTiles(b, fused[c/3]);
                 buildGraph(b);
                 stepBatch(b, STEPS_PER_GRAPH);             // warm-up
                 hipCheck(hipEventRecord(e0, b->stream), "event");
@@ -377,22 +396,22 @@ int lbm_batch_autotune(lbm_batch* b, int stepsPerTrial, double* timesMs, int* be
                 hipCheck(hipEventElapsedTime(&ms, e0, e1), "event");
                 times[c] = std::min(times[c], (double) ms/trials);
             }
-        int bestThreads = candidates[0];
-        double bestTime = times[0];
-        for (int c = 0; c < 3; c++) {
+        int best = 0;
+        for (int c = 0; c < 9; c++) {
             if (timesMs)
                 timesMs[c] = times[c];
-            if (times[c] < bestTime) {
-                bestTime = times[c];
-                bestThreads = candidates[c];
-            }
+            if (times[c] < times[best])
+                best = c;
         }
         (void) hipEventDestroy(e0);
         (void) hipEventDestroy(e1);
-        buildTiles(b, bestThreads);
+        buildParticleTiles(b, couple[best % 3]);
+        buildTiles(b, fused[best/3]);
         buildGraph(b);
-        if (best)
-            *best = bestThreads;
+        if (bestFused)
+            *bestFused = fused[best/3];
+        if (bestCouple)
+            *bestCouple = couple[best % 3];
     })
 }
 

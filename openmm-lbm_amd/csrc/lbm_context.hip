@@ -156,6 +156,14 @@ struct lbm_context {
     double* stageForce = nullptr;
     // The processing order of the stencil coupling (explicit drag).
     CoupleOrder* order = nullptr;
+    // The one-writer spread of the explicit stencil coupling (spreadTiles), where the lattice allows it: its bins, and
+    // which of the two count arrays the next lattice step fills.
+    bool binned = false;
+    SpreadBins bins;
+    // Dense particles interpolate from the moments of every node (momentField) rather than pulling per slot.
+    double4* momentField = nullptr;
+    int* binCounts[2] = {nullptr, nullptr};
+    int binParity = 0;
     // Set by lbm_execute_device for the duration of one call: the caller's force array, into which the stencil coupling
     // adds the forces of the coupled particles itself (forcesAdded then says so, and no separate addForces runs).
     double* forceTarget = nullptr;
@@ -359,8 +367,19 @@ void couple(lbm_context* c, const double* dPos, const double* dVel, const double
     catch (const std::runtime_error& e) {
         throw Error(LBM_ERROR_RUNTIME, e.what());
     }
+    const SpreadBins* bins = nullptr;
+    if (c->binned && isStep) {
+        c->bins.count = c->binCounts[c->binParity];
+        c->bins.nextCount = c->binCounts[c->binParity ^ 1];
+        c->binParity ^= 1;
+        bins = &c->bins;
+    }
+    if (c->momentField != nullptr)
+        launchMomentField(c->p.storage, c->popA, c->mask, a, c->ucm, c->momentField, c->stream);
     launchCouple(c->p.storage, c->popA, c->mask, dPos, dVel, c->particles, order, c->mass, c->ucm, a, c->particleForce,
-            c->reactionFixed, c->touched, partials, c->forceTarget, c->stream);
+            c->reactionFixed, c->touched, partials, c->forceTarget, bins, c->momentField, c->stream);
+    if (bins != nullptr)
+        launchSpreadTiles(dPos, c->particles, c->mask, a, *bins, c->reactionFixed, c->touched, c->stream);
     c->forcesAdded = c->forceTarget != nullptr;
 }
 
@@ -489,7 +508,8 @@ void destroy(lbm_context* c) {
     centredDestroy(c->centred);
     coupleOrderDestroy(c->order);
     void* buffers[] = {c->popA, c->popB, c->mask, c->wallNodes, c->wallLinks, c->boundary, c->reducePartials, c->totals,
-                       c->overflow,
+                       c->overflow, c->binCounts[0], c->binCounts[1], c->bins.beads, c->bins.anchor, c->bins.force,
+                       c->bins.slotNode, c->bins.slotWeight, c->momentField,
                        c->ucm, c->wallPartials, c->particles, c->mass, c->particleForce, c->reactionFixed,
                        c->touched, c->stagePos, c->stageVel, c->closureX, c->closureNode, c->closureU, c->closureDu,
                        c->closureOod, c->closureCounters, c->closureField, c->stageForce};
@@ -639,6 +659,40 @@ void create(const lbm_params* params, const int* solid, int numSolid, const int*
         }
         catch (const std::runtime_error& e) {
             throw Error(LBM_ERROR_RUNTIME, e.what());
+        }
+        // The one-writer spread (spreadTiles) for the explicit drag on lattices of whole tiles with periodic faces.
+        // LBM_AMD_SPREAD=atomic or =tiles forces the choice where it fits; by default the tiles serve dense particles
+        // (the stencils cover more than a quarter of the lattice), whose concurrent atomics contend.
+        const char* spreadEnv = getenv("LBM_AMD_SPREAD");
+        bool anyOpen = c->openAxis(0) || c->openAxis(1) || c->openAxis(2);
+        bool fit = p.drag_scheme == LBM_DRAG_EXPLICIT && spreadTilesFit(p.nx, p.ny, p.nz, anyOpen);
+        bool want = spreadEnv ? std::string(spreadEnv) == "tiles" : c->dense;
+        // The moment field: by default for dense particles with the explicit drag; LBM_AMD_FIELD=0 or 1 forces it.
+        const char* fieldEnv = getenv("LBM_AMD_FIELD");
+        bool field = fieldEnv ? fieldEnv[0] == '1' : c->dense;
+        if (p.drag_scheme == LBM_DRAG_EXPLICIT && field)
+            c->momentField = deviceAlloc<double4>(n);
+        if (fit && want) {
+            SpreadBins& b = c->bins;
+            b.ntx = (uint32_t) (p.nx/SPREAD_TX);
+            b.nty = (uint32_t) (p.ny/SPREAD_TY);
+            b.ntz = (uint32_t) (p.nz/SPREAD_TZ);
+            size_t tiles = (size_t) b.ntx*b.nty*b.ntz;
+            // Room for four times the mean number of beads per tile and 16 more; a bead beyond it spreads itself with
+            // the atomics.
+            double mean = (double) numParts/(double) tiles;
+            b.cap = std::min(SPREAD_MAX_CAP, ((int) std::ceil(4.0*mean) + 16 + 15)/16*16);
+            c->binCounts[0] = deviceAlloc<int>(tiles);
+            c->binCounts[1] = deviceAlloc<int>(tiles);
+            HIP_CHECK(hipMemsetAsync(c->binCounts[0], 0, tiles*sizeof(int), c->stream));
+            HIP_CHECK(hipMemsetAsync(c->binCounts[1], 0, tiles*sizeof(int), c->stream));
+            b.beads = deviceAlloc<int>(tiles*(size_t) b.cap);
+            b.anchor = deviceAlloc<uint16_t>(tiles*(size_t) b.cap);
+            b.force = deviceAlloc<double>(3*(size_t) numParts);
+            int lanes = coupleLanes(p.interpolation_stencil);
+            b.slotNode = deviceAlloc<uint32_t>((size_t) numParts*lanes);
+            b.slotWeight = deviceAlloc<double>((size_t) numParts*lanes);
+            c->binned = true;
         }
     }
     // The fluid at equilibrium with the initial velocity; solid nodes hold no fluid.

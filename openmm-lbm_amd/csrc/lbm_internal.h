@@ -84,6 +84,29 @@ struct ParticleArgs {
  */
 constexpr double REACTION_LIMIT = 64.0;
 
+/*
+ * The bins of the one-writer spread of the particle reactions (spreadTiles): the lattice in tiles of 32 x 4 x 4 nodes
+ * (x fastest; a tile row of 32 nodes is one word of the touched mask); per tile the indices of up to cap beads whose
+ * stencil starts in it (their anchor node), and their count.  Two count arrays alternate: the spread of one step
+ * clears the one the next step fills.  force: each binned bead's coupling force, lattice units.
+ */
+constexpr int SPREAD_TX = 32, SPREAD_TY = 4, SPREAD_TZ = 4;
+constexpr int SPREAD_THREADS = SPREAD_TX*SPREAD_TY*SPREAD_TZ;
+constexpr int SPREAD_MAX_CAP = 128;
+
+struct SpreadBins {
+    int* count = nullptr;
+    int* nextCount = nullptr;
+    int* beads = nullptr;
+    uint16_t* anchor = nullptr;      // [tile][cap]: the bead's first stencil node within its tile, x | y << 5 | z << 7
+    double* force = nullptr;
+    uint32_t* slotNode = nullptr;    // [bead][lane]: the fluid node of each stencil slot (NO_NODE for solid or unused)
+    double* slotWeight = nullptr;    // [bead][lane]: its weight
+    int cap = 0;
+    uint32_t ntx = 0, nty = 0, ntz = 0;
+};
+constexpr uint32_t NO_NODE = 0xFFFFFFFFu;
+
 __device__ __forceinline__ bool reactionInRange(double r, unsigned int* overflow) {
     if (fabs(r) < REACTION_LIMIT)
         return true;
@@ -129,8 +152,8 @@ struct ReplicaDesc {
 /* Host-side launchers (defined in lbm_fluid.hip / lbm_particles.hip). */
 void launchFusedBatch(int threads, bool fluct, const ReplicaDesc* table, const uint16_t* tileReplica, uint32_t numTiles,
         const int64_t* stepCounter, hipStream_t stream);
-void launchCoupleBatch(int stencil, const ReplicaDesc* table, const uint16_t* particleTiles, int numParticleTiles,
-        const double* positions, const double* velocities, const double* mass, double* forces,
+void launchCoupleBatch(int stencil, int threads, const ReplicaDesc* table, const uint16_t* particleTiles,
+        int numParticleTiles, const double* positions, const double* velocities, const double* mass, double* forces,
         const int64_t* stepCounter, hipStream_t stream);
 int coupleLanes(int stencil);
 void launchVerletBatch(const ReplicaDesc* table, const uint16_t* particleReplica, int numParticles, double* positions,
@@ -174,7 +197,16 @@ void launchReflect(const double* positions, double* velocities, const int* parti
 void launchCouple(int storage, const void* src, const uint32_t* mask, const double* positions, const double* velocities,
         const int* particles, const int* order, const double* mass, const double* ucm, const ParticleArgs& a,
         double* particleForce, unsigned long long* reactionFixed, uint32_t* touched, double* partials,
-        double* systemForces, hipStream_t stream);
+        double* systemForces, const SpreadBins* bins, const double4* field, hipStream_t stream);
+/* The moments (jx, jy, jz, rho) of every fluid node into field, for launchCouple's dense path. */
+void launchMomentField(int storage, const void* src, const uint32_t* mask, const ParticleArgs& a, const double* ucm,
+        double4* field, hipStream_t stream);
+/* The one-writer spread of the binned beads' reactions, one work-group per tile (after launchCouple with bins). */
+void launchSpreadTiles(const double* positions, const int* particles, const uint32_t* mask, const ParticleArgs& a,
+        const SpreadBins& bins, unsigned long long* reactionFixed, uint32_t* touched, hipStream_t stream);
+/* Whether the lattice suits the tile bins: every axis periodic, nx a multiple of 32 and ny, nz of 4, at least two tiles
+   along each axis. */
+bool spreadTilesFit(int nx, int ny, int nz, bool anyOpen);
 
 /*
  * The coupled particles sorted by nearest node (z slowest, descending): processed in this order, the particles of one

@@ -77,12 +77,101 @@ __global__ void __launch_bounds__(BLOCK) reflectParticles(const double* __restri
 }
 
 /* ------------------------------------------------------------------------------------------ */
-template <typename S, int LANES>
+/* The stencil width along one axis of the stencil that a group of LANES lanes couples with. */
+template <int LANES>
+constexpr int groupWidth() {
+    return LANES == 64 ? 4 : LANES == 32 ? 3 : LANES == 8 ? 2 : 1;
+}
+
+/* The first node of a stencil along one axis (the rule of axisSlot), wrapped into [0, n). */
+template <int STENCIL>
+__device__ __forceinline__ int anchorIndex(double s, int n) {
+    int first = STENCIL == LBM_STENCIL_NEAREST ? nearestIndex(s, n) :
+                STENCIL == LBM_STENCIL_THREE_POINT ? (int) floor(s + 0.5) - 1 :
+                STENCIL == LBM_STENCIL_KEYS ? (int) floor(s) - 1 : (int) floor(s);
+    return first < 0 ? first + n : (first >= n ? first - n : first);
+}
+
+/* The bin of a particle at x: the tile of its stencil's first node; *offset receives that node's place in the tile
+   (x | y << 5 | z << 7). */
+template <int STENCIL>
+__device__ __forceinline__ uint32_t anchorTile(const double* x, const ParticleArgs& a, const SpreadBins& bins,
+        uint16_t* offset) {
+    const int size[3] = {(int) a.g.nx, (int) a.g.ny, (int) a.g.nz};
+    const int tileSize[3] = {SPREAD_TX, SPREAD_TY, SPREAD_TZ};
+    uint32_t t[3], o[3];
+    for (int k = 0; k < 3; k++) {
+        int first = anchorIndex<STENCIL>(wrapCoordinate(x[k], a.dx, size[k]), size[k]);
+        t[k] = (uint32_t) (first/tileSize[k]);
+        o[k] = (uint32_t) (first - (int) t[k]*tileSize[k]);
+    }
+    *offset = (uint16_t) (o[0] | (o[1] << 5) | (o[2] << 7));
+    return t[0] + bins.ntx*(t[1] + bins.nty*t[2]);
+}
+
+/*
+ * finishGroup for the one-writer spread: the drag force as in finishGroup; in a lattice step the reactions on solid
+ * nodes go to the wall as there, and the particle goes into the bin of its anchor tile with its force, for
+ * spreadTiles.  A particle whose bin is full spreads its reactions here with the fixed-point atomics instead (the sums
+ * are integers, so both ways give the same bits).
+ */
+template <int LANES>
+__device__ __forceinline__ void finishGroupBinned(const StencilSlot& sl, const double u[3], const double* vel,
+        const double* x, uint32_t index, double m, bool active, int lane, const ParticleArgs& a, double* forceOut,
+        unsigned long long* __restrict__ reactionFixed, uint32_t* __restrict__ touched, double wallAcc[3],
+        const double* noise, double* systemForce, const SpreadBins& bins) {
+    double f[3] = {0, 0, 0};
+    if (active) {
+        dragForce(vel, u, index, m, a, f, noise);
+        if (lane == 0)
+            for (int k = 0; k < 3; k++) {
+                forceOut[k] = f[k]*a.forceScale;
+                if (systemForce != nullptr)
+                    atomicAdd(systemForce + k, f[k]*a.forceScale);
+            }
+    }
+    if (!a.isStep)
+        return;
+    int full = 0;
+    if (active && lane == 0) {
+        uint16_t offset;
+        uint32_t tile = anchorTile<groupStencil<LANES>()>(x, a, bins, &offset);
+        int slot = atomicAdd(bins.count + tile, 1);
+        if (slot < bins.cap) {
+            bins.beads[(size_t) tile*bins.cap + slot] = (int) index;
+            bins.anchor[(size_t) tile*bins.cap + slot] = offset;
+            for (int k = 0; k < 3; k++)
+                bins.force[3*(size_t) index + k] = f[k];
+        }
+        else
+            full = 1;
+    }
+    full = __shfl(full, 0, LANES);
+    if (active && !full) {
+        // The slots for spreadTiles: the node (fluid nodes only) and the weight of each lane.
+        size_t at = (size_t) index*LANES + lane;
+        bins.slotNode[at] = sl.inStencil && !sl.solid ? sl.node : NO_NODE;
+        bins.slotWeight[at] = sl.weight;
+    }
+    bool flagged = false;
+    if (sl.inStencil) {
+        if (full)
+            flagged = spreadReaction(sl, f, a, reactionFixed, wallAcc);
+        else if (sl.solid)
+            for (int k = 0; k < 3; k++)
+                wallAcc[k] = -sl.weight*f[k];
+    }
+    if (__any(full))
+        markTouched<groupWidth<LANES>()>(touched, sl.node, flagged, lane);
+}
+
+template <typename S, int LANES, bool BINNED>
 __global__ void __launch_bounds__(BLOCK) coupleParticles(const S* __restrict__ src, const uint32_t* __restrict__ mask,
         const double* __restrict__ positions, const double* __restrict__ velocities, const int* __restrict__ particles,
         const int* __restrict__ order, const double* __restrict__ mass, const double* __restrict__ ucm, ParticleArgs a,
         double* __restrict__ particleForce, unsigned long long* __restrict__ reactionFixed, uint32_t* __restrict__ touched,
-        double* __restrict__ partials, double* __restrict__ systemForces) {
+        double* __restrict__ partials, double* __restrict__ systemForces, SpreadBins bins,
+        const double4* __restrict__ field) {
     constexpr int GROUPS = BLOCK/LANES;
     int lane = threadIdx.x % LANES;
     int rank = blockIdx.x*GROUPS + threadIdx.x/LANES;    // position in the processing order
@@ -91,7 +180,7 @@ __global__ void __launch_bounds__(BLOCK) coupleParticles(const S* __restrict__ s
     int part = active ? particles[i] : 0;
     double m = active ? mass[i] : 0.0;
     double u[3];
-    StencilSlot sl = interpolateGroup<S, LANES>(src, mask, positions + 3*(size_t) part, active, lane, ucm, a, u);
+    StencilSlot sl = interpolateGroup<S, LANES>(src, mask, positions + 3*(size_t) part, active, lane, ucm, a, u, field);
     // The normals of the work-group's particles, drawn once by its first GROUPS lanes instead of by every group: a
     // wavefront pays for the FP64 Box-Muller once however many of its lanes need it.  Drawn after the interpolation,
     // so that the first wavefront's loads are in flight meanwhile.
@@ -110,22 +199,212 @@ __global__ void __launch_bounds__(BLOCK) coupleParticles(const S* __restrict__ s
         myNoise = noise[threadIdx.x/LANES];
     }
     double wallAcc[3] = {0, 0, 0};
-    finishGroup<LANES>(sl, u, velocities + 3*(size_t) part, (uint32_t) i, m, active, lane, a,
-            particleForce + 3*(size_t) (active ? i : 0), reactionFixed, touched, wallAcc, myNoise,
-            systemForces != nullptr ? systemForces + 3*(size_t) part : nullptr);
+    if constexpr (BINNED)
+        finishGroupBinned<LANES>(sl, u, velocities + 3*(size_t) part, positions + 3*(size_t) part, (uint32_t) i, m, active,
+                lane, a, particleForce + 3*(size_t) (active ? i : 0), reactionFixed, touched, wallAcc, myNoise,
+                systemForces != nullptr ? systemForces + 3*(size_t) part : nullptr, bins);
+    else
+        finishGroup<LANES>(sl, u, velocities + 3*(size_t) part, (uint32_t) i, m, active, lane, a,
+                particleForce + 3*(size_t) (active ? i : 0), reactionFixed, touched, wallAcc, myNoise,
+                systemForces != nullptr ? systemForces + 3*(size_t) part : nullptr);
     if (partials != nullptr)
         storeWallPartial<BLOCK>(wallAcc, partials);
 }
 
-/* Batched replicas: the coupling of every particle of every replica in one launch.  Each work-group couples up to
-   BLOCK/LANES particles of one replica (particleTiles[blockIdx.x] names it), so the replica's descriptor is uniform
-   across the work-group and read with scalar loads. */
+/*
+ * The one-writer spread: one work-group per tile of 32 x 4 x 4 nodes, one thread per node.  The beads that can reach
+ * the tile are those binned in it and in its neighbours one tile back along each axis (a stencil is at most 4 nodes
+ * wide, no wider than a tile); each wavefront takes one of them at a time, one lane per stencil slot, and adds the
+ * fixed-point reactions -w_j F of its slots inside the tile (fluid nodes only: the wall's part was taken by the
+ * coupling) to the tile's accumulators in LDS.  Then each node's sums are written by its own thread, and each tile row's
+ * touched-mask word by the row's first lane: no global atomics.  The slots, weights and fixed-point values are those of
+ * the atomic spread (spreadReaction), and integer sums do not depend on their order, so both give the same bits.  The
+ * reference builds its per-node sums from a sort of the stencil slots by node (lbmCoupling.cc:251-272, 1245-1266).
+ */
 template <int LANES>
-__global__ void __launch_bounds__(BLOCK) coupleBatch(const ReplicaDesc* __restrict__ table,
+__global__ void __launch_bounds__(SPREAD_THREADS) spreadTiles(ParticleArgs a, SpreadBins bins,
+        unsigned long long* __restrict__ reactionFixed, uint32_t* __restrict__ touched) {
+    __shared__ unsigned long long acc[3*SPREAD_THREADS];
+    __shared__ int candidates[8*SPREAD_MAX_CAP];
+    __shared__ double candidateForce[8*SPREAD_MAX_CAP][3];
+    __shared__ int counts[8], firsts[8];
+    __shared__ int numCandidates;
+    const uint32_t tile = blockIdx.x;
+    const uint32_t tx = tile % bins.ntx, ty = (tile/bins.ntx) % bins.nty, tz = tile/(bins.ntx*bins.nty);
+    const int t = (int) threadIdx.x;
+    if (t < 8) {
+        uint32_t fx = (t & 1) ? (tx == 0 ? bins.ntx - 1 : tx - 1) : tx;
+        uint32_t fy = (t & 2) ? (ty == 0 ? bins.nty - 1 : ty - 1) : ty;
+        uint32_t fz = (t & 4) ? (tz == 0 ? bins.ntz - 1 : tz - 1) : tz;
+        uint32_t from = fx + bins.ntx*(fy + bins.nty*fz);
+        counts[t] = min(bins.count[from], bins.cap);
+        firsts[t] = (int) from;
+    }
+    if (t == 0)
+        bins.nextCount[tile] = 0;
+    __syncthreads();
+    int total = 0;
+    for (int d = 0; d < 8; d++)
+        total += counts[d];
+    if (total == 0)
+        return;
+    // The candidates that reach into this tile, gathered by every thread: a bead binned one tile back along an axis
+    // reaches it when its first stencil node lies within W - 1 nodes of that tile's end.
+    constexpr int W = groupWidth<LANES>();
+    if (t == 0)
+        numCandidates = 0;
+    for (int k = 0; k < 3; k++)
+        acc[k*SPREAD_THREADS + t] = 0ull;
+    __syncthreads();
+    for (int c = t; c < total; c += SPREAD_THREADS) {
+        int r = c, d = 0;
+        while (r >= counts[d]) {
+            r -= counts[d];
+            d++;
+        }
+        const size_t entry = (size_t) firsts[d]*bins.cap + r;
+        const uint32_t off = bins.anchor[entry];
+        const bool reaches = (!(d & 1) || (int) (off & 31u) >= SPREAD_TX - W + 1) &&
+                             (!(d & 2) || (int) ((off >> 5) & 3u) >= SPREAD_TY - W + 1) &&
+                             (!(d & 4) || (int) ((off >> 7) & 3u) >= SPREAD_TZ - W + 1);
+        if (reaches) {
+            const int at = atomicAdd(&numCandidates, 1);
+            const int i = bins.beads[entry];
+            candidates[at] = i;
+            for (int k = 0; k < 3; k++)
+                candidateForce[at][k] = bins.force[3*(size_t) i + k];
+        }
+    }
+    __syncthreads();
+    const int lo[3] = {(int) tx*SPREAD_TX, (int) ty*SPREAD_TY, (int) tz*SPREAD_TZ};
+    const int lane = t & 63;
+    const int items = numCandidates*LANES;
+    // One item per (candidate, slot); each thread loads UNROLL items before it adds any, so that their loads overlap.
+    constexpr int UNROLL = 4;
+    for (int base = t; base < items; base += SPREAD_THREADS*UNROLL) {
+        uint32_t node[UNROLL];
+        double weight[UNROLL], f[UNROLL][3];
+#pragma unroll
+        for (int u = 0; u < UNROLL; u++) {
+            const int item = base + u*SPREAD_THREADS;
+            node[u] = NO_NODE;
+            if (item < items) {
+                const int c = item/LANES;
+                const size_t at = (size_t) candidates[c]*LANES + item % LANES;
+                node[u] = bins.slotNode[at];
+                weight[u] = bins.slotWeight[at];
+                for (int k = 0; k < 3; k++)
+                    f[u][k] = candidateForce[c][k];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < UNROLL; u++) {
+            if (node[u] == NO_NODE)
+                continue;
+            uint32_t ix, iy, iz;
+            nodeCoords(a.g, node[u], ix, iy, iz);
+            const int lx = (int) ix - lo[0], ly = (int) iy - lo[1], lz = (int) iz - lo[2];
+            if (lx < 0 || lx >= SPREAD_TX || ly < 0 || ly >= SPREAD_TY || lz < 0 || lz >= SPREAD_TZ)
+                continue;
+            const int local = lx + SPREAD_TX*(ly + SPREAD_TY*lz);
+            for (int k = 0; k < 3; k++) {
+                double r = -weight[u]*f[u][k];
+                if (!reactionInRange(r, a.overflow))
+                    continue;
+                long long fixed = llrint(r*FIXED_SCALE);
+                if (fixed != 0)
+                    atomicAdd(&acc[k*SPREAD_THREADS + local], (unsigned long long) fixed);
+            }
+        }
+    }
+    __syncthreads();
+    const int lx = t % SPREAD_TX, ly = (t/SPREAD_TX) % SPREAD_TY, lz = t/(SPREAD_TX*SPREAD_TY);
+    const uint32_t node = (uint32_t) (lo[0] + lx) + a.g.nx*((uint32_t) (lo[1] + ly) + a.g.ny*(uint32_t) (lo[2] + lz));
+    const unsigned long long v[3] = {acc[t], acc[SPREAD_THREADS + t], acc[2*SPREAD_THREADS + t]};
+    const bool any = (v[0] | v[1] | v[2]) != 0ull;
+    const uint64_t ballot = __ballot(any);
+    const uint32_t rowBits = (uint32_t) (ballot >> (lane & 32));
+    uint32_t old = 0;
+    if (lx == 0 && rowBits != 0u)
+        old = touched[node >> 5];            // bits of beads that spread themselves (full bins)
+    old = __shfl(old, lane & 32, 64);
+    if (any) {
+        const bool add = (old >> lx) & 1u;
+        for (int k = 0; k < 3; k++) {
+            size_t at = reactionIndex(node, k, a.g.n, a.reactionPlanes);
+            reactionFixed[at] = add ? reactionFixed[at] + v[k] : v[k];
+        }
+    }
+    if (lx == 0 && rowBits != 0u)
+        touched[node >> 5] = old | rowBits;
+}
+
+template <typename S>
+__global__ void __launch_bounds__(BLOCK) momentField(const S* __restrict__ src, const uint32_t* __restrict__ mask,
+        Geometry g, const double* __restrict__ ucm, int removal, double4* __restrict__ field) {
+    for (uint32_t node = blockIdx.x*BLOCK + threadIdx.x; node < g.n; node += gridDim.x*BLOCK) {
+        if (solidAt(mask, node))
+            continue;          // solid slots do not read the field
+        momentFieldNode(src, g, node, ucm, removal, field);
+    }
+}
+
+void launchMomentField(int storage, const void* src, const uint32_t* mask, const ParticleArgs& a, const double* ucm,
+        double4* field, hipStream_t stream) {
+    unsigned grid = (unsigned) std::min<uint64_t>((a.g.n + BLOCK - 1)/BLOCK, 65535u*4u);
+    switch (storage) {
+    case LBM_STORAGE_FP16:
+        momentField<__half><<<grid, BLOCK, 0, stream>>>((const __half*) src, mask, a.g, ucm, a.removal, field);
+        break;
+    case LBM_STORAGE_BF16:
+        momentField<__hip_bfloat16><<<grid, BLOCK, 0, stream>>>((const __hip_bfloat16*) src, mask, a.g, ucm, a.removal,
+                field);
+        break;
+    case LBM_STORAGE_FP64:
+        momentField<double><<<grid, BLOCK, 0, stream>>>((const double*) src, mask, a.g, ucm, a.removal, field);
+        break;
+    default:
+        momentField<float><<<grid, BLOCK, 0, stream>>>((const float*) src, mask, a.g, ucm, a.removal, field);
+        break;
+    }
+}
+
+bool spreadTilesFit(int nx, int ny, int nz, bool anyOpen) {
+    return !anyOpen && nx % SPREAD_TX == 0 && ny % SPREAD_TY == 0 && nz % SPREAD_TZ == 0 && nx >= 2*SPREAD_TX &&
+           ny >= 2*SPREAD_TY && nz >= 2*SPREAD_TZ;
+}
+
+void launchSpreadTiles(const double* positions, const int* particles, const uint32_t* mask, const ParticleArgs& a,
+        const SpreadBins& bins, unsigned long long* reactionFixed, uint32_t* touched, hipStream_t stream) {
+    (void) positions;
+    (void) particles;
+    (void) mask;
+    unsigned grid = bins.ntx*bins.nty*bins.ntz;
+    switch (coupleLanes(a.stencil)) {
+    case 64:
+        spreadTiles<64><<<grid, SPREAD_THREADS, 0, stream>>>(a, bins, reactionFixed, touched);
+        break;
+    case 32:
+        spreadTiles<32><<<grid, SPREAD_THREADS, 0, stream>>>(a, bins, reactionFixed, touched);
+        break;
+    case 8:
+        spreadTiles<8><<<grid, SPREAD_THREADS, 0, stream>>>(a, bins, reactionFixed, touched);
+        break;
+    default:
+        spreadTiles<1><<<grid, SPREAD_THREADS, 0, stream>>>(a, bins, reactionFixed, touched);
+        break;
+    }
+}
+
+/* Batched replicas: the coupling of every particle of every replica in one launch.  Each work-group of THREADS
+   threads couples up to THREADS/LANES particles of one replica (particleTiles[blockIdx.x] names it), so the replica's
+   descriptor is uniform across the work-group and read with scalar loads. */
+template <int LANES, int THREADS>
+__global__ void __launch_bounds__(THREADS) coupleBatch(const ReplicaDesc* __restrict__ table,
         const uint16_t* __restrict__ particleTiles, const double* __restrict__ positions,
         const double* __restrict__ velocities, const double* __restrict__ mass, double* __restrict__ forces,
         const int64_t* __restrict__ stepCounter) {
-    constexpr int GROUPS = BLOCK/LANES;
+    constexpr int GROUPS = THREADS/LANES;
     int lane = threadIdx.x % LANES;
     const ReplicaDesc* d = table + particleTiles[blockIdx.x];
     int local = (int) (blockIdx.x - d->particleTileBegin)*GROUPS + threadIdx.x/LANES;
@@ -216,77 +495,120 @@ int coupleBlocks(int stencil, int numParticles) {
     return (numParticles + groups - 1)/groups;
 }
 
-template <typename S>
-static void launchCoupleT(const void* src, const uint32_t* mask, const double* positions, const double* velocities,
+template <typename S, bool BINNED>
+static void launchCoupleB(const void* src, const uint32_t* mask, const double* positions, const double* velocities,
         const int* particles, const int* order, const double* mass, const double* ucm, const ParticleArgs& a,
         double* particleForce, unsigned long long* reactionFixed, uint32_t* touched, double* partials,
-        double* systemForces, hipStream_t stream) {
+        double* systemForces, const SpreadBins& bins, const double4* field, hipStream_t stream) {
     unsigned grid = (unsigned) coupleBlocks(a.stencil, a.numParticles);
     const S* s = (const S*) src;
     switch (coupleLanes(a.stencil)) {
     case 8:
-        coupleParticles<S, 8><<<grid, BLOCK, 0, stream>>>(s, mask, positions, velocities, particles, order, mass, ucm, a,
-                particleForce, reactionFixed, touched, partials, systemForces);
+        coupleParticles<S, 8, BINNED><<<grid, BLOCK, 0, stream>>>(s, mask, positions, velocities, particles, order, mass,
+                ucm, a, particleForce, reactionFixed, touched, partials, systemForces, bins, field);
         break;
     case 32:
-        coupleParticles<S, 32><<<grid, BLOCK, 0, stream>>>(s, mask, positions, velocities, particles, order, mass, ucm, a,
-                particleForce, reactionFixed, touched, partials, systemForces);
+        coupleParticles<S, 32, BINNED><<<grid, BLOCK, 0, stream>>>(s, mask, positions, velocities, particles, order, mass,
+                ucm, a, particleForce, reactionFixed, touched, partials, systemForces, bins, field);
         break;
     case 64:
-        coupleParticles<S, 64><<<grid, BLOCK, 0, stream>>>(s, mask, positions, velocities, particles, order, mass, ucm, a,
-                particleForce, reactionFixed, touched, partials, systemForces);
+        coupleParticles<S, 64, BINNED><<<grid, BLOCK, 0, stream>>>(s, mask, positions, velocities, particles, order, mass,
+                ucm, a, particleForce, reactionFixed, touched, partials, systemForces, bins, field);
         break;
     default:
-        coupleParticles<S, 1><<<grid, BLOCK, 0, stream>>>(s, mask, positions, velocities, particles, order, mass, ucm, a,
-                particleForce, reactionFixed, touched, partials, systemForces);
+        coupleParticles<S, 1, BINNED><<<grid, BLOCK, 0, stream>>>(s, mask, positions, velocities, particles, order, mass,
+                ucm, a, particleForce, reactionFixed, touched, partials, systemForces, bins, field);
         break;
     }
+}
+
+template <typename S>
+static void launchCoupleT(const void* src, const uint32_t* mask, const double* positions, const double* velocities,
+        const int* particles, const int* order, const double* mass, const double* ucm, const ParticleArgs& a,
+        double* particleForce, unsigned long long* reactionFixed, uint32_t* touched, double* partials,
+        double* systemForces, const SpreadBins* bins, const double4* field, hipStream_t stream) {
+    if (bins != nullptr && a.isStep)
+        launchCoupleB<S, true>(src, mask, positions, velocities, particles, order, mass, ucm, a, particleForce,
+                reactionFixed, touched, partials, systemForces, *bins, field, stream);
+    else
+        launchCoupleB<S, false>(src, mask, positions, velocities, particles, order, mass, ucm, a, particleForce,
+                reactionFixed, touched, partials, systemForces, SpreadBins(), field, stream);
 }
 
 void launchCouple(int storage, const void* src, const uint32_t* mask, const double* positions, const double* velocities,
         const int* particles, const int* order, const double* mass, const double* ucm, const ParticleArgs& a,
         double* particleForce, unsigned long long* reactionFixed, uint32_t* touched, double* partials,
-        double* systemForces, hipStream_t stream) {
+        double* systemForces, const SpreadBins* bins, const double4* field, hipStream_t stream) {
     if (a.numParticles == 0)
         return;
+    if (field != nullptr) {
+        // The moment field replaces the pulls: the storage type no longer matters.
+        launchCoupleT<float>(nullptr, mask, positions, velocities, particles, order, mass, ucm, a, particleForce,
+                reactionFixed, touched, partials, systemForces, bins, field, stream);
+        return;
+    }
     switch (storage) {
     case LBM_STORAGE_FP16:
         launchCoupleT<__half>(src, mask, positions, velocities, particles, order, mass, ucm, a, particleForce, reactionFixed,
-                touched, partials, systemForces, stream);
+                touched, partials, systemForces, bins, nullptr, stream);
         break;
     case LBM_STORAGE_BF16:
         launchCoupleT<__hip_bfloat16>(src, mask, positions, velocities, particles, order, mass, ucm, a, particleForce,
-                reactionFixed, touched, partials, systemForces, stream);
+                reactionFixed, touched, partials, systemForces, bins, nullptr, stream);
         break;
     case LBM_STORAGE_FP64:
         launchCoupleT<double>(src, mask, positions, velocities, particles, order, mass, ucm, a, particleForce, reactionFixed,
-                touched, partials, systemForces, stream);
+                touched, partials, systemForces, bins, nullptr, stream);
         break;
     default:
         launchCoupleT<float>(src, mask, positions, velocities, particles, order, mass, ucm, a, particleForce, reactionFixed,
-                touched, partials, systemForces, stream);
+                touched, partials, systemForces, bins, nullptr, stream);
         break;
     }
 }
 
-void launchCoupleBatch(int stencil, const ReplicaDesc* table, const uint16_t* particleTiles, int numParticleTiles,
+template <int LANES>
+static void launchCoupleBatchT(int threads, unsigned grid, const ReplicaDesc* table, const uint16_t* particleTiles,
         const double* positions, const double* velocities, const double* mass, double* forces,
+        const int64_t* stepCounter, hipStream_t stream) {
+    switch (threads) {
+    case 64:
+        coupleBatch<LANES, 64><<<grid, 64, 0, stream>>>(table, particleTiles, positions, velocities, mass, forces,
+                stepCounter);
+        break;
+    case 128:
+        coupleBatch<LANES, 128><<<grid, 128, 0, stream>>>(table, particleTiles, positions, velocities, mass, forces,
+                stepCounter);
+        break;
+    default:
+        coupleBatch<LANES, BLOCK><<<grid, BLOCK, 0, stream>>>(table, particleTiles, positions, velocities, mass, forces,
+                stepCounter);
+        break;
+    }
+}
+
+void launchCoupleBatch(int stencil, int threads, const ReplicaDesc* table, const uint16_t* particleTiles,
+        int numParticleTiles, const double* positions, const double* velocities, const double* mass, double* forces,
         const int64_t* stepCounter, hipStream_t stream) {
     if (numParticleTiles == 0)
         return;
     unsigned grid = (unsigned) numParticleTiles;
     switch (coupleLanes(stencil)) {
     case 8:
-        coupleBatch<8><<<grid, BLOCK, 0, stream>>>(table, particleTiles, positions, velocities, mass, forces, stepCounter);
+        launchCoupleBatchT<8>(threads, grid, table, particleTiles, positions, velocities, mass, forces, stepCounter,
+                stream);
         break;
     case 32:
-        coupleBatch<32><<<grid, BLOCK, 0, stream>>>(table, particleTiles, positions, velocities, mass, forces, stepCounter);
+        launchCoupleBatchT<32>(threads, grid, table, particleTiles, positions, velocities, mass, forces, stepCounter,
+                stream);
         break;
     case 64:
-        coupleBatch<64><<<grid, BLOCK, 0, stream>>>(table, particleTiles, positions, velocities, mass, forces, stepCounter);
+        launchCoupleBatchT<64>(threads, grid, table, particleTiles, positions, velocities, mass, forces, stepCounter,
+                stream);
         break;
     default:
-        coupleBatch<1><<<grid, BLOCK, 0, stream>>>(table, particleTiles, positions, velocities, mass, forces, stepCounter);
+        launchCoupleBatchT<1>(threads, grid, table, particleTiles, positions, velocities, mass, forces, stepCounter,
+                stream);
         break;
     }
 }

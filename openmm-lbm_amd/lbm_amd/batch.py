@@ -19,7 +19,7 @@ _I64 = ctypes.POINTER(ctypes.c_int64)
 BATCH_FUNCTIONS = {
     "lbm_batch_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(_lib.LBMParams), _I, _D, _D, _D,
                                         ctypes.POINTER(_P)]),
-    "lbm_batch_autotune": (ctypes.c_int, [_P, ctypes.c_int, _D, _I]),
+    "lbm_batch_autotune": (ctypes.c_int, [_P, ctypes.c_int, _D, _I, _I]),
     "lbm_batch_step": (ctypes.c_int, [_P, ctypes.c_int]),
     "lbm_batch_synchronize": (ctypes.c_int, [_P]),
     "lbm_batch_profile_fused": (ctypes.c_int, [_P, ctypes.c_int, _D, _I]),
@@ -29,7 +29,8 @@ BATCH_FUNCTIONS = {
     "lbm_batch_destroy": (ctypes.c_int, [_P]),
 }
 
-WORKGROUP_SIZES = (256, 512, 1024)
+WORKGROUP_SIZES = (256, 512, 1024)          # nodes per work-group of the lattice pass
+COUPLING_SIZES = (64, 128, 256)              # threads per work-group of the coupling
 
 
 def _d(a):
@@ -86,12 +87,14 @@ class Batch:
             self.handle = None
 
     def autotune(self, stepsPerTrial=20):
-        """Times each work-group size of the fused pass (the replicas advance meanwhile); returns
-        ({threads: ms per step}, chosen threads)."""
-        times = np.zeros(3)
-        best = ctypes.c_int()
-        self._check(self.lib.lbm_batch_autotune(self.handle, int(stepsPerTrial), _d(times), ctypes.byref(best)))
-        return dict(zip(WORKGROUP_SIZES, times.tolist())), best.value
+        """Times each tile shape (nodes per lattice-pass work-group x threads per coupling work-group; the replicas
+        advance meanwhile); returns ({(fused, couple): ms per step}, (fused, couple) chosen)."""
+        times = np.zeros(9)
+        fused, couple = ctypes.c_int(), ctypes.c_int()
+        self._check(self.lib.lbm_batch_autotune(self.handle, int(stepsPerTrial), _d(times), ctypes.byref(fused),
+                                                ctypes.byref(couple)))
+        shapes = [(f, c) for f in WORKGROUP_SIZES for c in COUPLING_SIZES]
+        return dict(zip(shapes, times.tolist())), (fused.value, couple.value)
 
     def step(self, numSteps=1):
         self._check(self.lib.lbm_batch_step(self.handle, int(numSteps)))

@@ -446,3 +446,52 @@ def test_reaction_beyond_fixed_point_range_is_reported():
     with pytest.raises(LBMError, match="diverged") as err:
         c.step(1)
     assert err.value.status == _lib.LBM_ERROR_RUNTIME
+
+
+@pytest.mark.parametrize("storage", [LBMForce.FP32, LBMForce.FP16], ids=["fp32", "fp16"])
+@pytest.mark.parametrize("stencil", [LBMForce.Keys, LBMForce.ThreePoint, LBMForce.Trilinear, LBMForce.NearestNode],
+                         ids=["keys", "threepoint", "trilinear", "nearest"])
+@pytest.mark.parametrize("case", ["sparse-walls", "dense-overflow"])
+def test_coupling_variants_are_bit_identical(case, stencil, storage, monkeypatch):
+    """The ways of coupling the explicit drag give the same run bit for bit: the reactions spread with fixed-point
+    atomics (LBM_AMD_SPREAD=atomic) or by the one-writer tile spread (=tiles: beads binned by the tile of their
+    stencil's first node, each tile's sums formed in LDS and written by one thread per node), with the node moments
+    pulled per stencil slot (LBM_AMD_FIELD=0) or read from a lattice-wide moment field (=1).  Fields, positions,
+    velocities, the wall force and a force evaluation between steps are compared.  Sparse beads between bounce-back
+    walls with fluctuations; and dense beads crowded into a few tiles (bins overflow, and those beads spread
+    themselves with the atomics in the same step) with the momentum removal every second step."""
+    nx, ny, nz = 64, 8, 12
+    s = Setup((nx, ny, nz), dx=1.0, dt=0.01, tau=0.7)
+    rng = np.random.default_rng(17)
+    if case == "sparse-walls":
+        pos = rng.uniform([0, 0, 1.5], [nx, ny, nz - 2.5], size=(60, 3))
+        pos[:6, 0] = [0.1, nx - 0.2, 31.9, 32.1, 63.5, 0.5]          # stencils across tile and periodic boundaries
+    else:
+        pos = np.concatenate([rng.normal([20.0, 3.0, 5.0], 1.2, size=(900, 3)),
+                              rng.uniform([0, 0, 0], [nx, ny, nz], size=(300, 3))])
+    s.add_particles(pos, mass=120.0, velocities=rng.normal(scale=0.4, size=(len(pos), 3)))
+
+    def conf(f):
+        if case == "sparse-walls":
+            f.setSolidNodes([i + nx*(j + ny*k) for k in (0, nz - 1) for j in range(ny) for i in range(nx)])
+            f.setWallScheme(LBMForce.BounceBack)
+            f.setFluidFluctuations(True)
+            f.setTemperature(300.0)
+        else:
+            f.setFluidMomentumRemovalFrequency(2)
+            f.setTemperature(100.0)
+        f.setBodyAcceleration((1e-5/1e-4, 0, 0))
+        f.setRandomNumberSeed(5)
+        f.setFriction(10.0)
+        f.setInterpolationStencil(stencil)
+    s.configure.append(conf)
+    runs = []
+    for spread, field in (("atomic", "0"), ("tiles", "0"), ("atomic", "1"), ("tiles", "1")):
+        monkeypatch.setenv("LBM_AMD_SPREAD", spread)
+        monkeypatch.setenv("LBM_AMD_FIELD", field)
+        c = s.build(None, storage=storage)
+        c.step(7)
+        runs.append((c.getFluidState(), c.positions.copy(), c.velocities.copy(), c.getWallForce(), c.getForces()))
+    for other in runs[1:]:
+        for a, b in zip(runs[0], other):
+            assert np.array_equal(a, b)

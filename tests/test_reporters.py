@@ -1,9 +1,11 @@
 """The run reporters (lbm_amd/reporters.py), on the CPU oracle: the temperature file (python/openmmlbm.i:619-680) and
-the VTK files (:682-860), read back and compared with the Context's own fields and particles."""
+the VTK files (:682-860), read back and compared with the Context's own fields and particles; on the GPU library, the
+same files against the oracle's."""
 import struct
 import xml.etree.ElementTree as ET
 
 import numpy as np
+import pytest
 
 import oracle_lib
 from lbm_amd import LBMForce
@@ -12,7 +14,7 @@ from lbm_amd.workloads import BOLTZ
 from parity import Setup
 
 
-def small_context(drag=LBMForce.Explicit):
+def small_context(drag=LBMForce.Explicit, backend="oracle"):
     n = 6
     s = Setup((n, n, n), tau=0.8)
     rng = np.random.default_rng(2)
@@ -26,7 +28,9 @@ def small_context(drag=LBMForce.Explicit):
         f.setSolidNodes(solid)
         f.setBodyAcceleration((0.3, 0, 0))
     s.configure.append(conf)
-    return s.build(oracle_lib.load_oracle()), s
+    if backend == "oracle":
+        return s.build(oracle_lib.load_oracle()), s
+    return s.build(None, storage=LBMForce.FP64 if backend == "fp64" else LBMForce.FP32), s
 
 
 def read_vtk(path):
@@ -111,3 +115,34 @@ def test_temperature_continues_an_appended_file(tmp_path):
         run(ctx, 4, [LBMTemperatureReporter(stream, 2, ctx.force)])
     lines = path.read_text().splitlines()
     assert [line.split()[0] for line in lines] == ['#"Step","Time', "2", "4", "6", "8"]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("backend, tol", [("fp64", 1e-12), ("fp32", 1e-5)], ids=["gpu-fp64", "gpu-fp32"])
+def test_reporters_on_the_gpu_library(tmp_path, backend, tol):
+    """Both reporters over the gfx950 library write the files that they write over the oracle: the same names, the
+    same XML, and arrays and temperatures that agree to 1e-12 (FP64 storage) or 1e-5 (FP32) relative to their
+    largest value."""
+    files = {}
+    for name in ("oracle", backend):
+        ctx, _ = small_context(LBMForce.Explicit, name)
+        prefix = str(tmp_path/f"{name}_run")
+        temperature = str(tmp_path/f"{name}_temperature.txt")
+        run(ctx, 8, [LBMVTKReporter(prefix, 4, ctx.force, double=True), LBMTemperatureReporter(temperature, 2, ctx.force)])
+        files[name] = (prefix, temperature)
+    (p0, t0), (p1, t1) = files["oracle"], files[backend]
+    for kind, ext in (("velocity", "vti"), ("density", "vti"), ("particles", "vtp")):
+        for step in (4, 8):
+            r0, a0 = read_vtk(f"{p0}_{kind}_{step:010d}.{ext}")
+            r1, a1 = read_vtk(f"{p1}_{kind}_{step:010d}.{ext}")
+            assert sorted(a0) == sorted(a1)
+            assert ET.tostring(r0.find(".//FieldData")) == ET.tostring(r1.find(".//FieldData"))
+            for name in a0:
+                x0, x1 = a0[name].astype(np.float64), a1[name].astype(np.float64)
+                scale = max(np.abs(x0).max(), 1e-300)
+                assert np.abs(x1 - x0).max() <= tol*scale, (kind, step, name)
+    rows0 = [line.split() for line in open(t0).read().splitlines()[1:]]
+    rows1 = [line.split() for line in open(t1).read().splitlines()[1:]]
+    assert [r[:2] for r in rows0] == [r[:2] for r in rows1]
+    for a, b in zip(rows0, rows1):
+        assert abs(float(b[2]) - float(a[2])) <= tol*max(abs(float(a[2])), 1e-3)
