@@ -248,6 +248,8 @@ void createBatch(lbm_batch* b, int R, const lbm_params* params, const int* numPa
         g.n = (uint32_t) ((int64_t) p.nx*p.ny*p.nz);
         g.divx = fastDivOf(p.nx);
         g.divxy = fastDivOf(g.nxy);
+        g.rowShift = 0;
+        g.qstride = g.nx;
         g.openX = g.openY = g.openZ = 0;
         double cellMass = p.density*p.dx*p.dx*p.dx;
         uint32_t seed = (uint32_t) p.random_number_seed;

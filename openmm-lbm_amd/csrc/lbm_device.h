@@ -505,23 +505,34 @@ struct Geometry {
     uint32_t nx, ny, nz, nxy, n;
     FastDiv divx, divxy;
     int openX, openY, openZ;
+    uint32_t rowShift = 0;      // log2 of the rows per block of the population layout (elementOf)
+    uint32_t qstride = 0;       // elements between populations of one node: nx << rowShift
 };
 
 /*
- * Population layout.  Each row (j, k) of the lattice holds the 19 populations of its nx nodes one after the other:
- * population q of node (i, j, k) is the element ((k*ny + j)*19 + q)*nx + i = q*nx + 19*node - 18*i.  A work-group
- * of consecutive nodes then reads and writes a few contiguous row blocks of 19*nx values instead of 19 planes
- * N values apart, which the HBM serves faster (on 256^3 a bare pull measured 73% of the peak instead of 69%).
+ * Population layout.  The rows (j, k) of the lattice are grouped in blocks of R = 2^rowShift consecutive rows (r = j +
+ * ny k; R divides ny); a block holds its 19 populations one after the other, each as R rows of nx values.  Population
+ * q of node (i, j, k) is the element ((r >> rowShift)*19 + q)*R*nx + (r & (R - 1))*nx + i.  With R = 1 each row holds
+ * its 19 populations in turn, q*nx + 19*node - 18*i.  A work-group of consecutive nodes then reads and writes a few
+ * contiguous runs of R*nx values per population instead of 19 planes N values apart, which the HBM serves faster (on
+ * 256^3 a bare pull measured 73% of the peak instead of 69%); larger R lengthens the runs for 2-byte storage.
  */
 
-/* Element of population 0 of a node (add q*nx for population q). */
+/* Element of population 0 of the rows' block base for row r (add q*qstride for population q, then i). */
 template <typename I>
-__device__ __forceinline__ I elementOf(uint32_t node, uint32_t i) {
-    return (I) NQ*(I) node - (I) (NQ - 1)*(I) i;
+__device__ __forceinline__ I rowElement(const Geometry& g, uint32_t r) {
+    return (I) (r >> g.rowShift)*(I) NQ*(I) g.qstride + (I) (r & ((1u << g.rowShift) - 1u))*(I) g.nx;
 }
 
-/* The offsets, in population elements (rows are 19*nx elements apart), that take a node to its neighbours one step
-   along -c (the pull source) on a periodic lattice. */
+/* Element of population 0 of node (i, j, k). */
+template <typename I>
+__device__ __forceinline__ I elementOf(const Geometry& g, uint32_t i, uint32_t j, uint32_t k) {
+    return rowElement<I>(g, j + g.ny*k) + (I) i;
+}
+
+/* The offsets, in population elements, that take a node to its neighbours one step along -c (the pull source) on a
+   periodic lattice.  Moving by ny rows keeps a row's place in its block (R divides ny), so the offsets add up for the
+   diagonal directions. */
 template <typename I>
 struct NeighboursOf {
     I xm, xp, ym, yp, zm, zp;   // offsets to (i-1), (i+1), (j-1), (j+1), (k-1), (k+1), wrapped
@@ -536,12 +547,14 @@ __device__ __forceinline__ void nodeCoords(const Geometry& g, uint32_t node, uin
 
 template <typename I>
 __device__ __forceinline__ NeighboursOf<I> elementNeighbours(const Geometry& g, uint32_t i, uint32_t j, uint32_t k) {
-    const I row = (I) NQ*(I) g.nx, slab = (I) NQ*(I) g.nxy, all = (I) NQ*(I) g.n;
+    const I slab = (I) NQ*(I) g.nxy, all = (I) NQ*(I) g.n;
+    const uint32_t r = j + g.ny*k;
+    const I here = rowElement<I>(g, r);
     NeighboursOf<I> nb;
     nb.xm = i > 0 ? (I) -1 : (I) g.nx - 1;
     nb.xp = i + 1 < g.nx ? (I) 1 : (I) 1 - (I) g.nx;
-    nb.ym = j > 0 ? -row : slab - row;
-    nb.yp = j + 1 < g.ny ? row : row - slab;
+    nb.ym = rowElement<I>(g, j > 0 ? r - 1 : r + g.ny - 1) - here;
+    nb.yp = rowElement<I>(g, j + 1 < g.ny ? r + 1 : r + 1 - g.ny) - here;
     nb.zm = k > 0 ? -slab : all - slab;
     nb.zp = k + 1 < g.nz ? slab : slab - all;
     return nb;
@@ -570,7 +583,7 @@ __device__ __forceinline__ I pushOffset(const NeighboursOf<I>& nb) {
 struct NodeSlots {
     uint64_t e;                      // element of population 0 of the node
     NeighboursOf<long long> nb;      // element offsets to its neighbours
-    uint64_t nx;                     // element stride between populations
+    uint64_t nx;                     // element stride between populations (Geometry::qstride)
 
     /* Population q of the node itself, of its pull source x - c_q, of the node x + c_q. */
     __device__ __forceinline__ uint64_t own(int q) const { return (uint64_t) q*nx + e; }
@@ -583,7 +596,8 @@ struct NodeSlots {
 };
 
 __device__ __forceinline__ NodeSlots nodeSlots(const Geometry& g, uint32_t node, uint32_t i, uint32_t j, uint32_t k) {
-    return NodeSlots{elementOf<uint64_t>(node, i), elementNeighbours<long long>(g, i, j, k), g.nx};
+    (void) node;
+    return NodeSlots{elementOf<uint64_t>(g, i, j, k), elementNeighbours<long long>(g, i, j, k), g.qstride};
 }
 
 /* The element at a 32-bit byte offset from a base (the array plus q*nx for population q): the base stays in scalar

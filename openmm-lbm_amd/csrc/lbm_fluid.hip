@@ -134,7 +134,7 @@ fusedRows(const S* __restrict__ src, S* __restrict__ dst,
     uint32_t i, j, k;
     nodeCoords(a.g, node, i, j, k);
     collideNode<S, REACTION, FLUCT, false>(fr, dst, reactionFixed, touchedWord, ucm, a, node,
-            elementOf<uint32_t>(node, i));
+            elementOf<uint32_t>(a.g, i, j, k));
 }
 
 /*
@@ -698,8 +698,8 @@ static void launchPathT(int path, bool fluct, const void* src, void* dst, const 
     }
 }
 
-bool fusedRowsFit(uint32_t nx, bool wide) {
-    return !wide && (nx % ROW_THREADS == 0 || nx == 128);
+bool fusedRowsFit(uint32_t nx, bool wide, uint32_t rowShift) {
+    return !wide && rowShift == 0 && (nx % ROW_THREADS == 0 || nx == 128);
 }
 
 void launchFused(int storage, int path, bool solids, bool reaction, bool fluct, const void* src, void* dst,

@@ -107,9 +107,9 @@ __device__ __forceinline__ void collideNode(const typename StorageTraits<S>::Rea
         if constexpr (FLUCT)
             v += noise[q];
         if constexpr (WIDE)
-            storePopStream(dst + (uint64_t) q*a.g.nx + e, v, r);
+            storePopStream(dst + (uint64_t) q*a.g.qstride + e, v, r);
         else
-            storePopStream(atByte(dst + (size_t) q*a.g.nx, e*(uint32_t) sizeof(S)), v, r);
+            storePopStream(atByte(dst + (size_t) q*a.g.qstride, e*(uint32_t) sizeof(S)), v, r);
     }
 }
 
@@ -127,9 +127,9 @@ __device__ __forceinline__ void fusedNode(const S* __restrict__ src, S* __restri
     uint32_t i, j, k;
     nodeCoords(a.g, node, i, j, k);
     using Offset = typename std::conditional<WIDE, long long, int>::type;
-    const ElementIndex<WIDE> e = elementOf<ElementIndex<WIDE>>(node, i);
+    const ElementIndex<WIDE> e = elementOf<ElementIndex<WIDE>>(a.g, i, j, k);
     typename StorageTraits<S>::Real fr[NQ];
-    pullNodeStream<S, WIDE>(src, a.g.nx, e, elementNeighbours<Offset>(a.g, i, j, k), fr);
+    pullNodeStream<S, WIDE>(src, a.g.qstride, e, elementNeighbours<Offset>(a.g, i, j, k), fr);
     collideNode<S, REACTION, FLUCT, WIDE>(fr, dst, reactionFixed, touchedWord, ucm, a, node, e);
 }
 

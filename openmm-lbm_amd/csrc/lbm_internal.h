@@ -160,7 +160,7 @@ void launchVerletBatch(const ReplicaDesc* table, const uint16_t* particleReplica
         double* velocities, const double* forces, const double* invMass, int64_t* stepCounter, hipStream_t stream);
 /* Which fused-pass kernel a context runs: fusedStep with 32-bit or 64-bit offsets, or fusedRows (LDS-staged pulls). */
 enum FusedPath { FUSED_DIRECT = 0, FUSED_WIDE = 1, FUSED_ROWS = 2 };
-bool fusedRowsFit(uint32_t nx, bool wide);
+bool fusedRowsFit(uint32_t nx, bool wide, uint32_t rowShift);
 void launchFused(int storage, int path, bool solids, bool reaction, bool fluct, const void* src, void* dst,
         const uint32_t* mask, unsigned long long* reactionFixed, uint32_t* touched, const double* ucm, const StepArgs& a,
         hipStream_t stream);

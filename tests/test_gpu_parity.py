@@ -495,3 +495,41 @@ def test_coupling_variants_are_bit_identical(case, stencil, storage, monkeypatch
     for other in runs[1:]:
         for a, b in zip(runs[0], other):
             assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("storage", [LBMForce.FP32, LBMForce.FP16, LBMForce.BF16, LBMForce.FP64],
+                         ids=["fp32", "fp16", "bf16", "fp64"])
+def test_row_block_layouts_are_bit_identical(storage, monkeypatch):
+    """The population layout with 1, 2 or 4 lattice rows per block (LBM_AMD_ROW_BLOCK) changes where values live, not
+    what is computed: the same run bit for bit, with regularized walls along y (the rows crossed by the walls' links
+    span blocks), Velocity and DensityVelocity faces along z, fluctuations and Keys beads.  ny = 12 takes every block
+    size; ny = 6 falls back from 4 rows to 2."""
+    for ny in (12, 6):
+        nx, nz = 10, 7
+        s = Setup((nx, ny, nz), dx=1.0, dt=0.01, tau=0.75)
+        rng = np.random.default_rng(9)
+        s.add_particles(rng.uniform([0, 1.5, 1.5], [nx, ny - 1.5, nz - 1.5], size=(20, 3)), mass=150.0,
+                        velocities=rng.normal(scale=0.3, size=(20, 3)))
+
+        def conf(f):
+            f.setSolidNodes([i + nx*ny*k for k in range(nz) for i in range(nx)])        # the plane j = 0
+            f.setWallScheme(LBMForce.Regularized)
+            f.setFaceBoundary(LBMForce.ZMin, LBMForce.Velocity)
+            f.setFaceBoundary(LBMForce.ZMax, LBMForce.DensityVelocity)
+            f.setFaceVelocity(LBMForce.ZMin, (0.0, 0.0, 0.3))
+            f.setFaceDensity(LBMForce.ZMax, 602.214*1.002)
+            f.setFluidFluctuations(True)
+            f.setTemperature(300.0)
+            f.setRandomNumberSeed(31)
+            f.setFriction(8.0)
+            f.setInterpolationStencil(LBMForce.Keys)
+        s.configure.append(conf)
+        runs = []
+        for rows in ("1", "2", "4"):
+            monkeypatch.setenv("LBM_AMD_ROW_BLOCK", rows)
+            c = s.build(None, storage=storage)
+            c.step(9)
+            runs.append((c.getFluidState(), c.positions.copy(), c.getWallForce(), c.getFluidFields()[1]))
+        for other in runs[1:]:
+            for a, b in zip(runs[0], other):
+                assert np.array_equal(a, b)
