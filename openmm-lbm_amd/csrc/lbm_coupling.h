@@ -143,9 +143,6 @@ struct StencilSlot {
     bool solid;
 };
 
-/* The fluid velocity of one node as the coupling sees it (computeMoments): u = j/rho of the pulled populations, with
-   j -= rho u_cm when the momentum removal is due, times weight (ReferenceLBMKernels.cpp:503-553, 643-697).  Zero where
-   rho <= 0. */
 /* rho and j (with j -= rho u_cm when the removal is due) of one node from its pulled populations, FP64. */
 template <typename S>
 __device__ __forceinline__ double nodeMoments(const S* __restrict__ src, const Geometry& g, uint32_t node,
@@ -175,6 +172,9 @@ __device__ __forceinline__ double nodeMoments(const S* __restrict__ src, const G
     return rho;
 }
 
+/* The fluid velocity of one node as the coupling sees it (computeMoments): u = j/rho of the pulled populations, with
+   j -= rho u_cm when the momentum removal is due, times weight (ReferenceLBMKernels.cpp:503-553, 643-697).  Zero where
+   rho <= 0. */
 template <typename S>
 __device__ __forceinline__ void nodeVelocity(const S* __restrict__ src, const Geometry& g, uint32_t node,
         const double* __restrict__ ucm, int removal, double weight, double u[3]) {
@@ -271,6 +271,24 @@ __device__ __forceinline__ StencilSlot interpolateGroup(const S* __restrict__ sr
    random numbers of the step (keyed by index and step). */
 __device__ __forceinline__ void particleNoise(uint32_t index, const ParticleArgs& a, double noise[4]) {
     drawNormals<double, 4>(a.seed, TAG_PARTICLE, index, a.step, noise);
+}
+
+/*
+ * Normal k (0-3) of particleNoise's four, from the words drawNormals gives its pair p = k/2: the radius is word p of the
+ * first Philox block, the angle half p of the first word of the third; even k take the cosine, odd k the sine, as in
+ * boxMullerPair.  Four lanes drawing one normal each give the particle's normals in the time of one FP64 logarithm,
+ * square root and cosine or sine.
+ */
+__device__ __forceinline__ double particleNormal(uint32_t index, int k, const ParticleArgs& a) {
+    const uint32_t lo = (uint32_t) (uint64_t) a.step, hi = (uint32_t) ((uint64_t) a.step >> 32);
+    const uint4x radii = philox(index, lo, hi, 0u, a.seed, TAG_PARTICLE);
+    const uint4x angles = philox(index, lo, hi, 2u, a.seed, TAG_PARTICLE);
+    const int p = k >> 1;
+    const uint32_t word = p == 0 ? radii.x : radii.y, half = (angles.x >> (16*p)) & 0xFFFFu;
+    const double twoPi = 6.283185307179586476925287;
+    const double r = sqrt(-2.0*log(((double) word + 1.0)*2.3283064365386963e-10));
+    const double angle = twoPi*((double) half*1.52587890625e-05);
+    return r*((k & 1) ? sin(angle) : cos(angle));
 }
 
 /* preset: the particle's normals when already drawn (particleNoise), else nullptr. */

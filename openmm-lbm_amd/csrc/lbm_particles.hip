@@ -181,19 +181,16 @@ __global__ void __launch_bounds__(BLOCK) coupleParticles(const S* __restrict__ s
     double m = active ? mass[i] : 0.0;
     double u[3];
     StencilSlot sl = interpolateGroup<S, LANES>(src, mask, positions + 3*(size_t) part, active, lane, ucm, a, u, field);
-    // The normals of the work-group's particles, drawn once by its first GROUPS lanes instead of by every group: a
-    // wavefront pays for the FP64 Box-Muller once however many of its lanes need it.  Drawn after the interpolation,
-    // so that the first wavefront's loads are in flight meanwhile.
+    // The normals of the work-group's particles, drawn by its first 4*GROUPS lanes, one normal each, instead of by
+    // every group: a wavefront pays for the FP64 logarithm and cosine or sine once however many of its lanes need
+    // them.  Drawn after the interpolation, so that the first wavefront's loads are in flight meanwhile.
     __shared__ double noise[LANES > 1 ? GROUPS : 1][4];
     const double* myNoise = nullptr;
     if (LANES > 1 && a.randomForce) {
-        if (threadIdx.x < GROUPS) {
-            double r[4] = {0, 0, 0, 0};
-            int k = blockIdx.x*GROUPS + threadIdx.x;
-            if (k < a.numParticles)
-                particleNoise(order != nullptr ? order[k] : k, a, r);
-            for (int k = 0; k < 4; k++)
-                noise[threadIdx.x][k] = r[k];
+        if (threadIdx.x < 4*GROUPS) {
+            int k = blockIdx.x*GROUPS + threadIdx.x/4;
+            noise[threadIdx.x/4][threadIdx.x & 3] =
+                    k < a.numParticles ? particleNormal(order != nullptr ? order[k] : k, threadIdx.x & 3, a) : 0.0;
         }
         __syncthreads();
         myNoise = noise[threadIdx.x/LANES];
@@ -416,17 +413,13 @@ __global__ void __launch_bounds__(THREADS) coupleBatch(const ReplicaDesc* __rest
     const float* src = (a.step & 1) ? d->popB : d->popA;
     double u[3];
     StencilSlot sl = interpolateGroup<float, LANES>(src, nullptr, positions + 3*(size_t) pp, active, lane, nullptr, a, u);
-    // The work-group's normals, drawn once by its first GROUPS lanes (as in coupleParticles).
+    // The work-group's normals, drawn by its first 4*GROUPS lanes (as in coupleParticles).
     __shared__ double noise[LANES > 1 ? GROUPS : 1][4];
     const double* myNoise = nullptr;
     if (LANES > 1 && a.randomForce) {
-        if (threadIdx.x < GROUPS) {
-            double r[4] = {0, 0, 0, 0};
-            int k = (int) (blockIdx.x - d->particleTileBegin)*GROUPS + threadIdx.x;
-            if (k < a.numParticles)
-                particleNoise((uint32_t) k, a, r);
-            for (int c = 0; c < 4; c++)
-                noise[threadIdx.x][c] = r[c];
+        if (threadIdx.x < 4*GROUPS) {
+            int k = (int) (blockIdx.x - d->particleTileBegin)*GROUPS + threadIdx.x/4;
+            noise[threadIdx.x/4][threadIdx.x & 3] = k < a.numParticles ? particleNormal((uint32_t) k, threadIdx.x & 3, a) : 0.0;
         }
         __syncthreads();
         myNoise = noise[threadIdx.x/LANES];
