@@ -509,7 +509,7 @@ void destroy(lbm_context* c) {
     coupleOrderDestroy(c->order);
     void* buffers[] = {c->popA, c->popB, c->mask, c->wallNodes, c->wallLinks, c->boundary, c->reducePartials, c->totals,
                        c->overflow, c->binCounts[0], c->binCounts[1], c->bins.beads, c->bins.anchor, c->bins.force,
-                       c->bins.slotNode, c->bins.slotWeight, c->momentField,
+                       c->bins.slotWeight, c->momentField,
                        c->ucm, c->wallPartials, c->particles, c->mass, c->particleForce, c->reactionFixed,
                        c->touched, c->stagePos, c->stageVel, c->closureX, c->closureNode, c->closureU, c->closureDu,
                        c->closureOod, c->closureCounters, c->closureField, c->stageForce};
@@ -699,7 +699,6 @@ void create(const lbm_params* params, const int* solid, int numSolid, const int*
             b.anchor = deviceAlloc<uint16_t>(tiles*(size_t) b.cap);
             b.force = deviceAlloc<double>(3*(size_t) numParts);
             int lanes = coupleLanes(p.interpolation_stencil);
-            b.slotNode = deviceAlloc<uint32_t>((size_t) numParts*lanes);
             b.slotWeight = deviceAlloc<double>((size_t) numParts*lanes);
             c->binned = true;
         }

@@ -100,8 +100,7 @@ struct SpreadBins {
     int* beads = nullptr;
     uint16_t* anchor = nullptr;      // [tile][cap]: the bead's first stencil node within its tile, x | y << 5 | z << 7
     double* force = nullptr;
-    uint32_t* slotNode = nullptr;    // [bead][lane]: the fluid node of each stencil slot (NO_NODE for solid or unused)
-    double* slotWeight = nullptr;    // [bead][lane]: its weight
+    double* slotWeight = nullptr;    // [bead][slot]: the weight of each stencil slot
     int cap = 0;
     uint32_t ntx = 0, nty = 0, ntz = 0;
 };

@@ -147,12 +147,8 @@ __device__ __forceinline__ void finishGroupBinned(const StencilSlot& sl, const d
             full = 1;
     }
     full = __shfl(full, 0, LANES);
-    if (active && !full) {
-        // The slots for spreadTiles: the node (fluid nodes only) and the weight of each lane.
-        size_t at = (size_t) index*LANES + lane;
-        bins.slotNode[at] = sl.inStencil && !sl.solid ? sl.node : NO_NODE;
-        bins.slotWeight[at] = sl.weight;
-    }
+    if (active && !full && sl.inStencil)
+        bins.slotWeight[(size_t) index*LANES + lane] = sl.weight;     // spreadTiles forms the slot's node itself
     bool flagged = false;
     if (sl.inStencil) {
         if (full)
@@ -211,18 +207,23 @@ __global__ void __launch_bounds__(BLOCK) coupleParticles(const S* __restrict__ s
 /*
  * The one-writer spread: one work-group per tile of 32 x 4 x 4 nodes, one thread per node.  The beads that can reach
  * the tile are those binned in it and in its neighbours one tile back along each axis (a stencil is at most 4 nodes
- * wide, no wider than a tile); each wavefront takes one of them at a time, one lane per stencil slot, and adds the
- * fixed-point reactions -w_j F of its slots inside the tile (fluid nodes only: the wall's part was taken by the
- * coupling) to the tile's accumulators in LDS.  Then each node's sums are written by its own thread, and each tile row's
- * touched-mask word by the row's first lane: no global atomics.  The slots, weights and fixed-point values are those of
- * the atomic spread (spreadReaction), and integer sums do not depend on their order, so both give the same bits.  The
- * reference builds its per-node sums from a sort of the stencil slots by node (lbmCoupling.cc:251-272, 1245-1266).
+ * wide, no wider than a tile).  The work-group takes their stencil rows along x one per thread, forms the row's nodes
+ * from the bead's first stencil node (its tile and anchor offset) and, where the row lies in the tile, loads its slots'
+ * weights and adds the fixed-point reactions -w F of the slots inside the tile to the tile's accumulators in LDS.  Then each fluid node's sums are
+ * written by its own thread (a solid node's part went to the wall in the coupling), and each tile row's touched-mask
+ * word by the row's first lane: no global atomics.  Weights and fixed-point values are those of the atomic spread
+ * (spreadReaction), and integer sums do not depend on their order, so both give the same bits.  Lattices of whole tiles
+ * with periodic faces only (spreadTilesFit): no stencil there is merged, so slot (sx, sy, sz) lies at the first node +
+ * (sx, sy, sz), wrapped.  The reference builds its per-node sums from a sort of the stencil slots by node
+ * (lbmCoupling.cc:251-272, 1245-1266).
  */
 template <int LANES>
 __global__ void __launch_bounds__(SPREAD_THREADS) spreadTiles(ParticleArgs a, SpreadBins bins,
-        unsigned long long* __restrict__ reactionFixed, uint32_t* __restrict__ touched) {
+        const uint32_t* __restrict__ mask, unsigned long long* __restrict__ reactionFixed, uint32_t* __restrict__ touched) {
+    constexpr int W = groupWidth<LANES>();
     __shared__ unsigned long long acc[3*SPREAD_THREADS];
     __shared__ int candidates[8*SPREAD_MAX_CAP];
+    __shared__ short candidateLocal[8*SPREAD_MAX_CAP][3];
     __shared__ double candidateForce[8*SPREAD_MAX_CAP][3];
     __shared__ int counts[8], firsts[8];
     __shared__ int numCandidates;
@@ -246,8 +247,8 @@ __global__ void __launch_bounds__(SPREAD_THREADS) spreadTiles(ParticleArgs a, Sp
     if (total == 0)
         return;
     // The candidates that reach into this tile, gathered by every thread: a bead binned one tile back along an axis
-    // reaches it when its first stencil node lies within W - 1 nodes of that tile's end.
-    constexpr int W = groupWidth<LANES>();
+    // reaches it when its first stencil node lies within W - 1 nodes of that tile's end.  Each keeps its first node
+    // relative to this tile's origin (negative for the tiles behind).
     if (t == 0)
         numCandidates = 0;
     for (int k = 0; k < 3; k++)
@@ -261,64 +262,74 @@ __global__ void __launch_bounds__(SPREAD_THREADS) spreadTiles(ParticleArgs a, Sp
         }
         const size_t entry = (size_t) firsts[d]*bins.cap + r;
         const uint32_t off = bins.anchor[entry];
-        const bool reaches = (!(d & 1) || (int) (off & 31u) >= SPREAD_TX - W + 1) &&
-                             (!(d & 2) || (int) ((off >> 5) & 3u) >= SPREAD_TY - W + 1) &&
-                             (!(d & 4) || (int) ((off >> 7) & 3u) >= SPREAD_TZ - W + 1);
+        const int ox = (int) (off & 31u), oy = (int) ((off >> 5) & 3u), oz = (int) ((off >> 7) & 3u);
+        const bool reaches = (!(d & 1) || ox >= SPREAD_TX - W + 1) && (!(d & 2) || oy >= SPREAD_TY - W + 1) &&
+                             (!(d & 4) || oz >= SPREAD_TZ - W + 1);
         if (reaches) {
             const int at = atomicAdd(&numCandidates, 1);
             const int i = bins.beads[entry];
             candidates[at] = i;
+            candidateLocal[at][0] = (short) ((d & 1) ? ox - SPREAD_TX : ox);
+            candidateLocal[at][1] = (short) ((d & 2) ? oy - SPREAD_TY : oy);
+            candidateLocal[at][2] = (short) ((d & 4) ? oz - SPREAD_TZ : oz);
             for (int k = 0; k < 3; k++)
                 candidateForce[at][k] = bins.force[3*(size_t) i + k];
         }
     }
     __syncthreads();
-    const int lo[3] = {(int) tx*SPREAD_TX, (int) ty*SPREAD_TY, (int) tz*SPREAD_TZ};
     const int lane = t & 63;
-    const int items = numCandidates*LANES;
-    // One item per (candidate, slot); each thread loads UNROLL items before it adds any, so that their loads overlap.
-    constexpr int UNROLL = 4;
+    // One item per (candidate, stencil row along x); each thread loads the weights of UNROLL rows before it adds any,
+    // so that their loads overlap.
+    constexpr int ROWS = W*W;
+    const int items = numCandidates*ROWS;
+    constexpr int UNROLL = 2;
     for (int base = t; base < items; base += SPREAD_THREADS*UNROLL) {
-        uint32_t node[UNROLL];
-        double weight[UNROLL], f[UNROLL][3];
+        int rowLocal[UNROLL], x0[UNROLL], cand[UNROLL];
+        double weight[UNROLL][W];
 #pragma unroll
         for (int u = 0; u < UNROLL; u++) {
             const int item = base + u*SPREAD_THREADS;
-            node[u] = NO_NODE;
+            rowLocal[u] = -1;
             if (item < items) {
-                const int c = item/LANES;
-                const size_t at = (size_t) candidates[c]*LANES + item % LANES;
-                node[u] = bins.slotNode[at];
-                weight[u] = bins.slotWeight[at];
-                for (int k = 0; k < 3; k++)
-                    f[u][k] = candidateForce[c][k];
+                const int c = item/ROWS, row = item % ROWS;
+                const int ly = candidateLocal[c][1] + row % W, lz = candidateLocal[c][2] + row/W;
+                if (ly >= 0 && ly < SPREAD_TY && lz >= 0 && lz < SPREAD_TZ) {
+                    const double* w = bins.slotWeight + (size_t) candidates[c]*LANES + W*row;
+#pragma unroll
+                    for (int sx = 0; sx < W; sx++)
+                        weight[u][sx] = w[sx];
+                    rowLocal[u] = SPREAD_TX*(ly + SPREAD_TY*lz);
+                    x0[u] = candidateLocal[c][0];
+                    cand[u] = c;
+                }
             }
         }
 #pragma unroll
         for (int u = 0; u < UNROLL; u++) {
-            if (node[u] == NO_NODE)
+            if (rowLocal[u] < 0)
                 continue;
-            uint32_t ix, iy, iz;
-            nodeCoords(a.g, node[u], ix, iy, iz);
-            const int lx = (int) ix - lo[0], ly = (int) iy - lo[1], lz = (int) iz - lo[2];
-            if (lx < 0 || lx >= SPREAD_TX || ly < 0 || ly >= SPREAD_TY || lz < 0 || lz >= SPREAD_TZ)
-                continue;
-            const int local = lx + SPREAD_TX*(ly + SPREAD_TY*lz);
-            for (int k = 0; k < 3; k++) {
-                double r = -weight[u]*f[u][k];
-                if (!reactionInRange(r, a.overflow))
+#pragma unroll
+            for (int sx = 0; sx < W; sx++) {
+                const int lx = x0[u] + sx;
+                if (lx < 0 || lx >= SPREAD_TX)
                     continue;
-                long long fixed = llrint(r*FIXED_SCALE);
-                if (fixed != 0)
-                    atomicAdd(&acc[k*SPREAD_THREADS + local], (unsigned long long) fixed);
+                for (int k = 0; k < 3; k++) {
+                    double r = -weight[u][sx]*candidateForce[cand[u]][k];
+                    if (!reactionInRange(r, a.overflow))
+                        continue;
+                    long long fixed = llrint(r*FIXED_SCALE);
+                    if (fixed != 0)
+                        atomicAdd(&acc[k*SPREAD_THREADS + rowLocal[u] + lx], (unsigned long long) fixed);
+                }
             }
         }
     }
     __syncthreads();
+    const int lo[3] = {(int) tx*SPREAD_TX, (int) ty*SPREAD_TY, (int) tz*SPREAD_TZ};
     const int lx = t % SPREAD_TX, ly = (t/SPREAD_TX) % SPREAD_TY, lz = t/(SPREAD_TX*SPREAD_TY);
     const uint32_t node = (uint32_t) (lo[0] + lx) + a.g.nx*((uint32_t) (lo[1] + ly) + a.g.ny*(uint32_t) (lo[2] + lz));
     const unsigned long long v[3] = {acc[t], acc[SPREAD_THREADS + t], acc[2*SPREAD_THREADS + t]};
-    const bool any = (v[0] | v[1] | v[2]) != 0ull;
+    const bool any = (v[0] | v[1] | v[2]) != 0ull && !solidAt(mask, node);
     const uint64_t ballot = __ballot(any);
     const uint32_t rowBits = (uint32_t) (ballot >> (lane & 32));
     uint32_t old = 0;
@@ -375,20 +386,19 @@ void launchSpreadTiles(const double* positions, const int* particles, const uint
         const SpreadBins& bins, unsigned long long* reactionFixed, uint32_t* touched, hipStream_t stream) {
     (void) positions;
     (void) particles;
-    (void) mask;
     unsigned grid = bins.ntx*bins.nty*bins.ntz;
     switch (coupleLanes(a.stencil)) {
     case 64:
-        spreadTiles<64><<<grid, SPREAD_THREADS, 0, stream>>>(a, bins, reactionFixed, touched);
+        spreadTiles<64><<<grid, SPREAD_THREADS, 0, stream>>>(a, bins, mask, reactionFixed, touched);
         break;
     case 32:
-        spreadTiles<32><<<grid, SPREAD_THREADS, 0, stream>>>(a, bins, reactionFixed, touched);
+        spreadTiles<32><<<grid, SPREAD_THREADS, 0, stream>>>(a, bins, mask, reactionFixed, touched);
         break;
     case 8:
-        spreadTiles<8><<<grid, SPREAD_THREADS, 0, stream>>>(a, bins, reactionFixed, touched);
+        spreadTiles<8><<<grid, SPREAD_THREADS, 0, stream>>>(a, bins, mask, reactionFixed, touched);
         break;
     default:
-        spreadTiles<1><<<grid, SPREAD_THREADS, 0, stream>>>(a, bins, reactionFixed, touched);
+        spreadTiles<1><<<grid, SPREAD_THREADS, 0, stream>>>(a, bins, mask, reactionFixed, touched);
         break;
     }
 }
