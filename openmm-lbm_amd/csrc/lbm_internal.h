@@ -104,7 +104,6 @@ struct SpreadBins {
     int cap = 0;
     uint32_t ntx = 0, nty = 0, ntz = 0;
 };
-constexpr uint32_t NO_NODE = 0xFFFFFFFFu;
 
 __device__ __forceinline__ bool reactionInRange(double r, unsigned int* overflow) {
     if (fabs(r) < REACTION_LIMIT)
