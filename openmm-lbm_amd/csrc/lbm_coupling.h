@@ -221,7 +221,7 @@ __device__ __forceinline__ void fieldVelocity(const double4* __restrict__ field,
 template <typename S, int LANES>
 __device__ __forceinline__ StencilSlot interpolateGroup(const S* __restrict__ src, const uint32_t* __restrict__ mask,
         const double* x, bool active, int lane, const double* __restrict__ ucm, const ParticleArgs& a, double u[3],
-        const double4* __restrict__ field = nullptr) {
+        const double4* __restrict__ field = nullptr, const AxisStencil* __restrict__ pre = nullptr) {
     constexpr int STENCIL = groupStencil<LANES>();
     constexpr int w = STENCIL == LBM_STENCIL_KEYS ? 4 : STENCIL == LBM_STENCIL_THREE_POINT ? 3 :
                       STENCIL == LBM_STENCIL_TRILINEAR ? 2 : 1;
@@ -238,7 +238,14 @@ __device__ __forceinline__ StencilSlot interpolateGroup(const S* __restrict__ sr
         sl.weight = 1.0;
         for (int k = 0; k < 3; k++) {
             double wk;
-            axisSlot<STENCIL>(wrapCoordinate(x[k], a.dx, size[k]), size[k], openAxis[k] != 0, sub[k], nodeIdx[k], wk);
+            if (pre != nullptr) {
+                // The particle's stencil as stencilSetup formed it: the same axisSlot values.
+                nodeIdx[k] = pre->index[k][sub[k]];
+                wk = pre->weight[k][sub[k]];
+            }
+            else
+                axisSlot<STENCIL>(wrapCoordinate(x[k], a.dx, size[k]), size[k], openAxis[k] != 0, sub[k], nodeIdx[k],
+                        wk);
             sl.weight *= wk;
         }
         sl.node = (uint32_t) (nodeIdx[0] + (int) a.g.nx*(nodeIdx[1] + (int) a.g.ny*nodeIdx[2]));

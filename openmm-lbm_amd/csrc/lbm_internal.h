@@ -94,6 +94,13 @@ constexpr int SPREAD_TX = 32, SPREAD_TY = 4, SPREAD_TZ = 4;
 constexpr int SPREAD_THREADS = SPREAD_TX*SPREAD_TY*SPREAD_TZ;
 constexpr int SPREAD_MAX_CAP = 128;
 
+/* A particle's stencil along each axis (axisSlot of every slot): the node index and weight of slot sub along axis k.
+   Formed once per particle and step by stencilSetup, one lane per particle, for the groups that couple it. */
+struct AxisStencil {
+    double weight[3][4];
+    int index[3][4];
+};
+
 struct SpreadBins {
     int* count = nullptr;
     int* nextCount = nullptr;
@@ -195,7 +202,7 @@ void launchReflect(const double* positions, double* velocities, const int* parti
 void launchCouple(int storage, const void* src, const uint32_t* mask, const double* positions, const double* velocities,
         const int* particles, const int* order, const double* mass, const double* ucm, const ParticleArgs& a,
         double* particleForce, unsigned long long* reactionFixed, uint32_t* touched, double* partials,
-        double* systemForces, const SpreadBins* bins, const double4* field, hipStream_t stream);
+        double* systemForces, const SpreadBins* bins, const double4* field, AxisStencil* stencils, hipStream_t stream);
 /* The moments (jx, jy, jz, rho) of every fluid node into field, for launchCouple's dense path. */
 void launchMomentField(int storage, const void* src, const uint32_t* mask, const ParticleArgs& a, const double* ucm,
         double4* field, hipStream_t stream);

@@ -96,12 +96,14 @@ __device__ __forceinline__ int anchorIndex(double s, int n) {
    (x | y << 5 | z << 7). */
 template <int STENCIL>
 __device__ __forceinline__ uint32_t anchorTile(const double* x, const ParticleArgs& a, const SpreadBins& bins,
-        uint16_t* offset) {
+        uint16_t* offset, const AxisStencil* pre) {
     const int size[3] = {(int) a.g.nx, (int) a.g.ny, (int) a.g.nz};
     const int tileSize[3] = {SPREAD_TX, SPREAD_TY, SPREAD_TZ};
     uint32_t t[3], o[3];
     for (int k = 0; k < 3; k++) {
-        int first = anchorIndex<STENCIL>(wrapCoordinate(x[k], a.dx, size[k]), size[k]);
+        // The first slot's node: on the lattices of whole tiles with periodic faces that the bins serve, no stencil is
+        // merged and it is the first node wrapped.
+        int first = pre != nullptr ? pre->index[k][0] : anchorIndex<STENCIL>(wrapCoordinate(x[k], a.dx, size[k]), size[k]);
         t[k] = (uint32_t) (first/tileSize[k]);
         o[k] = (uint32_t) (first - (int) t[k]*tileSize[k]);
     }
@@ -119,7 +121,7 @@ template <int LANES>
 __device__ __forceinline__ void finishGroupBinned(const StencilSlot& sl, const double u[3], const double* vel,
         const double* x, uint32_t index, double m, bool active, int lane, const ParticleArgs& a, double* forceOut,
         unsigned long long* __restrict__ reactionFixed, uint32_t* __restrict__ touched, double wallAcc[3],
-        const double* noise, double* systemForce, const SpreadBins& bins) {
+        const double* noise, double* systemForce, const SpreadBins& bins, const AxisStencil* pre) {
     double f[3] = {0, 0, 0};
     if (active) {
         dragForce(vel, u, index, m, a, f, noise);
@@ -135,7 +137,7 @@ __device__ __forceinline__ void finishGroupBinned(const StencilSlot& sl, const d
     int full = 0;
     if (active && lane == 0) {
         uint16_t offset;
-        uint32_t tile = anchorTile<groupStencil<LANES>()>(x, a, bins, &offset);
+        uint32_t tile = anchorTile<groupStencil<LANES>()>(x, a, bins, &offset, pre);
         int slot = atomicAdd(bins.count + tile, 1);
         if (slot < bins.cap) {
             bins.beads[(size_t) tile*bins.cap + slot] = (int) index;
@@ -161,13 +163,42 @@ __device__ __forceinline__ void finishGroupBinned(const StencilSlot& sl, const d
         markTouched<groupWidth<LANES>()>(touched, sl.node, flagged, lane);
 }
 
+/*
+ * The stencil of every coupled particle along each axis, four lanes per particle (lane k < 3 takes axis k): the wrapped
+ * coordinate and axisSlot of each slot, which every lane of a coupling group would otherwise form again (the FP64
+ * divisions of the wrap and the kernel polynomials run once per particle and axis instead of once per group).
+ */
+template <int STENCIL>
+__global__ void __launch_bounds__(BLOCK) stencilSetup(const double* __restrict__ positions,
+        const int* __restrict__ particles, ParticleArgs a, AxisStencil* __restrict__ out) {
+    constexpr int w = STENCIL == LBM_STENCIL_KEYS ? 4 : STENCIL == LBM_STENCIL_THREE_POINT ? 3 :
+                      STENCIL == LBM_STENCIL_TRILINEAR ? 2 : 1;
+    const int t = blockIdx.x*BLOCK + threadIdx.x;
+    const int i = t >> 2, k = t & 3;
+    if (i >= a.numParticles || k == 3)
+        return;
+    const int size[3] = {(int) a.g.nx, (int) a.g.ny, (int) a.g.nz};
+    const int openAxis[3] = {a.g.openX, a.g.openY, a.g.openZ};
+    const double s = wrapCoordinate(positions[3*(size_t) particles[i] + k], a.dx, size[k]);
+    AxisStencil& st = out[i];
+#pragma unroll
+    for (int sub = 0; sub < 4; sub++) {
+        int index = 0;
+        double weight = 0.0;
+        if (sub < w)
+            axisSlot<STENCIL>(s, size[k], openAxis[k] != 0, sub, index, weight);
+        st.index[k][sub] = index;
+        st.weight[k][sub] = weight;
+    }
+}
+
 template <typename S, int LANES, bool BINNED>
 __global__ void __launch_bounds__(BLOCK) coupleParticles(const S* __restrict__ src, const uint32_t* __restrict__ mask,
         const double* __restrict__ positions, const double* __restrict__ velocities, const int* __restrict__ particles,
         const int* __restrict__ order, const double* __restrict__ mass, const double* __restrict__ ucm, ParticleArgs a,
         double* __restrict__ particleForce, unsigned long long* __restrict__ reactionFixed, uint32_t* __restrict__ touched,
         double* __restrict__ partials, double* __restrict__ systemForces, SpreadBins bins,
-        const double4* __restrict__ field) {
+        const double4* __restrict__ field, const AxisStencil* __restrict__ stencils) {
     constexpr int GROUPS = BLOCK/LANES;
     int lane = threadIdx.x % LANES;
     int rank = blockIdx.x*GROUPS + threadIdx.x/LANES;    // position in the processing order
@@ -176,7 +207,9 @@ __global__ void __launch_bounds__(BLOCK) coupleParticles(const S* __restrict__ s
     int part = active ? particles[i] : 0;
     double m = active ? mass[i] : 0.0;
     double u[3];
-    StencilSlot sl = interpolateGroup<S, LANES>(src, mask, positions + 3*(size_t) part, active, lane, ucm, a, u, field);
+    const AxisStencil* pre = stencils != nullptr && active ? stencils + i : nullptr;
+    StencilSlot sl = interpolateGroup<S, LANES>(src, mask, positions + 3*(size_t) part, active, lane, ucm, a, u, field,
+            pre);
     // The normals of the work-group's particles, drawn by its first 4*GROUPS lanes, one normal each, instead of by
     // every group: a wavefront pays for the FP64 logarithm and cosine or sine once however many of its lanes need
     // them.  Drawn after the interpolation, so that the first wavefront's loads are in flight meanwhile.
@@ -195,7 +228,7 @@ __global__ void __launch_bounds__(BLOCK) coupleParticles(const S* __restrict__ s
     if constexpr (BINNED)
         finishGroupBinned<LANES>(sl, u, velocities + 3*(size_t) part, positions + 3*(size_t) part, (uint32_t) i, m, active,
                 lane, a, particleForce + 3*(size_t) (active ? i : 0), reactionFixed, touched, wallAcc, myNoise,
-                systemForces != nullptr ? systemForces + 3*(size_t) part : nullptr, bins);
+                systemForces != nullptr ? systemForces + 3*(size_t) part : nullptr, bins, pre);
     else
         finishGroup<LANES>(sl, u, velocities + 3*(size_t) part, (uint32_t) i, m, active, lane, a,
                 particleForce + 3*(size_t) (active ? i : 0), reactionFixed, touched, wallAcc, myNoise,
@@ -502,25 +535,42 @@ template <typename S, bool BINNED>
 static void launchCoupleB(const void* src, const uint32_t* mask, const double* positions, const double* velocities,
         const int* particles, const int* order, const double* mass, const double* ucm, const ParticleArgs& a,
         double* particleForce, unsigned long long* reactionFixed, uint32_t* touched, double* partials,
-        double* systemForces, const SpreadBins& bins, const double4* field, hipStream_t stream) {
+        double* systemForces, const SpreadBins& bins, const double4* field, AxisStencil* stencils, hipStream_t stream) {
     unsigned grid = (unsigned) coupleBlocks(a.stencil, a.numParticles);
     const S* s = (const S*) src;
+    if (stencils != nullptr) {
+        const unsigned setupGrid = (unsigned) ((4*(size_t) a.numParticles + BLOCK - 1)/BLOCK);
+        switch (a.stencil) {
+        case LBM_STENCIL_KEYS:
+            stencilSetup<LBM_STENCIL_KEYS><<<setupGrid, BLOCK, 0, stream>>>(positions, particles, a, stencils);
+            break;
+        case LBM_STENCIL_THREE_POINT:
+            stencilSetup<LBM_STENCIL_THREE_POINT><<<setupGrid, BLOCK, 0, stream>>>(positions, particles, a, stencils);
+            break;
+        case LBM_STENCIL_TRILINEAR:
+            stencilSetup<LBM_STENCIL_TRILINEAR><<<setupGrid, BLOCK, 0, stream>>>(positions, particles, a, stencils);
+            break;
+        default:
+            stencils = nullptr;          // one lane per particle: nothing to share
+            break;
+        }
+    }
     switch (coupleLanes(a.stencil)) {
     case 8:
         coupleParticles<S, 8, BINNED><<<grid, BLOCK, 0, stream>>>(s, mask, positions, velocities, particles, order, mass,
-                ucm, a, particleForce, reactionFixed, touched, partials, systemForces, bins, field);
+                ucm, a, particleForce, reactionFixed, touched, partials, systemForces, bins, field, stencils);
         break;
     case 32:
         coupleParticles<S, 32, BINNED><<<grid, BLOCK, 0, stream>>>(s, mask, positions, velocities, particles, order, mass,
-                ucm, a, particleForce, reactionFixed, touched, partials, systemForces, bins, field);
+                ucm, a, particleForce, reactionFixed, touched, partials, systemForces, bins, field, stencils);
         break;
     case 64:
         coupleParticles<S, 64, BINNED><<<grid, BLOCK, 0, stream>>>(s, mask, positions, velocities, particles, order, mass,
-                ucm, a, particleForce, reactionFixed, touched, partials, systemForces, bins, field);
+                ucm, a, particleForce, reactionFixed, touched, partials, systemForces, bins, field, stencils);
         break;
     default:
         coupleParticles<S, 1, BINNED><<<grid, BLOCK, 0, stream>>>(s, mask, positions, velocities, particles, order, mass,
-                ucm, a, particleForce, reactionFixed, touched, partials, systemForces, bins, field);
+                ucm, a, particleForce, reactionFixed, touched, partials, systemForces, bins, field, stencils);
         break;
     }
 }
@@ -529,43 +579,43 @@ template <typename S>
 static void launchCoupleT(const void* src, const uint32_t* mask, const double* positions, const double* velocities,
         const int* particles, const int* order, const double* mass, const double* ucm, const ParticleArgs& a,
         double* particleForce, unsigned long long* reactionFixed, uint32_t* touched, double* partials,
-        double* systemForces, const SpreadBins* bins, const double4* field, hipStream_t stream) {
+        double* systemForces, const SpreadBins* bins, const double4* field, AxisStencil* stencils, hipStream_t stream) {
     if (bins != nullptr && a.isStep)
         launchCoupleB<S, true>(src, mask, positions, velocities, particles, order, mass, ucm, a, particleForce,
-                reactionFixed, touched, partials, systemForces, *bins, field, stream);
+                reactionFixed, touched, partials, systemForces, *bins, field, stencils, stream);
     else
         launchCoupleB<S, false>(src, mask, positions, velocities, particles, order, mass, ucm, a, particleForce,
-                reactionFixed, touched, partials, systemForces, SpreadBins(), field, stream);
+                reactionFixed, touched, partials, systemForces, SpreadBins(), field, stencils, stream);
 }
 
 void launchCouple(int storage, const void* src, const uint32_t* mask, const double* positions, const double* velocities,
         const int* particles, const int* order, const double* mass, const double* ucm, const ParticleArgs& a,
         double* particleForce, unsigned long long* reactionFixed, uint32_t* touched, double* partials,
-        double* systemForces, const SpreadBins* bins, const double4* field, hipStream_t stream) {
+        double* systemForces, const SpreadBins* bins, const double4* field, AxisStencil* stencils, hipStream_t stream) {
     if (a.numParticles == 0)
         return;
     if (field != nullptr) {
         // The moment field replaces the pulls: the storage type no longer matters.
         launchCoupleT<float>(nullptr, mask, positions, velocities, particles, order, mass, ucm, a, particleForce,
-                reactionFixed, touched, partials, systemForces, bins, field, stream);
+                reactionFixed, touched, partials, systemForces, bins, field, stencils, stream);
         return;
     }
     switch (storage) {
     case LBM_STORAGE_FP16:
         launchCoupleT<__half>(src, mask, positions, velocities, particles, order, mass, ucm, a, particleForce, reactionFixed,
-                touched, partials, systemForces, bins, nullptr, stream);
+                touched, partials, systemForces, bins, nullptr, stencils, stream);
         break;
     case LBM_STORAGE_BF16:
         launchCoupleT<__hip_bfloat16>(src, mask, positions, velocities, particles, order, mass, ucm, a, particleForce,
-                reactionFixed, touched, partials, systemForces, bins, nullptr, stream);
+                reactionFixed, touched, partials, systemForces, bins, nullptr, stencils, stream);
         break;
     case LBM_STORAGE_FP64:
         launchCoupleT<double>(src, mask, positions, velocities, particles, order, mass, ucm, a, particleForce, reactionFixed,
-                touched, partials, systemForces, bins, nullptr, stream);
+                touched, partials, systemForces, bins, nullptr, stencils, stream);
         break;
     default:
         launchCoupleT<float>(src, mask, positions, velocities, particles, order, mass, ucm, a, particleForce, reactionFixed,
-                touched, partials, systemForces, bins, nullptr, stream);
+                touched, partials, systemForces, bins, nullptr, stencils, stream);
         break;
     }
 }

@@ -456,7 +456,8 @@ def test_coupling_variants_are_bit_identical(case, stencil, storage, monkeypatch
     """The ways of coupling the explicit drag give the same run bit for bit: the reactions spread with fixed-point
     atomics (LBM_AMD_SPREAD=atomic) or by the one-writer tile spread (=tiles: beads binned by the tile of their
     stencil's first node, each tile's sums formed in LDS and written by one thread per node), with the node moments
-    pulled per stencil slot (LBM_AMD_FIELD=0) or read from a lattice-wide moment field (=1).  Fields, positions,
+    pulled per stencil slot (LBM_AMD_FIELD=0) or read from a lattice-wide moment field (=1), with each particle's
+    stencil formed by every lane of its group (LBM_AMD_STENCIL_SETUP=0) or once per particle (=1).  Fields, positions,
     velocities, the wall force and a force evaluation between steps are compared.  Sparse beads between bounce-back
     walls with fluctuations; and dense beads crowded into a few tiles (bins overflow, and those beads spread
     themselves with the atomics in the same step) with the momentum removal every second step."""
@@ -486,9 +487,11 @@ def test_coupling_variants_are_bit_identical(case, stencil, storage, monkeypatch
         f.setInterpolationStencil(stencil)
     s.configure.append(conf)
     runs = []
-    for spread, field in (("atomic", "0"), ("tiles", "0"), ("atomic", "1"), ("tiles", "1")):
+    for spread, field, setup in (("atomic", "0", "0"), ("tiles", "0", "1"), ("atomic", "1", "1"), ("tiles", "1", "0"),
+                                 ("atomic", "0", "1")):
         monkeypatch.setenv("LBM_AMD_SPREAD", spread)
         monkeypatch.setenv("LBM_AMD_FIELD", field)
+        monkeypatch.setenv("LBM_AMD_STENCIL_SETUP", setup)
         c = s.build(None, storage=storage)
         c.step(7)
         runs.append((c.getFluidState(), c.positions.copy(), c.velocities.copy(), c.getWallForce(), c.getForces()))
