@@ -165,6 +165,7 @@ struct lbm_context {
     // The particles' stencils along each axis, formed once per particle before the stencil coupling (stencilSetup).
     AxisStencil* axisStencils = nullptr;
     int* binCounts[2] = {nullptr, nullptr};
+    int* binSpilled = nullptr;               // [2]: the spilled flags of the two count arrays
     int binParity = 0;
     // Set by lbm_execute_device for the duration of one call: the caller's force array, into which the stencil coupling
     // adds the forces of the coupled particles itself (forcesAdded then says so, and no separate addForces runs).
@@ -373,11 +374,16 @@ void couple(lbm_context* c, const double* dPos, const double* dVel, const double
     if (c->binned && isStep) {
         c->bins.count = c->binCounts[c->binParity];
         c->bins.nextCount = c->binCounts[c->binParity ^ 1];
+        c->bins.spilled = c->binSpilled + c->binParity;
+        c->bins.nextSpilled = c->binSpilled + (c->binParity ^ 1);
         c->binParity ^= 1;
         bins = &c->bins;
     }
     if (c->momentField != nullptr)
-        launchMomentField(c->p.storage, c->popA, c->mask, a, c->ucm, c->momentField, c->stream);
+        launchMomentField(c->p.storage, c->popA, c->mask, a, c->ucm, c->momentField, dPos, c->particles, c->axisStencils,
+                c->stream);
+    else if (c->axisStencils != nullptr)
+        launchStencilSetup(dPos, c->particles, a, c->axisStencils, c->stream);
     launchCouple(c->p.storage, c->popA, c->mask, dPos, dVel, c->particles, order, c->mass, c->ucm, a, c->particleForce,
             c->reactionFixed, c->touched, partials, c->forceTarget, bins, c->momentField, c->axisStencils, c->stream);
     if (bins != nullptr)
@@ -510,7 +516,7 @@ void destroy(lbm_context* c) {
     centredDestroy(c->centred);
     coupleOrderDestroy(c->order);
     void* buffers[] = {c->popA, c->popB, c->mask, c->wallNodes, c->wallLinks, c->boundary, c->reducePartials, c->totals,
-                       c->overflow, c->binCounts[0], c->binCounts[1], c->bins.beads, c->bins.anchor, c->bins.force,
+                       c->overflow, c->binCounts[0], c->binCounts[1], c->binSpilled, c->bins.beads, c->bins.anchor, c->bins.force,
                        c->bins.slotWeight, c->momentField, c->axisStencils,
                        c->ucm, c->wallPartials, c->particles, c->mass, c->particleForce, c->reactionFixed,
                        c->touched, c->stagePos, c->stageVel, c->closureX, c->closureNode, c->closureU, c->closureDu,
@@ -706,7 +712,9 @@ void create(const lbm_params* params, const int* solid, int numSolid, const int*
             HIP_CHECK(hipMemsetAsync(c->binCounts[1], 0, tiles*sizeof(int), c->stream));
             b.beads = deviceAlloc<int>(tiles*(size_t) b.cap);
             b.anchor = deviceAlloc<uint16_t>(tiles*(size_t) b.cap);
-            b.force = deviceAlloc<double>(3*(size_t) numParts);
+            b.force = deviceAlloc<double>(3*tiles*(size_t) b.cap);
+            c->binSpilled = deviceAlloc<int>(2);
+            HIP_CHECK(hipMemsetAsync(c->binSpilled, 0, 2*sizeof(int), c->stream));
             int lanes = coupleLanes(p.interpolation_stencil);
             b.slotWeight = deviceAlloc<double>((size_t) numParts*lanes);
             c->binned = true;

@@ -106,8 +106,10 @@ struct SpreadBins {
     int* nextCount = nullptr;
     int* beads = nullptr;
     uint16_t* anchor = nullptr;      // [tile][cap]: the bead's first stencil node within its tile, x | y << 5 | z << 7
-    double* force = nullptr;
+    double* force = nullptr;         // [tile][cap][3]: the bead's drag force
     double* slotWeight = nullptr;    // [bead][slot]: the weight of each stencil slot
+    int* spilled = nullptr;          // set when a bead of the step found its bin full (spread with the atomics)
+    int* nextSpilled = nullptr;      // the flag of the next step, cleared by spreadTiles
     int cap = 0;
     uint32_t ntx = 0, nty = 0, ntz = 0;
 };
@@ -203,9 +205,14 @@ void launchCouple(int storage, const void* src, const uint32_t* mask, const doub
         const int* particles, const int* order, const double* mass, const double* ucm, const ParticleArgs& a,
         double* particleForce, unsigned long long* reactionFixed, uint32_t* touched, double* partials,
         double* systemForces, const SpreadBins* bins, const double4* field, AxisStencil* stencils, hipStream_t stream);
-/* The moments (jx, jy, jz, rho) of every fluid node into field, for launchCouple's dense path. */
+/* The moments (jx, jy, jz, rho) of every fluid node into field, for launchCouple's dense path; with stencils, the
+   particles' stencils (launchStencilSetup) in the same launch. */
 void launchMomentField(int storage, const void* src, const uint32_t* mask, const ParticleArgs& a, const double* ucm,
-        double4* field, hipStream_t stream);
+        double4* field, const double* positions, const int* particles, AxisStencil* stencils, hipStream_t stream);
+/* The particles' stencils (AxisStencil) for launchCouple, in a launch of their own (launchMomentField forms them
+   alongside the moment field when given them). */
+void launchStencilSetup(const double* positions, const int* particles, const ParticleArgs& a, AxisStencil* stencils,
+        hipStream_t stream);
 /* The one-writer spread of the binned beads' reactions, one work-group per tile (after launchCouple with bins). */
 void launchSpreadTiles(const double* positions, const int* particles, const uint32_t* mask, const ParticleArgs& a,
         const SpreadBins& bins, unsigned long long* reactionFixed, uint32_t* touched, hipStream_t stream);

@@ -140,13 +140,16 @@ __device__ __forceinline__ void finishGroupBinned(const StencilSlot& sl, const d
         uint32_t tile = anchorTile<groupStencil<LANES>()>(x, a, bins, &offset, pre);
         int slot = atomicAdd(bins.count + tile, 1);
         if (slot < bins.cap) {
-            bins.beads[(size_t) tile*bins.cap + slot] = (int) index;
-            bins.anchor[(size_t) tile*bins.cap + slot] = offset;
+            const size_t entry = (size_t) tile*bins.cap + slot;
+            bins.beads[entry] = (int) index;
+            bins.anchor[entry] = offset;
             for (int k = 0; k < 3; k++)
-                bins.force[3*(size_t) index + k] = f[k];
+                bins.force[3*entry + k] = f[k];
         }
-        else
+        else {
             full = 1;
+            atomicOr(bins.spilled, 1);
+        }
     }
     full = __shfl(full, 0, LANES);
     if (active && !full && sl.inStencil)
@@ -166,29 +169,66 @@ __device__ __forceinline__ void finishGroupBinned(const StencilSlot& sl, const d
 /*
  * The stencil of every coupled particle along each axis, four lanes per particle (lane k < 3 takes axis k): the wrapped
  * coordinate and axisSlot of each slot, which every lane of a coupling group would otherwise form again (the FP64
- * divisions of the wrap and the kernel polynomials run once per particle and axis instead of once per group).
+ * divisions of the wrap and the kernel polynomials run once per particle and axis instead of once per group).  t: the
+ * thread's place among 4*numParticles.
  */
 template <int STENCIL>
-__global__ void __launch_bounds__(BLOCK) stencilSetup(const double* __restrict__ positions,
-        const int* __restrict__ particles, ParticleArgs a, AxisStencil* __restrict__ out) {
+__device__ __forceinline__ void stencilSetupLane(int t, const double* __restrict__ positions,
+        const int* __restrict__ particles, const ParticleArgs& a, AxisStencil* __restrict__ out) {
     constexpr int w = STENCIL == LBM_STENCIL_KEYS ? 4 : STENCIL == LBM_STENCIL_THREE_POINT ? 3 :
                       STENCIL == LBM_STENCIL_TRILINEAR ? 2 : 1;
-    const int t = blockIdx.x*BLOCK + threadIdx.x;
     const int i = t >> 2, k = t & 3;
     if (i >= a.numParticles || k == 3)
         return;
-    const int size[3] = {(int) a.g.nx, (int) a.g.ny, (int) a.g.nz};
-    const int openAxis[3] = {a.g.openX, a.g.openY, a.g.openZ};
-    const double s = wrapCoordinate(positions[3*(size_t) particles[i] + k], a.dx, size[k]);
+    // The size and openness of axis k, selected between values held in scalar registers: k differs between the lanes,
+    // and an array of the argument's fields indexed by it was compiled into one scalar load at the first lane's address
+    // (every lane then took that lane's axis).
+    const int nx = __builtin_amdgcn_readfirstlane((int) a.g.nx), ny = __builtin_amdgcn_readfirstlane((int) a.g.ny),
+              nz = __builtin_amdgcn_readfirstlane((int) a.g.nz);
+    const int ox = __builtin_amdgcn_readfirstlane(a.g.openX), oy = __builtin_amdgcn_readfirstlane(a.g.openY),
+              oz = __builtin_amdgcn_readfirstlane(a.g.openZ);
+    const int n = k == 0 ? nx : (k == 1 ? ny : nz);
+    const bool open = (k == 0 ? ox : (k == 1 ? oy : oz)) != 0;
+    const double s = wrapCoordinate(positions[3*(size_t) particles[i] + k], a.dx, n);
     AxisStencil& st = out[i];
 #pragma unroll
     for (int sub = 0; sub < 4; sub++) {
         int index = 0;
         double weight = 0.0;
         if (sub < w)
-            axisSlot<STENCIL>(s, size[k], openAxis[k] != 0, sub, index, weight);
+            axisSlot<STENCIL>(s, n, open, sub, index, weight);
         st.index[k][sub] = index;
         st.weight[k][sub] = weight;
+    }
+}
+
+template <int STENCIL>
+__global__ void __launch_bounds__(BLOCK) stencilSetup(const double* __restrict__ positions,
+        const int* __restrict__ particles, ParticleArgs a, AxisStencil* __restrict__ out) {
+    stencilSetupLane<STENCIL>(blockIdx.x*BLOCK + threadIdx.x, positions, particles, a, out);
+}
+
+static unsigned stencilSetupBlocks(int numParticles) {
+    return (unsigned) ((4*(size_t) numParticles + BLOCK - 1)/BLOCK);
+}
+
+void launchStencilSetup(const double* positions, const int* particles, const ParticleArgs& a, AxisStencil* stencils,
+        hipStream_t stream) {
+    if (a.numParticles == 0)
+        return;
+    const unsigned grid = stencilSetupBlocks(a.numParticles);
+    switch (a.stencil) {
+    case LBM_STENCIL_KEYS:
+        stencilSetup<LBM_STENCIL_KEYS><<<grid, BLOCK, 0, stream>>>(positions, particles, a, stencils);
+        break;
+    case LBM_STENCIL_THREE_POINT:
+        stencilSetup<LBM_STENCIL_THREE_POINT><<<grid, BLOCK, 0, stream>>>(positions, particles, a, stencils);
+        break;
+    case LBM_STENCIL_TRILINEAR:
+        stencilSetup<LBM_STENCIL_TRILINEAR><<<grid, BLOCK, 0, stream>>>(positions, particles, a, stencils);
+        break;
+    default:
+        break;
     }
 }
 
@@ -271,8 +311,11 @@ __global__ void __launch_bounds__(SPREAD_THREADS) spreadTiles(ParticleArgs a, Sp
         counts[t] = min(bins.count[from], bins.cap);
         firsts[t] = (int) from;
     }
-    if (t == 0)
+    if (t == 0) {
         bins.nextCount[tile] = 0;
+        if (tile == 0)
+            *bins.nextSpilled = 0;
+    }
     __syncthreads();
     int total = 0;
     for (int d = 0; d < 8; d++)
@@ -306,7 +349,7 @@ __global__ void __launch_bounds__(SPREAD_THREADS) spreadTiles(ParticleArgs a, Sp
             candidateLocal[at][1] = (short) ((d & 2) ? oy - SPREAD_TY : oy);
             candidateLocal[at][2] = (short) ((d & 4) ? oz - SPREAD_TZ : oz);
             for (int k = 0; k < 3; k++)
-                candidateForce[at][k] = bins.force[3*(size_t) i + k];
+                candidateForce[at][k] = bins.force[3*entry + k];
         }
     }
     __syncthreads();
@@ -365,9 +408,11 @@ __global__ void __launch_bounds__(SPREAD_THREADS) spreadTiles(ParticleArgs a, Sp
     const bool any = (v[0] | v[1] | v[2]) != 0ull && !solidAt(mask, node);
     const uint64_t ballot = __ballot(any);
     const uint32_t rowBits = (uint32_t) (ballot >> (lane & 32));
+    // Bits of beads that spread themselves (full bins); without any, the fused pass left the word clear.
+    const bool spilled = *bins.spilled != 0;
     uint32_t old = 0;
-    if (lx == 0 && rowBits != 0u)
-        old = touched[node >> 5];            // bits of beads that spread themselves (full bins)
+    if (spilled && lx == 0 && rowBits != 0u)
+        old = touched[node >> 5];
     old = __shfl(old, lane & 32, 64);
     if (any) {
         const bool add = (old >> lx) & 1u;
@@ -380,32 +425,68 @@ __global__ void __launch_bounds__(SPREAD_THREADS) spreadTiles(ParticleArgs a, Sp
         touched[node >> 5] = old | rowBits;
 }
 
-template <typename S>
+/* The moment field, after SETUP work-groups that form the particles' stencils (stencilSetupLane) when STENCIL is a
+   stencil wider than one node: the two are independent, and one launch overlaps the setup with the field's stream. */
+template <typename S, int STENCIL>
 __global__ void __launch_bounds__(BLOCK) momentField(const S* __restrict__ src, const uint32_t* __restrict__ mask,
-        Geometry g, const double* __restrict__ ucm, int removal, double4* __restrict__ field) {
-    for (uint32_t node = blockIdx.x*BLOCK + threadIdx.x; node < g.n; node += gridDim.x*BLOCK) {
+        Geometry g, const double* __restrict__ ucm, int removal, double4* __restrict__ field,
+        const double* __restrict__ positions, const int* __restrict__ particles, ParticleArgs a,
+        AxisStencil* __restrict__ stencils, unsigned setupBlocks) {
+    if constexpr (STENCIL != LBM_STENCIL_NEAREST) {
+        if (blockIdx.x < setupBlocks) {
+            stencilSetupLane<STENCIL>(blockIdx.x*BLOCK + threadIdx.x, positions, particles, a, stencils);
+            return;
+        }
+    }
+    const uint32_t block = blockIdx.x - setupBlocks, blocks = gridDim.x - setupBlocks;
+    for (uint32_t node = block*BLOCK + threadIdx.x; node < g.n; node += blocks*BLOCK) {
         if (solidAt(mask, node))
             continue;          // solid slots do not read the field
         momentFieldNode(src, g, node, ucm, removal, field);
     }
 }
 
+template <typename S>
+static void launchMomentFieldT(const S* src, const uint32_t* mask, const ParticleArgs& a, const double* ucm,
+        double4* field, const double* positions, const int* particles, AxisStencil* stencils, unsigned grid,
+        hipStream_t stream) {
+    const unsigned setup = stencils != nullptr ? stencilSetupBlocks(a.numParticles) : 0u;
+    switch (stencils != nullptr ? a.stencil : LBM_STENCIL_NEAREST) {
+    case LBM_STENCIL_KEYS:
+        momentField<S, LBM_STENCIL_KEYS><<<grid + setup, BLOCK, 0, stream>>>(src, mask, a.g, ucm, a.removal, field,
+                positions, particles, a, stencils, setup);
+        break;
+    case LBM_STENCIL_THREE_POINT:
+        momentField<S, LBM_STENCIL_THREE_POINT><<<grid + setup, BLOCK, 0, stream>>>(src, mask, a.g, ucm, a.removal,
+                field, positions, particles, a, stencils, setup);
+        break;
+    case LBM_STENCIL_TRILINEAR:
+        momentField<S, LBM_STENCIL_TRILINEAR><<<grid + setup, BLOCK, 0, stream>>>(src, mask, a.g, ucm, a.removal,
+                field, positions, particles, a, stencils, setup);
+        break;
+    default:
+        momentField<S, LBM_STENCIL_NEAREST><<<grid, BLOCK, 0, stream>>>(src, mask, a.g, ucm, a.removal, field,
+                positions, particles, a, nullptr, 0u);
+        break;
+    }
+}
+
 void launchMomentField(int storage, const void* src, const uint32_t* mask, const ParticleArgs& a, const double* ucm,
-        double4* field, hipStream_t stream) {
+        double4* field, const double* positions, const int* particles, AxisStencil* stencils, hipStream_t stream) {
     unsigned grid = (unsigned) std::min<uint64_t>((a.g.n + BLOCK - 1)/BLOCK, 65535u*4u);
     switch (storage) {
     case LBM_STORAGE_FP16:
-        momentField<__half><<<grid, BLOCK, 0, stream>>>((const __half*) src, mask, a.g, ucm, a.removal, field);
+        launchMomentFieldT((const __half*) src, mask, a, ucm, field, positions, particles, stencils, grid, stream);
         break;
     case LBM_STORAGE_BF16:
-        momentField<__hip_bfloat16><<<grid, BLOCK, 0, stream>>>((const __hip_bfloat16*) src, mask, a.g, ucm, a.removal,
-                field);
+        launchMomentFieldT((const __hip_bfloat16*) src, mask, a, ucm, field, positions, particles, stencils, grid,
+                stream);
         break;
     case LBM_STORAGE_FP64:
-        momentField<double><<<grid, BLOCK, 0, stream>>>((const double*) src, mask, a.g, ucm, a.removal, field);
+        launchMomentFieldT((const double*) src, mask, a, ucm, field, positions, particles, stencils, grid, stream);
         break;
     default:
-        momentField<float><<<grid, BLOCK, 0, stream>>>((const float*) src, mask, a.g, ucm, a.removal, field);
+        launchMomentFieldT((const float*) src, mask, a, ucm, field, positions, particles, stencils, grid, stream);
         break;
     }
 }
@@ -538,23 +619,8 @@ static void launchCoupleB(const void* src, const uint32_t* mask, const double* p
         double* systemForces, const SpreadBins& bins, const double4* field, AxisStencil* stencils, hipStream_t stream) {
     unsigned grid = (unsigned) coupleBlocks(a.stencil, a.numParticles);
     const S* s = (const S*) src;
-    if (stencils != nullptr) {
-        const unsigned setupGrid = (unsigned) ((4*(size_t) a.numParticles + BLOCK - 1)/BLOCK);
-        switch (a.stencil) {
-        case LBM_STENCIL_KEYS:
-            stencilSetup<LBM_STENCIL_KEYS><<<setupGrid, BLOCK, 0, stream>>>(positions, particles, a, stencils);
-            break;
-        case LBM_STENCIL_THREE_POINT:
-            stencilSetup<LBM_STENCIL_THREE_POINT><<<setupGrid, BLOCK, 0, stream>>>(positions, particles, a, stencils);
-            break;
-        case LBM_STENCIL_TRILINEAR:
-            stencilSetup<LBM_STENCIL_TRILINEAR><<<setupGrid, BLOCK, 0, stream>>>(positions, particles, a, stencils);
-            break;
-        default:
-            stencils = nullptr;          // one lane per particle: nothing to share
-            break;
-        }
-    }
+    if (a.stencil == LBM_STENCIL_NEAREST)
+        stencils = nullptr;              // one lane per particle: nothing to share
     switch (coupleLanes(a.stencil)) {
     case 8:
         coupleParticles<S, 8, BINNED><<<grid, BLOCK, 0, stream>>>(s, mask, positions, velocities, particles, order, mass,
