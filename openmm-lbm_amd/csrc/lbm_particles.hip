@@ -294,10 +294,13 @@ template <int LANES>
 __global__ void __launch_bounds__(SPREAD_THREADS) spreadTiles(ParticleArgs a, SpreadBins bins,
         const uint32_t* __restrict__ mask, unsigned long long* __restrict__ reactionFixed, uint32_t* __restrict__ touched) {
     constexpr int W = groupWidth<LANES>();
+    // The bin entries are taken SPREAD_CHUNK at a time, so that the candidates' LDS stays small and four work-groups
+    // share a CU; a tile's eight bins hold about 8x the mean beads per tile, one chunk in the usual case.
+    constexpr int SPREAD_CHUNK = 256;
     __shared__ unsigned long long acc[3*SPREAD_THREADS];
-    __shared__ int candidates[8*SPREAD_MAX_CAP];
-    __shared__ short candidateLocal[8*SPREAD_MAX_CAP][3];
-    __shared__ double candidateForce[8*SPREAD_MAX_CAP][3];
+    __shared__ int candidates[SPREAD_CHUNK];
+    __shared__ short candidateLocal[SPREAD_CHUNK][3];
+    __shared__ double candidateForce[SPREAD_CHUNK][3];
     __shared__ int counts[8], firsts[8];
     __shared__ int numCandidates;
     const uint32_t tile = blockIdx.x;
@@ -322,85 +325,86 @@ __global__ void __launch_bounds__(SPREAD_THREADS) spreadTiles(ParticleArgs a, Sp
         total += counts[d];
     if (total == 0)
         return;
-    // The candidates that reach into this tile, gathered by every thread: a bead binned one tile back along an axis
-    // reaches it when its first stencil node lies within W - 1 nodes of that tile's end.  Each keeps its first node
-    // relative to this tile's origin (negative for the tiles behind).
-    if (t == 0)
-        numCandidates = 0;
     for (int k = 0; k < 3; k++)
         acc[k*SPREAD_THREADS + t] = 0ull;
-    __syncthreads();
-    for (int c = t; c < total; c += SPREAD_THREADS) {
-        int r = c, d = 0;
-        while (r >= counts[d]) {
-            r -= counts[d];
-            d++;
-        }
-        const size_t entry = (size_t) firsts[d]*bins.cap + r;
-        const uint32_t off = bins.anchor[entry];
-        const int ox = (int) (off & 31u), oy = (int) ((off >> 5) & 3u), oz = (int) ((off >> 7) & 3u);
-        const bool reaches = (!(d & 1) || ox >= SPREAD_TX - W + 1) && (!(d & 2) || oy >= SPREAD_TY - W + 1) &&
-                             (!(d & 4) || oz >= SPREAD_TZ - W + 1);
-        if (reaches) {
-            const int at = atomicAdd(&numCandidates, 1);
-            const int i = bins.beads[entry];
-            candidates[at] = i;
-            candidateLocal[at][0] = (short) ((d & 1) ? ox - SPREAD_TX : ox);
-            candidateLocal[at][1] = (short) ((d & 2) ? oy - SPREAD_TY : oy);
-            candidateLocal[at][2] = (short) ((d & 4) ? oz - SPREAD_TZ : oz);
-            for (int k = 0; k < 3; k++)
-                candidateForce[at][k] = bins.force[3*entry + k];
-        }
-    }
-    __syncthreads();
     const int lane = t & 63;
-    // One item per (candidate, stencil row along x); each thread loads the weights of UNROLL rows before it adds any,
-    // so that their loads overlap.
-    constexpr int ROWS = W*W;
-    const int items = numCandidates*ROWS;
-    constexpr int UNROLL = 2;
-    for (int base = t; base < items; base += SPREAD_THREADS*UNROLL) {
-        int rowLocal[UNROLL], x0[UNROLL], cand[UNROLL];
-        double weight[UNROLL][W];
-#pragma unroll
-        for (int u = 0; u < UNROLL; u++) {
-            const int item = base + u*SPREAD_THREADS;
-            rowLocal[u] = -1;
-            if (item < items) {
-                const int c = item/ROWS, row = item % ROWS;
-                const int ly = candidateLocal[c][1] + row % W, lz = candidateLocal[c][2] + row/W;
-                if (ly >= 0 && ly < SPREAD_TY && lz >= 0 && lz < SPREAD_TZ) {
-                    const double* w = bins.slotWeight + (size_t) candidates[c]*LANES + W*row;
-#pragma unroll
-                    for (int sx = 0; sx < W; sx++)
-                        weight[u][sx] = w[sx];
-                    rowLocal[u] = SPREAD_TX*(ly + SPREAD_TY*lz);
-                    x0[u] = candidateLocal[c][0];
-                    cand[u] = c;
-                }
+    for (int chunk = 0; chunk < total; chunk += SPREAD_CHUNK) {
+        // The candidates of this chunk of bin entries that reach into this tile, gathered by every thread: a bead
+        // binned one tile back along an axis reaches it when its first stencil node lies within W - 1 nodes of that
+        // tile's end.  Each keeps its first node relative to this tile's origin (negative for the tiles behind).
+        if (t == 0)
+            numCandidates = 0;
+        __syncthreads();
+        for (int c = chunk + t; c < min(total, chunk + SPREAD_CHUNK); c += SPREAD_THREADS) {
+            int r = c, d = 0;
+            while (r >= counts[d]) {
+                r -= counts[d];
+                d++;
+            }
+            const size_t entry = (size_t) firsts[d]*bins.cap + r;
+            const uint32_t off = bins.anchor[entry];
+            const int ox = (int) (off & 31u), oy = (int) ((off >> 5) & 3u), oz = (int) ((off >> 7) & 3u);
+            const bool reaches = (!(d & 1) || ox >= SPREAD_TX - W + 1) && (!(d & 2) || oy >= SPREAD_TY - W + 1) &&
+                                 (!(d & 4) || oz >= SPREAD_TZ - W + 1);
+            if (reaches) {
+                const int at = atomicAdd(&numCandidates, 1);
+                candidates[at] = bins.beads[entry];
+                candidateLocal[at][0] = (short) ((d & 1) ? ox - SPREAD_TX : ox);
+                candidateLocal[at][1] = (short) ((d & 2) ? oy - SPREAD_TY : oy);
+                candidateLocal[at][2] = (short) ((d & 4) ? oz - SPREAD_TZ : oz);
+                for (int k = 0; k < 3; k++)
+                    candidateForce[at][k] = bins.force[3*entry + k];
             }
         }
+        __syncthreads();
+        // One item per (candidate, stencil row along x); each thread loads the weights of UNROLL rows before it adds
+        // any, so that their loads overlap.
+        constexpr int ROWS = W*W;
+        const int items = numCandidates*ROWS;
+        constexpr int UNROLL = 2;
+        for (int base = t; base < items; base += SPREAD_THREADS*UNROLL) {
+            int rowLocal[UNROLL], x0[UNROLL], cand[UNROLL];
+            double weight[UNROLL][W];
 #pragma unroll
-        for (int u = 0; u < UNROLL; u++) {
-            if (rowLocal[u] < 0)
-                continue;
+            for (int u = 0; u < UNROLL; u++) {
+                const int item = base + u*SPREAD_THREADS;
+                rowLocal[u] = -1;
+                if (item < items) {
+                    const int c = item/ROWS, row = item % ROWS;
+                    const int ly = candidateLocal[c][1] + row % W, lz = candidateLocal[c][2] + row/W;
+                    if (ly >= 0 && ly < SPREAD_TY && lz >= 0 && lz < SPREAD_TZ) {
+                        const double* w = bins.slotWeight + (size_t) candidates[c]*LANES + W*row;
 #pragma unroll
-            for (int sx = 0; sx < W; sx++) {
-                const int lx = x0[u] + sx;
-                if (lx < 0 || lx >= SPREAD_TX)
+                        for (int sx = 0; sx < W; sx++)
+                            weight[u][sx] = w[sx];
+                        rowLocal[u] = SPREAD_TX*(ly + SPREAD_TY*lz);
+                        x0[u] = candidateLocal[c][0];
+                        cand[u] = c;
+                    }
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < UNROLL; u++) {
+                if (rowLocal[u] < 0)
                     continue;
-                for (int k = 0; k < 3; k++) {
-                    double r = -weight[u][sx]*candidateForce[cand[u]][k];
-                    if (!reactionInRange(r, a.overflow))
+#pragma unroll
+                for (int sx = 0; sx < W; sx++) {
+                    const int lx = x0[u] + sx;
+                    if (lx < 0 || lx >= SPREAD_TX)
                         continue;
-                    long long fixed = llrint(r*FIXED_SCALE);
-                    if (fixed != 0)
-                        atomicAdd(&acc[k*SPREAD_THREADS + rowLocal[u] + lx], (unsigned long long) fixed);
+                    for (int k = 0; k < 3; k++) {
+                        double r = -weight[u][sx]*candidateForce[cand[u]][k];
+                        if (!reactionInRange(r, a.overflow))
+                            continue;
+                        long long fixed = llrint(r*FIXED_SCALE);
+                        if (fixed != 0)
+                            atomicAdd(&acc[k*SPREAD_THREADS + rowLocal[u] + lx], (unsigned long long) fixed);
+                    }
                 }
             }
         }
+        __syncthreads();             // before the next chunk's candidates replace these
     }
-    __syncthreads();
     const int lo[3] = {(int) tx*SPREAD_TX, (int) ty*SPREAD_TY, (int) tz*SPREAD_TZ};
     const int lx = t % SPREAD_TX, ly = (t/SPREAD_TX) % SPREAD_TY, lz = t/(SPREAD_TX*SPREAD_TY);
     const uint32_t node = (uint32_t) (lo[0] + lx) + a.g.nx*((uint32_t) (lo[1] + ly) + a.g.ny*(uint32_t) (lo[2] + lz));
