@@ -94,10 +94,12 @@ constexpr int SPREAD_TX = 32, SPREAD_TY = 4, SPREAD_TZ = 4;
 constexpr int SPREAD_THREADS = SPREAD_TX*SPREAD_TY*SPREAD_TZ;
 constexpr int SPREAD_MAX_CAP = 128;
 
-/* A particle's stencil along each axis (axisSlot of every slot): the node index and weight of slot sub along axis k.
-   Formed once per particle and step by stencilSetup, one lane per particle, for the groups that couple it. */
+/* A particle's stencil along each axis (axisSlot of every slot): the node index and weight of slot sub along axis k,
+   and the particle's four normals of the step when the coupling has a random force (particleNormal).  Formed once per
+   particle and step, four lanes per particle, for the groups that couple it. */
 struct AxisStencil {
     double weight[3][4];
+    double normal[4];
     int index[3][4];
 };
 

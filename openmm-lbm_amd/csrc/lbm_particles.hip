@@ -169,8 +169,9 @@ __device__ __forceinline__ void finishGroupBinned(const StencilSlot& sl, const d
 /*
  * The stencil of every coupled particle along each axis, four lanes per particle (lane k < 3 takes axis k): the wrapped
  * coordinate and axisSlot of each slot, which every lane of a coupling group would otherwise form again (the FP64
- * divisions of the wrap and the kernel polynomials run once per particle and axis instead of once per group).  t: the
- * thread's place among 4*numParticles.
+ * divisions of the wrap and the kernel polynomials run once per particle and axis instead of once per group); with a
+ * random force, lane k also draws the particle's normal k, off the coupling's path.  t: the thread's place among
+ * 4*numParticles.
  */
 template <int STENCIL>
 __device__ __forceinline__ void stencilSetupLane(int t, const double* __restrict__ positions,
@@ -178,7 +179,11 @@ __device__ __forceinline__ void stencilSetupLane(int t, const double* __restrict
     constexpr int w = STENCIL == LBM_STENCIL_KEYS ? 4 : STENCIL == LBM_STENCIL_THREE_POINT ? 3 :
                       STENCIL == LBM_STENCIL_TRILINEAR ? 2 : 1;
     const int i = t >> 2, k = t & 3;
-    if (i >= a.numParticles || k == 3)
+    if (i >= a.numParticles)
+        return;
+    if (a.randomForce)
+        out[i].normal[k] = particleNormal((uint32_t) i, k, a);
+    if (k == 3)
         return;
     // The size and openness of axis k, selected between values held in scalar registers: k differs between the lanes,
     // and an array of the argument's fields indexed by it was compiled into one scalar load at the first lane's address
@@ -253,9 +258,10 @@ __global__ void __launch_bounds__(BLOCK) coupleParticles(const S* __restrict__ s
     // The normals of the work-group's particles, drawn by its first 4*GROUPS lanes, one normal each, instead of by
     // every group: a wavefront pays for the FP64 logarithm and cosine or sine once however many of its lanes need
     // them.  Drawn after the interpolation, so that the first wavefront's loads are in flight meanwhile.
+    // With the stencils formed beforehand the normals come with them (no barrier here).
     __shared__ double noise[LANES > 1 ? GROUPS : 1][4];
     const double* myNoise = nullptr;
-    if (LANES > 1 && a.randomForce) {
+    if (LANES > 1 && a.randomForce && stencils == nullptr) {
         if (threadIdx.x < 4*GROUPS) {
             int k = blockIdx.x*GROUPS + threadIdx.x/4;
             noise[threadIdx.x/4][threadIdx.x & 3] =
@@ -264,6 +270,8 @@ __global__ void __launch_bounds__(BLOCK) coupleParticles(const S* __restrict__ s
         __syncthreads();
         myNoise = noise[threadIdx.x/LANES];
     }
+    else if (a.randomForce && pre != nullptr)
+        myNoise = pre->normal;
     double wallAcc[3] = {0, 0, 0};
     if constexpr (BINNED)
         finishGroupBinned<LANES>(sl, u, velocities + 3*(size_t) part, positions + 3*(size_t) part, (uint32_t) i, m, active,
