@@ -57,7 +57,7 @@ __device__ __forceinline__ void particleKnowns(const double* vel, const double* 
         const ParticleArgs& a, double known[3], double random[3]) {
     double noise[4] = {0, 0, 0, 0};
     if (a.randomForce)
-        drawNormals<double, 4>(a.seed, TAG_PARTICLE, i, a.step, noise);
+        particleNoise(i, a, noise);
     double amp = sqrt(2.0*a.gamma*m*a.kT);
     for (int k = 0; k < 3; k++) {
         known[k] = vel[k]*(1.0/a.velocityScale) + (fc ? fc[k] : 0.0)*(0.5/(m*a.forceScale));

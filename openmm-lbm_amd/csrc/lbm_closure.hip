@@ -164,12 +164,22 @@ __global__ void __launch_bounds__(BLOCK) closureCouple(const double* __restrict_
     bool active = i < a.numParticles;
     bool fallback = active && ood[i] != 0;
     bool network = active && !fallback;
+    // The work-group's counts, added to its own pair of counters (one writer each: no atomics on a shared word).
+    __shared__ unsigned waveCounts[BLOCK/64][2];
     unsigned long long netVotes = __ballot(network), fallVotes = __ballot(fallback);
     if ((threadIdx.x & 63) == 0) {
-        if (netVotes)
-            atomicAdd(counters, (unsigned long long) __popcll(netVotes));
-        if (fallVotes)
-            atomicAdd(counters + 1, (unsigned long long) __popcll(fallVotes));
+        waveCounts[threadIdx.x >> 6][0] = (unsigned) __popcll(netVotes);
+        waveCounts[threadIdx.x >> 6][1] = (unsigned) __popcll(fallVotes);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned net = 0, fall = 0;
+        for (int w = 0; w < BLOCK/64; w++) {
+            net += waveCounts[w][0];
+            fall += waveCounts[w][1];
+        }
+        counters[2*blockIdx.x] += net;
+        counters[2*blockIdx.x + 1] += fall;
     }
     if (!network)
         return;
@@ -271,6 +281,10 @@ void launchClosureCouple(const double* velocities, const int* particles, const d
         return;
     closureCouple<<<gridOf(a.numParticles, BLOCK), BLOCK, 0, stream>>>(velocities, particles, mass, a, centreNode,
             centreVelocity, du, ood, particleForce, reactionFixed, touched, counters);
+}
+
+int closureCoupleBlocks(int numParticles) {
+    return (int) gridOf(std::max(numParticles, 1), BLOCK);
 }
 
 int closureFallbackBlocks(int numParticles) {

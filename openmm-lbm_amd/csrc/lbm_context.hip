@@ -323,6 +323,7 @@ ParticleArgs particleArgs(lbm_context* c, int64_t step, bool isStep) {
     a.removal = c->removalDue(step) ? 1 : 0;
     a.isStep = isStep ? 1 : 0;
     a.randomForce = (a.kT > 0 && a.gamma > 0) ? 1 : 0;
+    a.noiseFP64 = c->p.storage == LBM_STORAGE_FP64 ? 1 : 0;
     a.seed = c->seed;
     a.step = step;
     a.hasSolids = c->mask != nullptr;
@@ -1198,8 +1199,9 @@ int lbm_set_closure(lbm_context* c, lbm_mlp* mlp) {
                 c->closureU = deviceAlloc<double>(3*n);
                 c->closureDu = deviceAlloc<float>(3*n);
                 c->closureOod = deviceAlloc<uint8_t>(n);
-                c->closureCounters = deviceAlloc<unsigned long long>(2);
-                HIP_CHECK(hipMemsetAsync(c->closureCounters, 0, 2*sizeof(unsigned long long), c->stream));
+                const size_t counters = 2*(size_t) closureCoupleBlocks(c->numParticles);
+                c->closureCounters = deviceAlloc<unsigned long long>(counters);
+                HIP_CHECK(hipMemsetAsync(c->closureCounters, 0, counters*sizeof(unsigned long long), c->stream));
             }
             // Dense particles (more neighbourhood nodes than lattice nodes): one velocity pass over the lattice.
             if (c->closureField == nullptr && 27*(int64_t) c->numParticles >= (int64_t) c->g.n)
@@ -1216,9 +1218,16 @@ int lbm_closure_stats(lbm_context* c, int64_t* network, int64_t* fallback) {
     LBM_TRY({
         unsigned long long counts[2] = {0, 0};
         if (c->closureCounters != nullptr) {
-            HIP_CHECK(hipMemcpyAsync(counts, c->closureCounters, sizeof(counts), hipMemcpyDeviceToHost, c->stream));
-            HIP_CHECK(hipMemsetAsync(c->closureCounters, 0, sizeof(counts), c->stream));
+            // Two counts per work-group of the closure coupling, summed here.
+            std::vector<unsigned long long> blocks(2*(size_t) closureCoupleBlocks(c->numParticles));
+            const size_t bytes = blocks.size()*sizeof(unsigned long long);
+            HIP_CHECK(hipMemcpyAsync(blocks.data(), c->closureCounters, bytes, hipMemcpyDeviceToHost, c->stream));
+            HIP_CHECK(hipMemsetAsync(c->closureCounters, 0, bytes, c->stream));
             HIP_CHECK(hipStreamSynchronize(c->stream));
+            for (size_t b = 0; b < blocks.size(); b += 2) {
+                counts[0] += blocks[b];
+                counts[1] += blocks[b + 1];
+            }
         }
         if (network)
             *network = (int64_t) counts[0];

@@ -277,14 +277,22 @@ __device__ __forceinline__ StencilSlot interpolateGroup(const S* __restrict__ sr
 /* The explicit Euler-Maruyama drag in lattice units, F = -gamma m (v - u) + sqrt(2 gamma m kT) xi, with the particle's
    random numbers of the step (keyed by index and step). */
 __device__ __forceinline__ void particleNoise(uint32_t index, const ParticleArgs& a, double noise[4]) {
-    drawNormals<double, 4>(a.seed, TAG_PARTICLE, index, a.step, noise);
+    // The precision of a random number: FP64 with FP64 storage, else FP32 as the fluid's noise (boxMullerPair).
+    if (a.noiseFP64)
+        drawNormals<double, 4>(a.seed, TAG_PARTICLE, index, a.step, noise);
+    else {
+        float r[4];
+        drawNormals<float, 4>(a.seed, TAG_PARTICLE, index, a.step, r);
+        for (int k = 0; k < 4; k++)
+            noise[k] = (double) r[k];
+    }
 }
 
 /*
  * Normal k (0-3) of particleNoise's four, from the words drawNormals gives its pair p = k/2: the radius is word p of the
  * first Philox block, the angle half p of the first word of the third; even k take the cosine, odd k the sine, as in
- * boxMullerPair.  Four lanes drawing one normal each give the particle's normals in the time of one FP64 logarithm,
- * square root and cosine or sine.
+ * boxMullerPair, in the precision particleNoise uses.  Four lanes drawing one normal each give the particle's normals
+ * in the time of one logarithm, square root and cosine or sine.
  */
 __device__ __forceinline__ double particleNormal(uint32_t index, int k, const ParticleArgs& a) {
     const uint32_t lo = (uint32_t) (uint64_t) a.step, hi = (uint32_t) ((uint64_t) a.step >> 32);
@@ -292,10 +300,16 @@ __device__ __forceinline__ double particleNormal(uint32_t index, int k, const Pa
     const uint4x angles = philox(index, lo, hi, 2u, a.seed, TAG_PARTICLE);
     const int p = k >> 1;
     const uint32_t word = p == 0 ? radii.x : radii.y, half = (angles.x >> (16*p)) & 0xFFFFu;
-    const double twoPi = 6.283185307179586476925287;
-    const double r = sqrt(-2.0*log(((double) word + 1.0)*2.3283064365386963e-10));
-    const double angle = twoPi*((double) half*1.52587890625e-05);
-    return r*((k & 1) ? sin(angle) : cos(angle));
+    double pair[2];
+    if (a.noiseFP64)
+        boxMullerPair<double>(word, half, pair);
+    else {
+        float f[2];
+        boxMullerPair<float>(word, half, f);
+        pair[0] = (double) f[0];
+        pair[1] = (double) f[1];
+    }
+    return pair[k & 1];
 }
 
 /* preset: the particle's normals when already drawn (particleNoise), else nullptr. */

@@ -69,6 +69,7 @@ struct ParticleArgs {
     int removal;
     int isStep;
     int randomForce;
+    int noiseFP64 = 0;            // particle normals in FP64 (FP64 storage); else FP32, from the hardware functions
     uint32_t seed;
     int64_t step;
     int hasSolids;

@@ -53,7 +53,8 @@ void launchMlpInfer(const MlpDevice* weights, const _Float16* X, int n, float ou
  *            its FP64 velocity, and the out-of-distribution flag of every particle;
  *  targets:  u_Keys - u(nearest) of every particle (training pairs; a.stencil must be Keys);
  *  couple:   nearest-node drag with u(nearest) + du for in-distribution particles, reaction on the nearest node;
- *            counters[0] and counters[1] count the particles coupled through the network and left to the fallback;
+ *            counters[2b] and counters[2b + 1] count the particles of work-group b coupled through the network and
+ *            left to the fallback (closureCoupleBlocks(n) work-groups);
  *  fallback: the stencil coupling (a.stencil) of the out-of-distribution particles, wall momentum into
  *            closureFallbackBlocks(n) partials.
  */
@@ -69,6 +70,8 @@ void launchClosureField(int storage, const void* src, const uint32_t* mask, cons
 void launchClosureTargets(int storage, const void* src, const uint32_t* mask, const double* positions,
         const int* particles, const double* ucm, const ParticleArgs& a, const double* centreVelocity, float* targets,
         hipStream_t stream);
+/* Work-groups of launchClosureCouple for numParticles particles: its counters hold two counts per work-group. */
+int closureCoupleBlocks(int numParticles);
 void launchClosureCouple(const double* velocities, const int* particles, const double* mass, const ParticleArgs& a,
         const uint32_t* centreNode, const double* centreVelocity, const float* du, const uint8_t* ood,
         double* particleForce, unsigned long long* reactionFixed, uint32_t* touched, unsigned long long* counters,
