@@ -151,7 +151,11 @@ int lbm_compute_lattice_parameters(const lbm_force_settings* settings, const dou
  * At most 2^30 nodes (LBM_ERROR_INVALID beyond).  LBM_AMD_WIDE_ADDRESSING=1 in the environment makes the lattice
  * pass use 64-bit indices at any size (it does by itself where an array of populations exceeds 4 GiB).
  * LBM_AMD_FUSED_ROWS=1 makes it stage its pulls through LDS where rows hold 128 or a multiple of 256 nodes (the same
- * results; slower on MI355X, so off by default).
+ * results; slower on MI355X, so off by default).  Other switches that change the work, never the results (each run
+ * is bit-identical across them): LBM_AMD_ROW_BLOCK=1|2|4 (lattice rows per block of the population layout, default
+ * 1); and, for the explicit drag, LBM_AMD_SPREAD=atomic|tiles (the reaction spread), LBM_AMD_FIELD=0|1 (stencils read a
+ * lattice-wide moment field) and LBM_AMD_STENCIL_SETUP=0|1 (each particle's stencil formed once), by default tiles,
+ * field and setup for dense particles (stencil slots above a quarter of the nodes) and none of them otherwise.
  */
 int lbm_create(const lbm_params* params, const int* solid_nodes, int num_solid, const int* particles,
         const double* particle_masses, int num_particles, int num_system_particles, lbm_context** out);
