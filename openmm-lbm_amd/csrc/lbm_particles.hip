@@ -255,10 +255,10 @@ __global__ void __launch_bounds__(BLOCK) coupleParticles(const S* __restrict__ s
     const AxisStencil* pre = stencils != nullptr && active ? stencils + i : nullptr;
     StencilSlot sl = interpolateGroup<S, LANES>(src, mask, positions + 3*(size_t) part, active, lane, ucm, a, u, field,
             pre);
-    // The normals of the work-group's particles, drawn by its first 4*GROUPS lanes, one normal each, instead of by
-    // every group: a wavefront pays for the FP64 logarithm and cosine or sine once however many of its lanes need
-    // them.  Drawn after the interpolation, so that the first wavefront's loads are in flight meanwhile.
-    // With the stencils formed beforehand the normals come with them (no barrier here).
+    // The normals of the work-group's particles: formed with the stencils where those were formed beforehand (no
+    // barrier here), else drawn by the work-group's first 4*GROUPS lanes, one normal each, instead of by every group:
+    // a wavefront pays for the logarithm and cosine or sine once however many of its lanes need them.  Drawn after the
+    // interpolation, so that the first wavefront's loads are in flight meanwhile.
     __shared__ double noise[LANES > 1 ? GROUPS : 1][4];
     const double* myNoise = nullptr;
     if (LANES > 1 && a.randomForce && stencils == nullptr) {
@@ -290,9 +290,9 @@ __global__ void __launch_bounds__(BLOCK) coupleParticles(const S* __restrict__ s
  * the tile are those binned in it and in its neighbours one tile back along each axis (a stencil is at most 4 nodes
  * wide, no wider than a tile).  The work-group takes their stencil rows along x one per thread, forms the row's nodes
  * from the bead's first stencil node (its tile and anchor offset) and, where the row lies in the tile, loads its slots'
- * weights and adds the fixed-point reactions -w F of the slots inside the tile to the tile's accumulators in LDS.  Then each fluid node's sums are
- * written by its own thread (a solid node's part went to the wall in the coupling), and each tile row's touched-mask
- * word by the row's first lane: no global atomics.  Weights and fixed-point values are those of the atomic spread
+ * weights and adds the fixed-point reactions -w F of the slots inside the tile to the tile's accumulators in LDS.
+ * Then each fluid node's sums are written by its own thread (a solid node's part went to the wall in the coupling),
+ * and each tile row's touched-mask word by the row's first lane: no global atomics.  Weights and fixed-point values are those of the atomic spread
  * (spreadReaction), and integer sums do not depend on their order, so both give the same bits.  Lattices of whole tiles
  * with periodic faces only (spreadTilesFit): no stencil there is merged, so slot (sx, sy, sz) lies at the first node +
  * (sx, sy, sz), wrapped.  The reference builds its per-node sums from a sort of the stencil slots by node
