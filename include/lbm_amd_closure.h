@@ -12,7 +12,12 @@
  *   [84:87]  the particle velocity v dt/dx
  *
  * and its target Du = u_Keys(x) - u(nearest), that is (F_Keys - F_nearest)/(gamma m) of the explicit drag at the same
- * state.  The reaction goes to the nearest node.  A particle whose inputs fall outside the training range (a
+ * state, all in the particle's frame: the Keys stencil and the lattice are symmetric under the 48 reflections and axis
+ * permutations of the cube, and the particle's frame is the one that makes its offset o non-negative and sorted.
+ * Frame axis i is lattice axis p_i (|o| in decreasing order, equal values in axis order), reflected where
+ * o_(p_i) < 0 (s_i = -1): a vector w is w'_i = s_i w_(p_i) there, the node nearest + d sits at d'_i = s_i d_(p_i) of
+ * the block, and the offset inputs are |o_(p_i)|.  The network's output is turned back before the coupling uses it.
+ * The reaction goes to the nearest node.  A particle whose inputs fall outside the training range (a
  * normalized velocity beyond the largest seen in training), or whose 3x3x3 neighbourhood holds a solid node or leaves
  * the lattice at an open face, is coupled with the Keys stencil instead: Keys is both the fallback and the judge of
  * the network's error.  The context must use the Keys stencil and the explicit drag.
@@ -82,7 +87,8 @@ int lbm_mlp_gradients(lbm_mlp* mlp, const float* features, const float* targets,
 
 /*
  * Training pairs at the context's current state for its coupled particles (positions in nm, velocities in nm/ps of
- * all num_particles particles of the System, host arrays): features [n][87] and targets [n][3], n = coupled count.
+ * all num_particles particles of the System, host arrays): features [n][87] and targets [n][3] in each particle's
+ * frame, n = coupled count.
  */
 int lbm_closure_pairs(lbm_context* ctx, const double* positions, const double* velocities, float* features,
         float* targets);

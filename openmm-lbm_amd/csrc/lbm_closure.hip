@@ -3,13 +3,21 @@
  *
  *  closureFeatures  one group of 32 lanes per particle: lanes 0-26 pull the 19 populations of the 27 nodes around the
  *                   nearest node and form u = j/rho (FP64, as the stencil coupling does); lane 27 writes the offset,
- *                   lane 28 the particle velocity, lanes 29-31 the zero padding.  The particle is out of distribution
- *                   if a normalized velocity exceeds its bound, a neighbour is solid, or the neighbourhood leaves the
- *                   lattice at an open face.
- *  closureTargets   u_Keys(x) - u(nearest) with the stencil interpolation of the coupling (training pairs).
- *  closureCouple    one lane per particle: the nearest-node drag with u(nearest) + du and its reaction on the nearest
- *                   node, exactly as the nearest-node stencil coupling computes them (ReferenceLBMKernels.cpp:503-553);
- *                   with du = 0 the two are identical bit for bit.
+ *                   lane 28 the particle velocity, lanes 29-31 the zero padding, all in the particle's frame (below).
+ *                   The particle is out of distribution if a normalized velocity exceeds its bound, a neighbour is
+ *                   solid, or the neighbourhood leaves the lattice at an open face.
+ *  closureTargets   u_Keys(x) - u(nearest) with the stencil interpolation of the coupling, in the particle's frame
+ *                   (training pairs).
+ *  closureCouple    one lane per particle: the nearest-node drag with u(nearest) + du (du turned back from the
+ *                   particle's frame) and its reaction on the nearest node, exactly as the nearest-node stencil
+ *                   coupling computes them (ReferenceLBMKernels.cpp:503-553); with du = 0 the two are identical bit for
+ *                   bit.
+ *
+ * The particle's frame: the Keys stencil and the lattice are symmetric under the 48 reflections and axis permutations
+ * of the cube, so the network sees every particle after the one that makes its sub-node offset o non-negative and
+ * sorted, o'_0 >= o'_1 >= o'_2 >= 0: frame axis i is lattice axis p_i, reflected where o_(p_i) < 0 (sign s_i = -1).
+ * A vector w of the lattice frame has w'_i = s_i w_(p_i) there, and the node nearest + d sits at d'_i = s_i d_(p_i) of
+ * the 3x3x3 block.  The network then learns on a 48th of the offsets' cube.
  *  closureFallback  the Keys coupling of the out-of-distribution particles (ReferenceLBMKernels.cpp:643-697): fixed
  *                   grid, each wave scans the flags of 64 particles at a time with a fixed stride, so the wall partial
  *                   sums have a fixed order.
@@ -24,12 +32,52 @@ namespace {
 constexpr int FEATURE_LANES = 32;
 constexpr int FALLBACK_MAX_BLOCKS = 512;
 
+/* The frame of a particle whose sub-node offset is o (FP64): bit k set where lattice axis k is reflected (o_k < 0);
+   bits 3-4 and 5-6 hold p_0 and p_1, the lattice axes of frame axes 0 and 1 (p_2 = 3 - p_0 - p_1), ordered by |o|
+   from the largest, equal values in axis order. */
+__device__ __forceinline__ uint8_t frameCode(const double o[3]) {
+    double m[3];
+    unsigned code = 0;
+    for (int k = 0; k < 3; k++) {
+        m[k] = fabs(o[k]);
+        if (o[k] < 0)
+            code |= 1u << k;
+    }
+    // Rank of each axis: the axes that precede it.
+    int r0 = (m[1] > m[0]) + (m[2] > m[0]);
+    int r1 = (m[0] >= m[1]) + (m[2] > m[1]);
+    int p0 = r0 == 0 ? 0 : (r1 == 0 ? 1 : 2);
+    int p1 = r0 == 1 ? 0 : (r1 == 1 ? 1 : 2);
+    return (uint8_t) (code | (unsigned) p0 << 3 | (unsigned) p1 << 5);
+}
+
+struct Frame {
+    int axis[3];      // p_i: the lattice axis of frame axis i
+    double sign[3];   // s_i: -1 where that lattice axis is reflected
+};
+
+__device__ __forceinline__ Frame frameOf(unsigned code) {
+    Frame f;
+    f.axis[0] = (int) (code >> 3) & 3;
+    f.axis[1] = (int) (code >> 5) & 3;
+    f.axis[2] = 3 - f.axis[0] - f.axis[1];
+    for (int i = 0; i < 3; i++)
+        f.sign[i] = (code >> f.axis[i]) & 1u ? -1.0 : 1.0;
+    return f;
+}
+
+/* v[k] for a lane-varying k, by selects (no indexed private array). */
+template <typename T>
+__device__ __forceinline__ T pick(const T v[3], int k) {
+    return k == 0 ? v[0] : (k == 1 ? v[1] : v[2]);
+}
+
 template <typename S>
 __global__ void __launch_bounds__(BLOCK) closureFeatures(const S* __restrict__ src, const uint32_t* __restrict__ mask,
         const double* __restrict__ positions, const double* __restrict__ velocities, const int* __restrict__ particles,
         const double* __restrict__ ucm, ParticleArgs a, ClosureNorm norm, const double* __restrict__ field,
         _Float16* __restrict__ X, float* __restrict__ raw, uint32_t* __restrict__ centreNode,
-        double* __restrict__ centreVelocity, uint8_t* __restrict__ ood) {
+        double* __restrict__ centreVelocity, uint8_t* __restrict__ ood, uint8_t* __restrict__ frame) {
     constexpr int GROUPS = BLOCK/FEATURE_LANES;
     int lane = threadIdx.x % FEATURE_LANES;
     int i = blockIdx.x*GROUPS + threadIdx.x/FEATURE_LANES;
@@ -38,19 +86,27 @@ __global__ void __launch_bounds__(BLOCK) closureFeatures(const S* __restrict__ s
     int part = active ? particles[i] : 0;
     const double* x = positions + 3*(size_t) part;
     int size[3] = {(int) a.g.nx, (int) a.g.ny, (int) a.g.nz};
-    // The wrapped coordinate and the nearest node (FP64, as the coupling computes them) by the group's first lane,
-    // shared with the others.
+    // The wrapped coordinate, the nearest node (FP64, as the coupling computes them) and the particle's frame by the
+    // group's first lane, shared with the others.
     double sw[3] = {0, 0, 0};
     int near[3] = {0, 0, 0};
-    if (active && lane == 0)
+    int code = 0;
+    if (active && lane == 0) {
+        double o[3];
         for (int k = 0; k < 3; k++) {
             sw[k] = wrapCoordinate(x[k], a.dx, size[k]);
             near[k] = nearestIndex(sw[k], size[k]);
+            o[k] = sw[k] - floor(sw[k] + 0.5);
         }
+        code = frameCode(o);
+        frame[i] = (uint8_t) code;
+    }
     for (int k = 0; k < 3; k++) {
         sw[k] = __shfl(sw[k], 0, FEATURE_LANES);
         near[k] = __shfl(near[k], 0, FEATURE_LANES);
     }
+    code = __shfl(code, 0, FEATURE_LANES);
+    const Frame fr = frameOf((unsigned) code);
     if (active) {
         int openAxis[3] = {a.g.openX, a.g.openY, a.g.openZ};
         _Float16* xi = X + (size_t) i*MLP_IN_PAD;
@@ -81,29 +137,35 @@ __global__ void __launch_bounds__(BLOCK) closureFeatures(const S* __restrict__ s
                 for (int c = 0; c < 3; c++)
                     centreVelocity[3*(size_t) i + c] = u[c];
             }
+            // This node's place in the particle's frame, and its velocity there.
+            int slot = 13;
+            for (int k = 0, stride = 1; k < 3; k++, stride *= 3)
+                slot += stride*(fr.sign[k] < 0 ? -pick(d, fr.axis[k]) : pick(d, fr.axis[k]));
             for (int c = 0; c < 3; c++) {
-                float r = (float) u[c];
+                float r = (float) (fr.sign[c]*pick(u, fr.axis[c]));
                 float f = r*norm.uScale;
                 if (!(fabsf(f) <= norm.uBound))
                     out = true;
-                xi[3 + 3*lane + c] = (_Float16) f;
+                xi[3 + 3*slot + c] = (_Float16) f;
                 if (ri)
-                    ri[3 + 3*lane + c] = r;
+                    ri[3 + 3*slot + c] = r;
             }
         }
         else if (lane == 27) {
+            float r[3];
+            for (int k = 0; k < 3; k++)
+                r[k] = (float) (2.0*(sw[k] - floor(sw[k] + 0.5)));
             for (int c = 0; c < 3; c++) {
-                double s = sw[c];
-                float r = (float) (2.0*(s - floor(s + 0.5)));
-                xi[c] = (_Float16) r;
+                float q = fabsf(pick(r, fr.axis[c]));
+                xi[c] = (_Float16) q;
                 if (ri)
-                    ri[c] = r;
+                    ri[c] = q;
             }
         }
         else if (lane == 28) {
             const double* v = velocities + 3*(size_t) part;
             for (int c = 0; c < 3; c++) {
-                float r = (float) (v[c]/a.velocityScale);
+                float r = (float) (fr.sign[c]*pick(v, fr.axis[c])/a.velocityScale);
                 float f = r*norm.vScale;
                 if (!(fabsf(f) <= norm.vBound))
                     out = true;
@@ -142,7 +204,8 @@ __global__ void __launch_bounds__(BLOCK) closureField(const S* __restrict__ src,
 template <typename S>
 __global__ void __launch_bounds__(BLOCK) closureTargets(const S* __restrict__ src, const uint32_t* __restrict__ mask,
         const double* __restrict__ positions, const int* __restrict__ particles, const double* __restrict__ ucm,
-        ParticleArgs a, const double* __restrict__ centreVelocity, float* __restrict__ targets) {
+        ParticleArgs a, const double* __restrict__ centreVelocity, const uint8_t* __restrict__ frame,
+        float* __restrict__ targets) {
     constexpr int LANES = 64, GROUPS = BLOCK/LANES;
     int lane = threadIdx.x % LANES;
     int i = blockIdx.x*GROUPS + threadIdx.x/LANES;
@@ -150,16 +213,22 @@ __global__ void __launch_bounds__(BLOCK) closureTargets(const S* __restrict__ sr
     int part = active ? particles[i] : 0;
     double u[3];
     interpolateGroup<S, LANES>(src, mask, positions + 3*(size_t) part, active, lane, ucm, a, u);
-    if (active && lane == 0)
+    if (active && lane == 0) {
+        const Frame fr = frameOf(frame[i]);
+        double du[3];
         for (int c = 0; c < 3; c++)
-            targets[3*(size_t) i + c] = (float) (u[c] - centreVelocity[3*(size_t) i + c]);
+            du[c] = u[c] - centreVelocity[3*(size_t) i + c];
+        for (int c = 0; c < 3; c++)
+            targets[3*(size_t) i + c] = (float) (fr.sign[c]*pick(du, fr.axis[c]));
+    }
 }
 
 __global__ void __launch_bounds__(BLOCK) closureCouple(const double* __restrict__ velocities,
         const int* __restrict__ particles, const double* __restrict__ mass, ParticleArgs a,
         const uint32_t* __restrict__ centreNode, const double* __restrict__ centreVelocity, const float* __restrict__ du,
-        const uint8_t* __restrict__ ood, double* __restrict__ particleForce, unsigned long long* __restrict__ reactionFixed,
-        uint32_t* __restrict__ touched, unsigned long long* __restrict__ counters) {
+        const uint8_t* __restrict__ ood, const uint8_t* __restrict__ frame, double* __restrict__ particleForce,
+        unsigned long long* __restrict__ reactionFixed, uint32_t* __restrict__ touched,
+        unsigned long long* __restrict__ counters) {
     int i = blockIdx.x*BLOCK + threadIdx.x;
     bool active = i < a.numParticles;
     bool fallback = active && ood[i] != 0;
@@ -183,9 +252,16 @@ __global__ void __launch_bounds__(BLOCK) closureCouple(const double* __restrict_
     }
     if (!network)
         return;
+    // du is in the particle's frame: lattice component p_k is s_k du_k.
+    const Frame fr = frameOf(frame[i]);
+    double dq[3];
+    for (int k = 0; k < 3; k++)
+        dq[k] = fr.sign[k]*(double) du[3*(size_t) i + k];
     double u[3];
-    for (int c = 0; c < 3; c++)
-        u[c] = centreVelocity[3*(size_t) i + c] + (double) du[3*(size_t) i + c];
+    for (int c = 0; c < 3; c++) {
+        double back = fr.axis[0] == c ? dq[0] : (fr.axis[1] == c ? dq[1] : dq[2]);
+        u[c] = centreVelocity[3*(size_t) i + c] + back;
+    }
     double f[3];
     dragForce(velocities + 3*(size_t) particles[i], u, (uint32_t) i, mass[i], a, f);
     for (int c = 0; c < 3; c++)
@@ -249,12 +325,12 @@ unsigned gridOf(int64_t items, int perBlock) {
 void launchClosureFeatures(int storage, const void* src, const uint32_t* mask, const double* positions,
         const double* velocities, const int* particles, const double* ucm, const ParticleArgs& a, const ClosureNorm& norm,
         const double* field, _Float16* X, float* raw, uint32_t* centreNode, double* centreVelocity, uint8_t* ood,
-        hipStream_t stream) {
+        uint8_t* frame, hipStream_t stream) {
     if (a.numParticles == 0)
         return;
     unsigned grid = gridOf(a.numParticles, BLOCK/FEATURE_LANES);
     LBM_CLOSURE_STORAGE(storage, (closureFeatures<S><<<grid, BLOCK, 0, stream>>>((const S*) src, mask, positions,
-            velocities, particles, ucm, a, norm, field, X, raw, centreNode, centreVelocity, ood)))
+            velocities, particles, ucm, a, norm, field, X, raw, centreNode, centreVelocity, ood, frame)))
 }
 
 void launchClosureField(int storage, const void* src, const uint32_t* mask, const double* ucm, const ParticleArgs& a,
@@ -264,23 +340,23 @@ void launchClosureField(int storage, const void* src, const uint32_t* mask, cons
 }
 
 void launchClosureTargets(int storage, const void* src, const uint32_t* mask, const double* positions,
-        const int* particles, const double* ucm, const ParticleArgs& a, const double* centreVelocity, float* targets,
-        hipStream_t stream) {
+        const int* particles, const double* ucm, const ParticleArgs& a, const double* centreVelocity,
+        const uint8_t* frame, float* targets, hipStream_t stream) {
     if (a.numParticles == 0)
         return;
     unsigned grid = gridOf(a.numParticles, BLOCK/64);
     LBM_CLOSURE_STORAGE(storage, (closureTargets<S><<<grid, BLOCK, 0, stream>>>((const S*) src, mask, positions,
-            particles, ucm, a, centreVelocity, targets)))
+            particles, ucm, a, centreVelocity, frame, targets)))
 }
 
 void launchClosureCouple(const double* velocities, const int* particles, const double* mass, const ParticleArgs& a,
         const uint32_t* centreNode, const double* centreVelocity, const float* du, const uint8_t* ood,
-        double* particleForce, unsigned long long* reactionFixed, uint32_t* touched, unsigned long long* counters,
-        hipStream_t stream) {
+        const uint8_t* frame, double* particleForce, unsigned long long* reactionFixed, uint32_t* touched,
+        unsigned long long* counters, hipStream_t stream) {
     if (a.numParticles == 0)
         return;
     closureCouple<<<gridOf(a.numParticles, BLOCK), BLOCK, 0, stream>>>(velocities, particles, mass, a, centreNode,
-            centreVelocity, du, ood, particleForce, reactionFixed, touched, counters);
+            centreVelocity, du, ood, frame, particleForce, reactionFixed, touched, counters);
 }
 
 int closureCoupleBlocks(int numParticles) {

@@ -149,6 +149,7 @@ struct lbm_context {
     double* closureU = nullptr;               // u(nearest), lattice units
     float* closureDu = nullptr;               // network output
     uint8_t* closureOod = nullptr;            // out-of-distribution flags
+    uint8_t* closureFrame = nullptr;          // each particle's frame (reflections and axis order of its inputs)
     unsigned long long* closureCounters = nullptr;   // couplings through the network / the fallback
     double* closureField = nullptr;           // node velocities [node][3], when the particles are dense
     // Centred drag: its solver state and the other forces of the host path.
@@ -354,11 +355,12 @@ void couple(lbm_context* c, const double* dPos, const double* dVel, const double
         if (c->closureField != nullptr)
             launchClosureField(c->p.storage, c->popA, c->mask, c->ucm, a, c->closureField, c->stream);
         launchClosureFeatures(c->p.storage, c->popA, c->mask, dPos, dVel, c->particles, c->ucm, a, norm,
-                c->closureField, c->closureX, nullptr, c->closureNode, c->closureU, c->closureOod, c->stream);
+                c->closureField, c->closureX, nullptr, c->closureNode, c->closureU, c->closureOod, c->closureFrame,
+                c->stream);
         launchMlpInfer(mlpDeviceWeights(c->closure), c->closureX, c->numParticles, norm.outScale, c->closureDu,
                 c->stream);
         launchClosureCouple(dVel, c->particles, c->mass, a, c->closureNode, c->closureU, c->closureDu, c->closureOod,
-                c->particleForce, c->reactionFixed, c->touched, c->closureCounters, c->stream);
+                c->closureFrame, c->particleForce, c->reactionFixed, c->touched, c->closureCounters, c->stream);
         launchClosureFallback(c->p.storage, c->popA, c->mask, dPos, dVel, c->particles, c->mass, c->ucm, a,
                 c->closureOod, c->particleForce, c->reactionFixed, c->touched, partials, c->stream);
         return;
@@ -521,7 +523,7 @@ void destroy(lbm_context* c) {
                        c->bins.slotWeight, c->momentField, c->axisStencils,
                        c->ucm, c->wallPartials, c->particles, c->mass, c->particleForce, c->reactionFixed,
                        c->touched, c->stagePos, c->stageVel, c->closureX, c->closureNode, c->closureU, c->closureDu,
-                       c->closureOod, c->closureCounters, c->closureField, c->stageForce};
+                       c->closureOod, c->closureFrame, c->closureCounters, c->closureField, c->stageForce};
     for (void* b : buffers)
         if (b)
             (void) hipFree(b);
@@ -1167,17 +1169,19 @@ int lbm_closure_pairs(lbm_context* c, const double* positions, const double* vel
         uint32_t* node = deviceAlloc<uint32_t>(n);
         double* u = deviceAlloc<double>(3*(size_t) n);
         uint8_t* ood = deviceAlloc<uint8_t>(n);
+        uint8_t* frame = deviceAlloc<uint8_t>(n);
         float* t = deviceAlloc<float>(3*(size_t) n);
         launchClosureFeatures(c->p.storage, c->popA, c->mask, c->stagePos, c->stageVel, c->particles, c->ucm, a, norm,
-                nullptr, X, raw, node, u, ood, c->stream);
-        launchClosureTargets(c->p.storage, c->popA, c->mask, c->stagePos, c->particles, c->ucm, a, u, t, c->stream);
+                nullptr, X, raw, node, u, ood, frame, c->stream);
+        launchClosureTargets(c->p.storage, c->popA, c->mask, c->stagePos, c->particles, c->ucm, a, u, frame, t,
+                c->stream);
         hipError_t e = hipGetLastError();
         if (e == hipSuccess)
             e = hipMemcpyAsync(features, raw, (size_t) n*MLP_IN*sizeof(float), hipMemcpyDeviceToHost, c->stream);
         if (e == hipSuccess)
             e = hipMemcpyAsync(targets, t, 3*(size_t) n*sizeof(float), hipMemcpyDeviceToHost, c->stream);
         (void) hipStreamSynchronize(c->stream);
-        void* scratch[] = {X, raw, node, u, ood, t};
+        void* scratch[] = {X, raw, node, u, ood, frame, t};
         for (void* b : scratch)
             (void) hipFree(b);
         HIP_CHECK(e);
@@ -1199,6 +1203,7 @@ int lbm_set_closure(lbm_context* c, lbm_mlp* mlp) {
                 c->closureU = deviceAlloc<double>(3*n);
                 c->closureDu = deviceAlloc<float>(3*n);
                 c->closureOod = deviceAlloc<uint8_t>(n);
+                c->closureFrame = deviceAlloc<uint8_t>(n);
                 const size_t counters = 2*(size_t) closureCoupleBlocks(c->numParticles);
                 c->closureCounters = deviceAlloc<unsigned long long>(counters);
                 HIP_CHECK(hipMemsetAsync(c->closureCounters, 0, counters*sizeof(unsigned long long), c->stream));

@@ -49,10 +49,12 @@ void launchMlpInfer(const MlpDevice* weights, const _Float16* X, int n, float ou
 
 /*
  * The closure's coupling kernels (lbm_closure.hip) for the coupled particles of one context.
- *  features: normalized FP16 inputs X [n][96] (and raw FP32 inputs [n][87] if raw != nullptr), the nearest node and
- *            its FP64 velocity, and the out-of-distribution flag of every particle;
- *  targets:  u_Keys - u(nearest) of every particle (training pairs; a.stencil must be Keys);
- *  couple:   nearest-node drag with u(nearest) + du for in-distribution particles, reaction on the nearest node;
+ *  features: normalized FP16 inputs X [n][96] (and raw FP32 inputs [n][87] if raw != nullptr) in each particle's
+ *            frame (the reflection and axis order that make its sub-node offset non-negative and sorted), the frame's
+ *            code, the nearest node and its FP64 velocity, and the out-of-distribution flag of every particle;
+ *  targets:  u_Keys - u(nearest) of every particle in its frame (training pairs; a.stencil must be Keys);
+ *  couple:   nearest-node drag with u(nearest) + du (du turned back from the particle's frame) for in-distribution
+ *            particles, reaction on the nearest node;
  *            counters[2b] and counters[2b + 1] count the particles of work-group b coupled through the network and
  *            left to the fallback (closureCoupleBlocks(n) work-groups);
  *  fallback: the stencil coupling (a.stencil) of the out-of-distribution particles, wall momentum into
@@ -61,21 +63,21 @@ void launchMlpInfer(const MlpDevice* weights, const _Float16* X, int n, float ou
 void launchClosureFeatures(int storage, const void* src, const uint32_t* mask, const double* positions,
         const double* velocities, const int* particles, const double* ucm, const ParticleArgs& a, const ClosureNorm& norm,
         const double* field, _Float16* X, float* raw, uint32_t* centreNode, double* centreVelocity, uint8_t* ood,
-        hipStream_t stream);
+        uint8_t* frame, hipStream_t stream);
 /* The node velocities u = j/rho of the whole lattice, FP64 [node][3] (zero at solid nodes), for
    launchClosureFeatures(field = ...) when 27 x particles exceed the nodes: one coalesced pass instead of 27 scattered
    pulls per particle.  The values are those the per-particle path forms, bit for bit. */
 void launchClosureField(int storage, const void* src, const uint32_t* mask, const double* ucm, const ParticleArgs& a,
         double* field, hipStream_t stream);
 void launchClosureTargets(int storage, const void* src, const uint32_t* mask, const double* positions,
-        const int* particles, const double* ucm, const ParticleArgs& a, const double* centreVelocity, float* targets,
-        hipStream_t stream);
+        const int* particles, const double* ucm, const ParticleArgs& a, const double* centreVelocity,
+        const uint8_t* frame, float* targets, hipStream_t stream);
 /* Work-groups of launchClosureCouple for numParticles particles: its counters hold two counts per work-group. */
 int closureCoupleBlocks(int numParticles);
 void launchClosureCouple(const double* velocities, const int* particles, const double* mass, const ParticleArgs& a,
         const uint32_t* centreNode, const double* centreVelocity, const float* du, const uint8_t* ood,
-        double* particleForce, unsigned long long* reactionFixed, uint32_t* touched, unsigned long long* counters,
-        hipStream_t stream);
+        const uint8_t* frame, double* particleForce, unsigned long long* reactionFixed, uint32_t* touched,
+        unsigned long long* counters, hipStream_t stream);
 int closureFallbackBlocks(int numParticles);
 void launchClosureFallback(int storage, const void* src, const uint32_t* mask, const double* positions,
         const double* velocities, const int* particles, const double* mass, const double* ucm, const ParticleArgs& a,

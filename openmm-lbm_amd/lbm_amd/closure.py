@@ -58,7 +58,9 @@ def _lib_of(library):
 
 
 def pairs(context, positions=None, velocities=None):
-    """Training pairs at the context's current state: (features [n][87], targets [n][3]) of its coupled particles."""
+    """Training pairs at the context's current state: (features [n][87], targets [n][3]) of its coupled particles,
+    each in the particle's frame (the reflection and axis order that make its sub-node offset non-negative and
+    sorted; include/lbm_amd_closure.h)."""
     lib = _lib_of(context.lib)
     pos = np.ascontiguousarray(context.positions if positions is None else positions, dtype=np.float64)
     vel = np.ascontiguousarray(context.velocities if velocities is None else velocities, dtype=np.float64)
@@ -122,7 +124,7 @@ class MLPClosure:
         return losses
 
     def forward(self, features):
-        """The network's Du (lattice units) for raw inputs."""
+        """The network's Du (lattice units, in the frame of the inputs) for raw inputs."""
         x = _f32(features, (-1, NUM_INPUTS))
         out = np.zeros((len(x), NUM_OUTPUTS), dtype=np.float32)
         self._check(self.lib.lbm_mlp_forward(self.handle, x.ctypes.data_as(_F), len(x), out.ctypes.data_as(_F)))

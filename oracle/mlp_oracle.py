@@ -10,13 +10,20 @@ The closure has no counterpart in the reference.  What it is built from does:
 * the Keys stencil: first node floor(s) - 1, four nodes per axis, phi(r) = 1 - 2.5 r^2 + 1.5 |r|^3 for |r| <= 1,
   2 - 4|r| + 2.5 r^2 - 0.5 |r|^3 for 1 < |r| < 2 (LBMStencils.h:42-104, slots x fastest, ReferenceLBMKernels.cpp:998-1018).
 
-Restated here in numpy, FP64:
+Restated here in numpy, FP64, first in the lattice's frame (lattice_features, lattice_targets):
 
-  features(x) = [ 2 (s - round(s))            3 values, the sub-node offset of the bead, scaled to [-1, 1)
+  features(x) = [ 2 (s - round(s))            3 values, the sub-node offset o of the bead, scaled to [-1, 1)
                   u(nearest + (a, b, c))      81 values: a, b, c in {-1, 0, 1}, a fastest, then b, then c; xyz each
                   v dt/dx ]                   3 values, the bead velocity in lattice units
   target(x)   = u_Keys(x) - u(nearest)        3 values, lattice units: (F_Keys - F_nearest)/(gamma m) of the explicit
                                               drag F = -gamma m (v - u) at the same state
+
+then in the bead's frame (features, targets), which is what the library's pairs hold and its network sees: the Keys
+stencil and the lattice are symmetric under the 48 reflections and axis permutations of the cube, and the frame is the
+one of them that makes the offset non-negative and sorted.  Frame axis i is lattice axis p_i (|o| in decreasing order,
+equal values in axis order), with sign s_i = -1 where o_(p_i) < 0; a vector w has w'_i = s_i w_(p_i), the node
+nearest + d sits at d'_i = s_i d_(p_i) of the block, and the offset becomes |o_(p_i)|.  to_frame / from_frame map
+vectors both ways.
 
 and the network: 87 -> 64 -> 64 -> 3, SiLU activations, on normalized inputs (offsets as above, velocities times
 u_scale, bead velocity times v_scale; inputs padded with zeros to 96) and normalized targets (times out_scale).  The
@@ -24,8 +31,11 @@ loss of a batch is the mean over beads of (1/3) |y - t|^2.  forward/backward can
 roundings (weights, inputs, hidden activations and back-propagated errors stored as FP16, sums in FP32 or better).
 
 Pinned by tests/test_closure_oracle.py: finite differences of the loss (backward), the Keys stencil's exactness on
-quadratic fields and at nodes, and the target and centre-velocity features against the forces of the FP64 oracle
-library's own nearest-node and Keys coupling (oracle/lbm_oracle.cpp) at the same state.
+quadratic fields and at nodes, the lattice-frame target and centre-velocity features against the forces of the FP64
+oracle library's own nearest-node and Keys coupling (oracle/lbm_oracle.cpp) at the same state, and the frame through
+what the symmetry fixes: the Keys weights of the 3x3x3 block applied in the bead's frame give the bead-frame image of
+the lattice-frame result, and a configuration reflected or with its axes exchanged has the same bead-frame features
+and targets.
 """
 import numpy as np
 
@@ -82,8 +92,8 @@ def node_index(ix, iy, iz, grid):
     return (ix % nx) + nx*((iy % ny) + ny*(iz % nz))
 
 
-def features(state, grid, dx, dt, positions, velocities, ucm=None):
-    """Raw (unnormalized) features, FP64, [n][87]."""
+def lattice_features(state, grid, dx, dt, positions, velocities, ucm=None):
+    """Raw (unnormalized) features in the lattice's frame, FP64, [n][87]."""
     u = node_velocities(state, ucm)
     s, idx = nearest(positions, dx, grid)
     n = len(s)
@@ -121,11 +131,70 @@ def keys_velocity(u, grid, dx, positions):
     return out
 
 
-def targets(state, grid, dx, positions, ucm=None):
+def lattice_targets(state, grid, dx, positions, ucm=None):
+    """u_Keys(x) - u(nearest) in the lattice's frame, [n][3]."""
     u = node_velocities(state, ucm)
     _, idx = nearest(positions, dx, grid)
     centre = u[node_index(idx[:, 0], idx[:, 1], idx[:, 2], grid)]
     return keys_velocity(u, grid, dx, positions) - centre
+
+
+def frame(offsets):
+    """The bead's frame for sub-node offsets o [n][3]: (p [n][3] lattice axis of each frame axis, s [n][3] signs)."""
+    o = np.asarray(offsets, dtype=np.float64)
+    p = np.argsort(-np.abs(o), axis=1, kind="stable")
+    s = np.where(np.take_along_axis(o, p, axis=1) < 0, -1.0, 1.0)
+    return p, s
+
+
+def frame_of(positions, dx, grid):
+    s, _ = nearest(positions, dx, grid)
+    return frame(s - np.floor(s + 0.5))
+
+
+def to_frame(w, p, s):
+    """Vectors [n][3] of the lattice's frame in the bead's frame: w'_i = s_i w_(p_i)."""
+    return np.take_along_axis(np.asarray(w, dtype=np.float64), p, axis=1)*s
+
+
+def from_frame(w, p, s):
+    """Vectors [n][3] of the bead's frame back in the lattice's frame."""
+    out = np.empty(np.shape(w))
+    np.put_along_axis(out, p, np.asarray(w, dtype=np.float64)*s, axis=1)
+    return out
+
+
+def features_to_frame(raw, p, s):
+    """Lattice-frame features [n][87] in the bead's frame."""
+    raw = np.asarray(raw, dtype=np.float64)
+    n = len(raw)
+    out = np.empty_like(raw)
+    out[:, 0:3] = np.abs(np.take_along_axis(raw[:, 0:3], p, axis=1))
+    rows = np.arange(n)
+    for c in (-1, 0, 1):
+        for b in (-1, 0, 1):
+            for a in (-1, 0, 1):
+                d_frame = np.array([a, b, c])
+                d = np.zeros((n, 3), dtype=np.int64)
+                np.put_along_axis(d, p, (s*d_frame[None, :]).astype(np.int64), axis=1)
+                src = (d[:, 0] + 1) + 3*(d[:, 1] + 1) + 9*(d[:, 2] + 1)
+                dst = (a + 1) + 3*(b + 1) + 9*(c + 1)
+                u = raw[rows[:, None], 3 + 3*src[:, None] + np.arange(3)[None, :]]
+                out[:, 3 + 3*dst:6 + 3*dst] = to_frame(u, p, s)
+    out[:, 84:87] = to_frame(raw[:, 84:87], p, s)
+    return out
+
+
+def features(state, grid, dx, dt, positions, velocities, ucm=None):
+    """Raw (unnormalized) features in each bead's frame, FP64, [n][87]: the library's training features."""
+    p, s = frame_of(positions, dx, grid)
+    return features_to_frame(lattice_features(state, grid, dx, dt, positions, velocities, ucm), p, s)
+
+
+def targets(state, grid, dx, positions, ucm=None):
+    """u_Keys(x) - u(nearest) in each bead's frame, [n][3]: the library's training targets."""
+    p, s = frame_of(positions, dx, grid)
+    return to_frame(lattice_targets(state, grid, dx, positions, ucm), p, s)
 
 
 def normalize(raw, u_scale, v_scale):

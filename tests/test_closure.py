@@ -159,7 +159,8 @@ def test_out_of_distribution_particles_take_the_keys_coupling():
 
 def test_closure_force_is_nearest_plus_network_correction():
     """At one state: F_closure - F_nearest = friction m dx/dt Du (in distribution) and F_closure = F_Keys (out of
-    distribution), with Du the network's output for the particle's inputs."""
+    distribution), with Du the network's output for the particle's inputs, turned from the particle's frame back to
+    the lattice's."""
     n = 12
     rng = np.random.default_rng(10)
     pos, vel = beads(rng, n, 64)
@@ -171,7 +172,8 @@ def test_closure_force_is_nearest_plus_network_correction():
     # Put roughly a quarter of the particles out of distribution through the bead-velocity bound.
     vb = np.quantile(np.abs(feats[:, 84:87]*norm["v_scale"]).max(axis=1), 0.75)
     net.setNormalization(**dict(norm, v_bound=float(vb)))
-    du = net.forward(feats).astype(np.float64)
+    p, sg = mo.frame_of(pos, 1.0, (n, n, n))
+    du = mo.from_frame(net.forward(feats).astype(np.float64), p, sg)
     ood = (np.abs(feats[:, 84:87]*np.float32(norm["v_scale"])) > np.float32(vb)).any(axis=1)
     net.attach(keys)
     for c in (keys, near):

@@ -5,7 +5,11 @@
   node value at a node;
 * the targets and the centre-node features: against the forces of the FP64 oracle library's own nearest-node and
   Keys coupling at the same state (T = 0), through F = -gamma m (v - u):
-  u_Keys - u_nearest = (F_Keys - F_nearest) dt / (friction mass dx).
+  u_Keys - u_nearest = (F_Keys - F_nearest) dt / (friction mass dx);
+* the bead's frame: offsets non-negative and sorted, vectors mapped there and back; the Keys weights of the 3x3x3
+  block give, applied in the bead's frame, the bead-frame image of their lattice-frame result (a wrong node, sign or
+  axis of the mapping changes the weights' arguments or the values they meet); and a configuration reflected through
+  a lattice plane or with two axes exchanged has the same bead-frame features and targets.
 """
 import os
 import sys
@@ -122,12 +126,18 @@ def test_targets_and_centre_features_match_the_oracle_coupling():
     fk, _ = coupling_forces(state, n, pos, vel, LBMForce.Keys, mass, friction)
     fn, _ = coupling_forces(state, n, pos, vel, LBMForce.NearestNode, mass, friction)
     expected = (fk - fn)*dt/(friction*mass*dx)
-    t = mo.targets(state, (n, n, n), dx, pos)
+    t = mo.lattice_targets(state, (n, n, n), dx, pos)
     assert np.abs(t - expected).max() <= 1e-12*max(1.0, np.abs(expected).max())
-    feats = mo.features(state, (n, n, n), dx, dt, pos, vel)
+    feats = mo.lattice_features(state, (n, n, n), dx, dt, pos, vel)
     centre = feats[:, 3 + 3*13:6 + 3*13]
     assert np.abs(centre - (vel*dt/dx + fn*dt/(friction*mass*dx))).max() < 1e-12
     assert np.all(np.abs(feats[:, 0:3]) <= 1.0)
+    # In the bead's frame: the same target turned, and the centre node stays the centre slot.
+    p, sg = mo.frame_of(pos, dx, (n, n, n))
+    assert np.abs(mo.from_frame(mo.targets(state, (n, n, n), dx, pos), p, sg) - expected).max() \
+        <= 1e-12*max(1.0, np.abs(expected).max())
+    framed = mo.features(state, (n, n, n), dx, dt, pos, vel)
+    assert np.abs(mo.from_frame(framed[:, 42:45], p, sg) - centre).max() < 1e-15
 
 
 def test_neighbour_features_match_the_oracle_fields():
@@ -141,7 +151,7 @@ def test_neighbour_features_match_the_oracle_fields():
     _, u_fields = ctx.getFluidFields()
     u_lat = u_fields*(s.dt/s.dx)
     pos = rng.uniform(0, n, size=(15, 3))
-    feats = mo.features(state, (n, n, n), s.dx, s.dt, pos, np.zeros((15, 3)))
+    feats = mo.lattice_features(state, (n, n, n), s.dx, s.dt, pos, np.zeros((15, 3)))
     _, idx = mo.nearest(pos, s.dx, (n, n, n))
     slot = 0
     for c in (-1, 0, 1):
@@ -150,3 +160,95 @@ def test_neighbour_features_match_the_oracle_fields():
                 nodes = mo.node_index(idx[:, 0] + a, idx[:, 1] + b, idx[:, 2] + c, (n, n, n))
                 assert np.abs(feats[:, 3 + 3*slot:6 + 3*slot] - u_lat[nodes]).max() < 1e-14
                 slot += 1
+
+
+def test_frame_sorts_the_offsets_and_maps_vectors_both_ways():
+    rng = np.random.default_rng(31)
+    o = rng.uniform(-0.5, 0.5, size=(2000, 3))
+    o[:100, 1] = -o[:100, 0]                     # equal magnitudes, opposite signs
+    o[100:200] = -0.5                            # the rounding tie on every axis
+    o[200:300, 2] = 0.0
+    p, sg = mo.frame(o)
+    framed = mo.to_frame(o, p, sg)
+    assert np.all(framed >= 0)
+    assert np.all(framed[:, 0] >= framed[:, 1]) and np.all(framed[:, 1] >= framed[:, 2])
+    assert np.array_equal(np.sort(p, axis=1), np.tile([0, 1, 2], (len(o), 1)))
+    # Equal magnitudes keep the axis order.
+    assert np.all(p[100:200] == [0, 1, 2])
+    tie = np.abs(o[:, 0]) == np.abs(o[:, 1])
+    assert np.all(np.where(p[tie] == 0)[1] < np.where(p[tie] == 1)[1])
+    w = rng.normal(size=o.shape)
+    assert np.array_equal(mo.from_frame(mo.to_frame(w, p, sg), p, sg), w)
+    # The frame is a rotation or reflection: lengths and dot products are kept.
+    v = rng.normal(size=o.shape)
+    assert np.allclose(np.sum(mo.to_frame(w, p, sg)*mo.to_frame(v, p, sg), axis=1), np.sum(w*v, axis=1), atol=1e-14)
+
+
+def block_keys(features):
+    """u_Keys - u(nearest) with the Keys weights of the 3x3x3 block only, from features of either frame."""
+    off = features[:, 0:3]/2.0
+    u = features[:, 3:84].reshape(-1, 27, 3)
+    out = -u[:, 13, :]
+    for c in range(3):
+        for b in range(3):
+            for a in range(3):
+                w = mo.keys_kernel(off[:, 0] - (a - 1))*mo.keys_kernel(off[:, 1] - (b - 1))*mo.keys_kernel(off[:, 2] - (c - 1))
+                out = out + w[:, None]*u[:, a + 3*b + 9*c, :]
+    return out
+
+
+def test_block_keys_weights_commute_with_the_frame():
+    n = 10
+    rng = np.random.default_rng(32)
+    state = random_state(n, rng)
+    pos = rng.uniform(0, n, size=(300, 3))
+    vel = rng.normal(scale=0.2, size=(300, 3))
+    lat = mo.lattice_features(state, (n, n, n), 1.0, 0.01, pos, vel)
+    p, sg = mo.frame_of(pos, 1.0, (n, n, n))
+    framed = mo.features(state, (n, n, n), 1.0, 0.01, pos, vel)
+    ref = mo.to_frame(block_keys(lat), p, sg)
+    assert np.abs(block_keys(framed) - ref).max() <= 1e-14*np.abs(ref).max()
+    assert np.abs(framed[:, 84:87] - mo.to_frame(lat[:, 84:87], p, sg)).max() == 0
+    # Planting a wrong sign in the block mapping is caught.
+    bad = framed.copy()
+    bad[:, 3:84] = bad[:, 3:84].reshape(-1, 27, 3)[:, ::-1, :].reshape(-1, 81)
+    assert np.abs(block_keys(bad) - ref).max() > 1e-3*np.abs(ref).max()
+
+
+def symmetric_image(state, n, R, pos, vel):
+    """The configuration mapped by the lattice symmetry R (a signed permutation matrix: c -> R c)."""
+    C = np.stack([mo.CX, mo.CY, mo.CZ], axis=1)
+    image = C @ R.T
+    qmap = [int(np.where((C == image[q]).all(axis=1))[0][0]) for q in range(19)]
+    z, y, x = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    xyz = np.stack([x.ravel(), y.ravel(), z.ravel()], axis=1)
+    to = (xyz @ R.T) % n
+    node_to = to[:, 0] + n*(to[:, 1] + n*to[:, 2])
+    f = np.asarray(state, dtype=np.float64).reshape(19, -1)
+    out = np.empty_like(f)
+    for q in range(19):
+        out[qmap[q], node_to] = f[q]
+    return out.ravel(), np.mod(pos @ R.T, n), vel @ R.T
+
+
+@pytest.mark.parametrize("name,R", [
+    ("reflect x", np.diag([-1, 1, 1])),
+    ("exchange x, y", np.array([[0, 1, 0], [1, 0, 0], [0, 0, 1]])),
+    ("reflect z, cycle axes", np.array([[0, 0, 1], [1, 0, 0], [0, -1, 0]])),
+])
+def test_bead_frame_inputs_are_invariant_under_lattice_symmetries(name, R):
+    n = 10
+    rng = np.random.default_rng(33)
+    state = random_state(n, rng)
+    pos = rng.uniform(0, n, size=(200, 3))
+    vel = rng.normal(scale=0.2, size=(200, 3))
+    f0 = mo.features(state, (n, n, n), 1.0, 0.01, pos, vel)
+    t0 = mo.targets(state, (n, n, n), 1.0, pos)
+    st, ps, vs = symmetric_image(state, n, R, pos, vel)
+    f1 = mo.features(st, (n, n, n), 1.0, 0.01, ps, vs)
+    t1 = mo.targets(st, (n, n, n), 1.0, ps)
+    assert np.abs(f1 - f0).max() <= 1e-12*np.abs(f0).max(), name
+    assert np.abs(t1 - t0).max() <= 1e-12*np.abs(t0).max(), name
+    # The lattice-frame features do change (the test is not vacuous).
+    assert np.abs(mo.lattice_features(st, (n, n, n), 1.0, 0.01, ps, vs)
+                  - mo.lattice_features(state, (n, n, n), 1.0, 0.01, pos, vel)).max() > 1e-3
