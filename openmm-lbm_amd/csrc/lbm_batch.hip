@@ -13,6 +13,7 @@
 #include "../../include/lbm_amd_batch.h"
 #include <algorithm>
 #include <chrono>
+#include <cstdlib>
 #include <sstream>
 #include <stdexcept>
 
@@ -67,6 +68,7 @@ struct lbm_batch {
     uint32_t numTiles = 0;
     int threads = FUSED_THREADS;
     int coupleThreads = BLOCK;              // work-group size of the coupling (particles per tile: coupleThreads/lanes)
+    int xcdOrder = 1;                       // bit 0: coupling tiles, bit 1: fused tiles in XCD-contiguous runs
     int numParticles = 0;
     uint16_t* particleReplica = nullptr;
     uint16_t* particleTiles = nullptr;      // replica of each work-group of the coupling
@@ -127,7 +129,7 @@ void buildTiles(lbm_batch* b, int threads) {
 
 void launchStep(lbm_batch* b, bool timed = false) {
     launchCoupleBatch(b->stencil, b->coupleThreads, b->dDesc, b->particleTiles, b->numParticleTiles, b->pos, b->vel,
-            b->mass, b->force, b->stepCounter, b->stream);
+            b->mass, b->force, b->stepCounter, (b->xcdOrder & 1) != 0, b->stream);
     std::pair<hipEvent_t, hipEvent_t> ev{nullptr, nullptr};
     if (timed) {
         hipCheck(hipEventCreate(&ev.first), "event");
@@ -135,7 +137,8 @@ void launchStep(lbm_batch* b, bool timed = false) {
         b->events.push_back(ev);
         hipCheck(hipEventRecord(ev.first, b->stream), "event");
     }
-    launchFusedBatch(b->threads, b->fluct, b->dDesc, b->tileReplica, b->numTiles, b->stepCounter, b->stream);
+    launchFusedBatch(b->threads, b->fluct, b->dDesc, b->tileReplica, b->numTiles, b->stepCounter,
+            (b->xcdOrder & 2) != 0, b->stream);
     if (timed)
         hipCheck(hipEventRecord(ev.second, b->stream), "event");
     launchVerletBatch(b->dDesc, b->particleReplica, b->numParticles, b->pos, b->vel, b->force, b->invMass,
@@ -232,6 +235,8 @@ void createBatch(lbm_batch* b, int R, const lbm_params* params, const int* numPa
             throw BatchError(LBM_ERROR_INVALID, "lbm_batch_create: negative particle count");
     }
     hipCheck(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking), "stream");
+    if (const char* xcdEnv = getenv("LBM_AMD_BATCH_XCD"))
+        b->xcdOrder = atoi(xcdEnv) & 3;
     b->desc.resize(R);
     int particleBegin = 0;
     std::vector<uint16_t> particleReplica;

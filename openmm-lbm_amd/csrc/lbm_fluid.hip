@@ -143,9 +143,10 @@ fusedRows(const S* __restrict__ src, S* __restrict__ dst,
  */
 template <bool FLUCT, int THREADS>
 __global__ void __launch_bounds__(THREADS) fusedBatch(const ReplicaDesc* __restrict__ table,
-        const uint16_t* __restrict__ tileReplica, const int64_t* __restrict__ stepCounter) {
-    const ReplicaDesc* d = table + tileReplica[blockIdx.x];
-    uint32_t node = (blockIdx.x - d->tileBegin)*THREADS + threadIdx.x;
+        const uint16_t* __restrict__ tileReplica, const int64_t* __restrict__ stepCounter, int xcdOrder) {
+    const uint32_t tile = xcdOrder ? xcdContiguousTile(blockIdx.x, gridDim.x) : blockIdx.x;
+    const ReplicaDesc* d = table + tileReplica[tile];
+    uint32_t node = (tile - d->tileBegin)*THREADS + threadIdx.x;
     StepArgs a = d->step;
     if (node >= a.g.n)
         return;
@@ -157,11 +158,11 @@ __global__ void __launch_bounds__(THREADS) fusedBatch(const ReplicaDesc* __restr
 }
 
 void launchFusedBatch(int threads, bool fluct, const ReplicaDesc* table, const uint16_t* tileReplica, uint32_t numTiles,
-        const int64_t* stepCounter, hipStream_t stream) {
+        const int64_t* stepCounter, bool xcdOrder, hipStream_t stream) {
 #define LBM_BATCH_CASE(T) \
     case T: \
-        if (fluct) fusedBatch<true, T><<<numTiles, T, 0, stream>>>(table, tileReplica, stepCounter); \
-        else fusedBatch<false, T><<<numTiles, T, 0, stream>>>(table, tileReplica, stepCounter); \
+        if (fluct) fusedBatch<true, T><<<numTiles, T, 0, stream>>>(table, tileReplica, stepCounter, xcdOrder); \
+        else fusedBatch<false, T><<<numTiles, T, 0, stream>>>(table, tileReplica, stepCounter, xcdOrder); \
         break;
     switch (threads) {
     LBM_BATCH_CASE(256)

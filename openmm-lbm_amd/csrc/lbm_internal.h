@@ -159,12 +159,23 @@ struct ReplicaDesc {
     uint32_t particleTileBegin;  // first work-group of this replica in the coupling grid
 };
 
-/* Host-side launchers (defined in lbm_fluid.hip / lbm_particles.hip). */
+/*
+ * The MI355X hands the work-groups of a launch to its 8 XCDs in turn (work-group b to XCD b mod 8), and each XCD has
+ * its own L2.  The tile of work-group b of n when every XCD takes a contiguous run of the tiles instead, so that the
+ * tiles of one replica share an L2: XCD x takes tiles [x q + min(x, r), ...), q = n / 8, r = n mod 8, a bijection of
+ * [0, n).
+ */
+__device__ __forceinline__ uint32_t xcdContiguousTile(uint32_t b, uint32_t n) {
+    const uint32_t x = b & 7u, i = b >> 3, q = n >> 3, r = n & 7u;
+    return x*q + (x < r ? x : r) + i;
+}
+
+/* Host-side launchers (defined in lbm_fluid.hip / lbm_particles.hip); xcdOrder: tiles by xcdContiguousTile. */
 void launchFusedBatch(int threads, bool fluct, const ReplicaDesc* table, const uint16_t* tileReplica, uint32_t numTiles,
-        const int64_t* stepCounter, hipStream_t stream);
+        const int64_t* stepCounter, bool xcdOrder, hipStream_t stream);
 void launchCoupleBatch(int stencil, int threads, const ReplicaDesc* table, const uint16_t* particleTiles,
         int numParticleTiles, const double* positions, const double* velocities, const double* mass, double* forces,
-        const int64_t* stepCounter, hipStream_t stream);
+        const int64_t* stepCounter, bool xcdOrder, hipStream_t stream);
 int coupleLanes(int stencil);
 void launchVerletBatch(const ReplicaDesc* table, const uint16_t* particleReplica, int numParticles, double* positions,
         double* velocities, const double* forces, const double* invMass, int64_t* stepCounter, hipStream_t stream);

@@ -536,11 +536,12 @@ template <int LANES, int THREADS>
 __global__ void __launch_bounds__(THREADS) coupleBatch(const ReplicaDesc* __restrict__ table,
         const uint16_t* __restrict__ particleTiles, const double* __restrict__ positions,
         const double* __restrict__ velocities, const double* __restrict__ mass, double* __restrict__ forces,
-        const int64_t* __restrict__ stepCounter) {
+        const int64_t* __restrict__ stepCounter, int xcdOrder) {
     constexpr int GROUPS = THREADS/LANES;
     int lane = threadIdx.x % LANES;
-    const ReplicaDesc* d = table + particleTiles[blockIdx.x];
-    int local = (int) (blockIdx.x - d->particleTileBegin)*GROUPS + threadIdx.x/LANES;
+    const uint32_t tile = xcdOrder ? xcdContiguousTile(blockIdx.x, gridDim.x) : blockIdx.x;
+    const ReplicaDesc* d = table + particleTiles[tile];
+    int local = (int) (tile - d->particleTileBegin)*GROUPS + threadIdx.x/LANES;
     bool active = local < d->part.numParticles;
     int pp = d->particleBegin + (active ? local : 0);
     ParticleArgs a = d->part;
@@ -554,7 +555,7 @@ __global__ void __launch_bounds__(THREADS) coupleBatch(const ReplicaDesc* __rest
     const double* myNoise = nullptr;
     if (LANES > 1 && a.randomForce) {
         if (threadIdx.x < 4*GROUPS) {
-            int k = (int) (blockIdx.x - d->particleTileBegin)*GROUPS + threadIdx.x/4;
+            int k = (int) (tile - d->particleTileBegin)*GROUPS + threadIdx.x/4;
             noise[threadIdx.x/4][threadIdx.x & 3] = k < a.numParticles ? particleNormal((uint32_t) k, threadIdx.x & 3, a) : 0.0;
         }
         __syncthreads();
@@ -701,45 +702,45 @@ void launchCouple(int storage, const void* src, const uint32_t* mask, const doub
 template <int LANES>
 static void launchCoupleBatchT(int threads, unsigned grid, const ReplicaDesc* table, const uint16_t* particleTiles,
         const double* positions, const double* velocities, const double* mass, double* forces,
-        const int64_t* stepCounter, hipStream_t stream) {
+        const int64_t* stepCounter, bool xcdOrder, hipStream_t stream) {
     switch (threads) {
     case 64:
         coupleBatch<LANES, 64><<<grid, 64, 0, stream>>>(table, particleTiles, positions, velocities, mass, forces,
-                stepCounter);
+                stepCounter, xcdOrder);
         break;
     case 128:
         coupleBatch<LANES, 128><<<grid, 128, 0, stream>>>(table, particleTiles, positions, velocities, mass, forces,
-                stepCounter);
+                stepCounter, xcdOrder);
         break;
     default:
         coupleBatch<LANES, BLOCK><<<grid, BLOCK, 0, stream>>>(table, particleTiles, positions, velocities, mass, forces,
-                stepCounter);
+                stepCounter, xcdOrder);
         break;
     }
 }
 
 void launchCoupleBatch(int stencil, int threads, const ReplicaDesc* table, const uint16_t* particleTiles,
         int numParticleTiles, const double* positions, const double* velocities, const double* mass, double* forces,
-        const int64_t* stepCounter, hipStream_t stream) {
+        const int64_t* stepCounter, bool xcdOrder, hipStream_t stream) {
     if (numParticleTiles == 0)
         return;
     unsigned grid = (unsigned) numParticleTiles;
     switch (coupleLanes(stencil)) {
     case 8:
         launchCoupleBatchT<8>(threads, grid, table, particleTiles, positions, velocities, mass, forces, stepCounter,
-                stream);
+                xcdOrder, stream);
         break;
     case 32:
         launchCoupleBatchT<32>(threads, grid, table, particleTiles, positions, velocities, mass, forces, stepCounter,
-                stream);
+                xcdOrder, stream);
         break;
     case 64:
         launchCoupleBatchT<64>(threads, grid, table, particleTiles, positions, velocities, mass, forces, stepCounter,
-                stream);
+                xcdOrder, stream);
         break;
     default:
         launchCoupleBatchT<1>(threads, grid, table, particleTiles, positions, velocities, mass, forces, stepCounter,
-                stream);
+                xcdOrder, stream);
         break;
     }
 }

@@ -88,6 +88,26 @@ def test_batch_matches_single_replicas_bit_for_bit():
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("order", ["1", "2", "3"])
+def test_batch_tile_orders_are_bit_identical(order, monkeypatch):
+    """The XCD-contiguous tile orders (LBM_AMD_BATCH_XCD, read when a batch is created) change which work-group
+    runs which tile, not the arithmetic: the batch's state after 23 steps matches the plain order's bit for bit.  A
+    replica of 700 beads gives the coupling 22 tiles, so the order is not the identity."""
+    replicas = uneven_replicas() + [make_replica((24, 24, 24), 0.7, 700, 2.5, 99, rng_seed=5)]
+    states = []
+    for env in ("0", order):
+        monkeypatch.setenv("LBM_AMD_BATCH_XCD", env)
+        batch = Batch(replicas)
+        batch.step(23)
+        batch.synchronize()
+        states.append(([batch.getFluidFields(r) for r in range(len(replicas))], batch.getParticles()))
+    (fields0, parts0), (fields1, parts1) = states
+    for r in range(len(replicas)):
+        assert np.array_equal(fields0[r][0], fields1[r][0]) and np.array_equal(fields0[r][1], fields1[r][1]), r
+        assert np.array_equal(parts0[r][0], parts1[r][0]) and np.array_equal(parts0[r][1], parts1[r][1]), r
+
+
+@pytest.mark.gpu
 def test_batch_autotune_keeps_the_state():
     """The autotuner tries nine tile shapes on the batch itself, which keeps stepping: the result after it is the
     single-lattice run of the same number of steps, bit for bit (tile shapes change the work split, not the
