@@ -655,11 +655,4 @@ __device__ __forceinline__ bool isSolid(const uint32_t* __restrict__ mask, uint3
     return (mask[node >> 5] >> (node & 31)) & 1u;
 }
 
-/* Block index remapped so that each XCD (blocks are dealt round-robin to the 8 XCDs) works on one
-   contiguous range of the lattice and keeps its neighbouring rows in its own L2. */
-__device__ __forceinline__ uint32_t xcdBlock(uint32_t b, uint32_t nb) {
-    uint32_t per = nb >> 3, rem = nb & 7, xcd = b & 7, idx = b >> 3;
-    return xcd < rem ? xcd*(per + 1) + idx : rem*(per + 1) + (xcd - rem)*per + idx;
-}
-
 }  // namespace lbm
