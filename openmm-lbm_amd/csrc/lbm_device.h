@@ -627,6 +627,17 @@ __device__ __forceinline__ void pullNode(const S* __restrict__ src, const Geomet
 template <bool WIDE>
 using ElementIndex = typename std::conditional<WIDE, uint64_t, uint32_t>::type;
 
+/* Whether the fused pass pulls population Q with a nontemporal (streaming) load or a plain one.  Measured as the
+   coupled step of the 256^3 lattice with 20,000 Keys beads: FP32 is fastest with every pull nontemporal (0.627 ms;
+   0.630 ms with the x-shifted pulls plain, 0.635 ms with all plain); FP16 with the x-shifted pulls plain and the others
+   nontemporal (0.402 ms; 0.404 ms all plain, 0.409 ms all nontemporal): a wavefront's 2-byte pulls span one 128 B
+   line, and an x-shifted one straddles the line of the neighbouring wavefront, which a plain load leaves in L2 for
+   it. */
+template <typename S, int Q>
+constexpr bool pullStreams() {
+    return !isReduced<S>() || CX[Q] == 0;
+}
+
 template <typename S, bool WIDE>
 __device__ __forceinline__ void pullNodeStream(const S* __restrict__ src, uint32_t nx, ElementIndex<WIDE> e,
         const NeighboursOf<typename std::conditional<WIDE, long long, int>::type>& nb,
@@ -634,7 +645,8 @@ __device__ __forceinline__ void pullNodeStream(const S* __restrict__ src, uint32
     if constexpr (WIDE) {
         staticFor<NQ>([&](auto I) {
             constexpr int q = decltype(I)::value;
-            f[q] = loadPopStream(src + (uint64_t) q*nx + (uint64_t) ((long long) e + pullOffset<q>(nb)));
+            const S* p = src + (uint64_t) q*nx + (uint64_t) ((long long) e + pullOffset<q>(nb));
+            f[q] = pullStreams<S, q>() ? loadPopStream(p) : loadPop(p);
         });
     }
     else {
@@ -645,7 +657,8 @@ __device__ __forceinline__ void pullNodeStream(const S* __restrict__ src, uint32
         const uint32_t at = e*B;
         staticFor<NQ>([&](auto I) {
             constexpr int q = decltype(I)::value;
-            f[q] = loadPopStream(atByte(src + (size_t) q*nx, at + pullOffset<q>(nbBytes)));
+            const S* p = atByte(src + (size_t) q*nx, at + pullOffset<q>(nbBytes));
+            f[q] = pullStreams<S, q>() ? loadPopStream(p) : loadPop(p);
         });
     }
 }
