@@ -336,7 +336,7 @@ def block_keys_floor(features, targets):
 def closure_study(steps, warmup, snapshots=10, interval=20, epochs=100, batch=2048):
     """config5: the learned closure (nearest node + MLP, Keys fallback) against the Keys stencil on 128^3 with 100,000
     chained beads.  Generates 10 x 100,000 training pairs from the Keys run itself, trains on the GPU, then times the
-    same coupled steps with Keys and with the closure, and judges the closure's bead forces against Keys at one state
+    same coupled steps with Keys and with the closure, and measures the closure's bead forces against Keys at one state
     (same random forces, so the difference is the interpolation alone)."""
     import numpy as np
     from lbm_amd import _lib, workloads

@@ -19,7 +19,7 @@
  * the block, and the offset inputs are |o_(p_i)|.  The network's output is turned back before the coupling uses it.
  * The reaction goes to the nearest node.  A particle whose inputs fall outside the training range (a
  * normalized velocity beyond the largest seen in training), or whose 3x3x3 neighbourhood holds a solid node or leaves
- * the lattice at an open face, is coupled with the Keys stencil instead: Keys is both the fallback and the judge of
+ * the lattice at an open face, is coupled with the Keys stencil instead: Keys is both the fallback and the yardstick of
  * the network's error.  The context must use the Keys stencil and the explicit drag.
  *
  * Parameters: a flat FP32 vector of LBM_MLP_NUM_PARAMS values, W1[i][o] (96 x 64, inputs padded to 96 with zero

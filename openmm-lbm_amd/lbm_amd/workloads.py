@@ -192,7 +192,7 @@ def config4(num_replicas=64, steps=1000, seed=4242, edges=(32, 48, 64, 96), max_
 def config5(n=128, num_chains=2000, chain_length=50, seed=5150, temperature=300.0):
     """128^3 periodic, tau = 0.7, dx = 1 nm, dt = 0.01 ps, rho0 = 602.214 Da/nm^3; 2,000 chains of 50 beads (100 Da,
     gamma = 10/ps) bonded by harmonic springs k = 1000 kJ/mol/nm^2, r0 = 0.38 nm, started as seeded random walks of
-    step r0 from uniform random origins; fluctuating fluid and particle noise at 300 K; Keys stencil (the judge and
+    step r0 from uniform random origins; fluctuating fluid and particle noise at 300 K; Keys stencil (the yardstick and
     fallback of the closure); removal off."""
     w = Workload("config5", (n, n, n), 0.7)
     rng = np.random.default_rng(seed)
@@ -221,7 +221,7 @@ def config5(n=128, num_chains=2000, chain_length=50, seed=5150, temperature=300.
     w.configure.append(conf)
     w.steps = 1000
     w.description = {"workload": f"config5 {n}^3 periodic, tau=0.7, {num_chains} harmonic chains x {chain_length} beads "
-                                 f"(k=1000, r0=0.38), fluctuating kT={BOLTZ*temperature:.3f}, Keys judge",
+                                 f"(k=1000, r0=0.38), fluctuating kT={BOLTZ*temperature:.3f}, Keys yardstick",
                      "lattice": [n, n, n], "beads": count, "chains": num_chains, "chain_length": chain_length,
                      "dx_nm": w.dx, "dt_ps": w.dt, "tau": 0.7}
     return w
