@@ -17,7 +17,7 @@ from lbm_amd import LBMForce
 from lbm_amd.closure import MLPClosure, pairs, stats, OFF_W1, OFF_B1, OFF_W2, OFF_B2, OFF_W3, OFF_B3, NUM_PARAMS
 from lbm_amd.workloads import BOLTZ
 from parity import Setup
-from test_closure_oracle import random_params, random_state
+from test_closure_oracle import random_params, random_state, symmetric_image
 
 sys.path.insert(0, os.path.join(oracle_lib.ROOT, "oracle"))
 import mlp_oracle as mo  # noqa: E402
@@ -62,6 +62,26 @@ def test_features_and_targets_match_the_oracle(storage):
     print("max |feature - oracle|", np.abs(feats - ref_f).max(), "max |target - oracle|", np.abs(targs - ref_t).max())
     assert np.abs(feats - ref_f).max() <= 1e-6*np.abs(ref_f).max()
     assert np.abs(targs - ref_t).max() <= 1e-6*np.abs(ref_f[:, 3:84]).max()
+
+
+@pytest.mark.parametrize("name,R", [
+    ("reflect y", np.diag([1, -1, 1])),
+    ("exchange y, z, reflect x", np.array([[-1, 0, 0], [0, 0, 1], [0, 1, 0]])),
+])
+def test_gpu_features_are_the_same_for_a_reflected_configuration(name, R):
+    """The library's pairs are in each particle's frame, so a configuration mapped by a symmetry of the lattice (the
+    populations, positions and velocities together) gives the same features and targets, particle by particle."""
+    n = 12
+    rng = np.random.default_rng(41)
+    pos = rng.uniform(0, n, size=(60, 3))
+    vel = rng.normal(scale=0.2, size=(60, 3))
+    state = random_state(n, rng)
+    f0, t0 = pairs(bead_setup(n, state, pos, vel).build())
+    st, ps, vs = symmetric_image(state, n, R, pos, vel)
+    f1, t1 = pairs(bead_setup(n, st, ps, vs).build())
+    print(name, "max |dfeature|", np.abs(f1 - f0).max(), "max |dtarget|", np.abs(t1 - t0).max())
+    assert np.abs(f1 - f0).max() <= 1e-6*np.abs(f0).max()
+    assert np.abs(t1 - t0).max() <= 1e-6*np.abs(f0[:, 3:84]).max()
 
 
 def network_with(rng, scale=0.3):
