@@ -17,7 +17,10 @@
  *    right-hand side.  A product spreads the vector onto the nodes and interpolates it back; the spread accumulates in
  *    64-bit fixed point with a scale set from the largest entry of the vector (2^52 / that entry), so the sums are
  *    exact to 2^-52 of it and independent of the order of the atomics.  Dot products reduce over a fixed grid in a
- *    fixed order.  The host checks the residual after every iteration.
+ *    fixed order.  The device tests the residual after every iteration and the kernels of later iterations then skip
+ *    their work; the host enqueues as many iterations as the previous solve took before it reads the residual, then
+ *    one at a time, so the iterations and results are those of a check after every iteration without a round trip
+ *    to the host for each.
  *
  * The reaction goes into the same fixed-point accumulators as the explicit drag (consumed by the fused pass), and into
  * the wall partial sums at solid nodes.
@@ -37,7 +40,10 @@ constexpr int CG_MAX_ITERATIONS = 1000;
 
 /* Device scalars of the conjugate gradients. */
 enum Scalar {
-    RZ = 0, RZ_NEW = 3, RR = 6, NORM = 9, DAD = 12, ALPHA = 15, BETA = 18, SCALE = 21, INV_SCALE = 22, NUM_SCALARS = 24
+    RZ = 0, RZ_NEW = 3, RR = 6, NORM = 9, DAD = 12, ALPHA = 15, BETA = 18, SCALE = 21, INV_SCALE = 22,
+    DONE = 23,        // 1 once the residual is within tolerance: the remaining iterations' kernels return at once
+    ITERS = 24,       // iterations run in this solve
+    NUM_SCALARS = 25
 };
 
 void check(hipError_t e, const char* what) {
@@ -257,8 +263,19 @@ __global__ void __launch_bounds__(BLOCK) cgSetup(const S* __restrict__ src, cons
     }
 }
 
+/* Every component's residual within CG_TOLERANCE of its right-hand side (the host's test and the device's). */
+__host__ __device__ inline bool residualConverged(const double* s) {
+    for (int k = 0; k < 3; k++)
+        if (s[RR + k] > CG_TOLERANCE*CG_TOLERANCE*s[NORM + k])
+            return false;
+    return true;
+}
+
 /* Largest |x| over the particles and components -> partials[block][0]. */
-__global__ void __launch_bounds__(BLOCK) cgMaxAbs(const double* __restrict__ x, int n, double* __restrict__ partials) {
+__global__ void __launch_bounds__(BLOCK) cgMaxAbs(const double* __restrict__ x, int n, double* __restrict__ partials,
+        const double* done) {
+    if (done != nullptr && *done != 0.0)
+        return;
     double m = 0.0;
     for (int i = blockIdx.x*BLOCK + threadIdx.x; i < 3*n; i += gridDim.x*BLOCK)
         m = fmax(m, fabs(x[i]));
@@ -294,7 +311,10 @@ __device__ __forceinline__ void combinePartials(const double* __restrict__ parti
 }
 
 /* Fixed-point scale of the next spread: 2^(52 - e), 2^e >= max |x| (1 if x is zero). */
-__global__ void __launch_bounds__(64) cgScale(const double* __restrict__ partials, double* __restrict__ scalars) {
+__global__ void __launch_bounds__(64) cgScale(const double* __restrict__ partials, double* __restrict__ scalars,
+        const double* done) {
+    if (done != nullptr && *done != 0.0)
+        return;
     double m[1];
     combinePartials<1, true>(partials, m);
     if (threadIdx.x != 0)
@@ -305,7 +325,10 @@ __global__ void __launch_bounds__(64) cgScale(const double* __restrict__ partial
 }
 
 /* spread[node] += w x (fixed point), one thread per particle slot. */
-__global__ void __launch_bounds__(BLOCK) cgSpread(const double* __restrict__ x, int n, CgArgs c) {
+__global__ void __launch_bounds__(BLOCK) cgSpread(const double* __restrict__ x, int n, CgArgs c,
+        const double* done) {
+    if (done != nullptr && *done != 0.0)
+        return;
     size_t t = (size_t) blockIdx.x*BLOCK + threadIdx.x;
     if (t >= (size_t) n*c.slots)
         return;
@@ -325,7 +348,9 @@ __global__ void __launch_bounds__(BLOCK) cgSpread(const double* __restrict__ x, 
 /* Ax = x (1 + a)/m + a sum_slots (w/rho) spread[node], one group of LANES lanes per particle. */
 template <int LANES>
 __global__ void __launch_bounds__(BLOCK) cgGather(const double* __restrict__ x, double* __restrict__ Ax,
-        const double* __restrict__ mass, int n, double aa, CgArgs c) {
+        const double* __restrict__ mass, int n, double aa, CgArgs c, const double* done) {
+    if (done != nullptr && *done != 0.0)
+        return;
     constexpr int GROUPS = BLOCK/LANES;
     int lane = threadIdx.x % LANES;
     int i = blockIdx.x*GROUPS + threadIdx.x/LANES;
@@ -350,7 +375,9 @@ __global__ void __launch_bounds__(BLOCK) cgGather(const double* __restrict__ x, 
             Ax[3*(size_t) i + k] = x[3*(size_t) i + k]*((1.0 + aa)/mass[i]) + s[k]*aa;
 }
 
-__global__ void __launch_bounds__(BLOCK) cgClear(int n, CgArgs c) {
+__global__ void __launch_bounds__(BLOCK) cgClear(int n, CgArgs c, const double* done) {
+    if (done != nullptr && *done != 0.0)
+        return;
     size_t t = (size_t) blockIdx.x*BLOCK + threadIdx.x;
     if (t >= (size_t) n*c.slots || c.slotW[t] == 0.0)
         return;
@@ -408,10 +435,14 @@ __global__ void __launch_bounds__(64) cgFinishInit(const double* __restrict__ pa
         scalars[NORM + k] = v[3 + k];
         scalars[RR + k] = w[k];
     }
+    scalars[DONE] = residualConverged(scalars) ? 1.0 : 0.0;
+    scalars[ITERS] = 0.0;
 }
 
 /* d . Ad -> alpha = r.z/(d.Ad). */
-__global__ void __launch_bounds__(BLOCK) cgDot(int n, CgArgs c) {
+__global__ void __launch_bounds__(BLOCK) cgDot(int n, CgArgs c, const double* done) {
+    if (done != nullptr && *done != 0.0)
+        return;
     double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     for (int i = blockIdx.x*BLOCK + threadIdx.x; i < n; i += gridDim.x*BLOCK)
         for (int k = 0; k < 3; k++)
@@ -419,7 +450,10 @@ __global__ void __launch_bounds__(BLOCK) cgDot(int n, CgArgs c) {
     reduceBlock(v, 3, c.partials);
 }
 
-__global__ void __launch_bounds__(64) cgFinishAlpha(const double* __restrict__ partials, double* __restrict__ scalars) {
+__global__ void __launch_bounds__(64) cgFinishAlpha(const double* __restrict__ partials, double* __restrict__ scalars,
+        const double* done) {
+    if (done != nullptr && *done != 0.0)
+        return;
     double v[3];
     combinePartials<3>(partials, v);
     if (threadIdx.x != 0)
@@ -431,7 +465,9 @@ __global__ void __launch_bounds__(64) cgFinishAlpha(const double* __restrict__ p
 }
 
 /* F += alpha d, r -= alpha Ad, z = r/diag; partials: r.z (new), r.r. */
-__global__ void __launch_bounds__(BLOCK) cgUpdate(int n, CgArgs c) {
+__global__ void __launch_bounds__(BLOCK) cgUpdate(int n, CgArgs c, const double* done) {
+    if (done != nullptr && *done != 0.0)
+        return;
     double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     double alpha[3] = {c.scalars[ALPHA], c.scalars[ALPHA + 1], c.scalars[ALPHA + 2]};
     for (int i = blockIdx.x*BLOCK + threadIdx.x; i < n; i += gridDim.x*BLOCK)
@@ -449,7 +485,10 @@ __global__ void __launch_bounds__(BLOCK) cgUpdate(int n, CgArgs c) {
 }
 
 /* beta = r.z(new)/r.z(old), r.z <- r.z(new), r.r. */
-__global__ void __launch_bounds__(64) cgFinishBeta(const double* __restrict__ partials, double* __restrict__ scalars) {
+__global__ void __launch_bounds__(64) cgFinishBeta(const double* __restrict__ partials, double* __restrict__ scalars,
+        const double* done) {
+    if (done != nullptr && *done != 0.0)
+        return;
     double v[6];
     combinePartials<6>(partials, v);
     if (threadIdx.x != 0)
@@ -461,10 +500,14 @@ __global__ void __launch_bounds__(64) cgFinishBeta(const double* __restrict__ pa
         scalars[RZ_NEW + k] = v[k];
         scalars[RR + k] = v[3 + k];
     }
+    scalars[DONE] = residualConverged(scalars) ? 1.0 : 0.0;
+    scalars[ITERS] += 1.0;
 }
 
 /* d = z + beta d, and the largest |d| of each block -> partials[block][0] (for the scale of the next spread). */
-__global__ void __launch_bounds__(BLOCK) cgDirection(int n, CgArgs c) {
+__global__ void __launch_bounds__(BLOCK) cgDirection(int n, CgArgs c, const double* done) {
+    if (done != nullptr && *done != 0.0)
+        return;
     double beta[3] = {c.scalars[BETA], c.scalars[BETA + 1], c.scalars[BETA + 2]};
     double m = 0.0;
     for (int i = blockIdx.x*BLOCK + threadIdx.x; i < n; i += gridDim.x*BLOCK)
@@ -538,7 +581,8 @@ struct CentredState {
     // Stencils: conjugate gradients.
     CgArgs cg{};
     double* partialsB = nullptr;
-    double* hostScalars = nullptr;        // pinned: the convergence check reads the scalars once per iteration
+    double* hostScalars = nullptr;        // pinned: the convergence check reads the scalars
+    int lastIterations = 1;               // iterations of the previous solve: enqueued before the first check
     std::vector<void*> buffers;
 };
 
@@ -609,30 +653,29 @@ void centredDestroy(CentredState* st) {
 
 namespace {
 
-/* Ax = A x.  The largest |x| of each block must be in c.partials (cgMaxAbs, or cgDirection for x = d). */
+/* Ax = A x.  The largest |x| of each block must be in c.partials (cgMaxAbs, or cgDirection for x = d).  done: the
+   solve's convergence flag inside the iteration loop (nullptr outside it). */
 template <typename S>
-void multiply(CentredState* st, const double* x, double* Ax, const double* mass, double aa, hipStream_t stream) {
+void multiply(CentredState* st, const double* x, double* Ax, const double* mass, double aa, const double* done,
+        hipStream_t stream) {
     CgArgs& c = st->cg;
     int n = st->n;
     size_t slotItems = (size_t) n*c.slots;
-    cgScale<<<1, 64, 0, stream>>>(c.partials, c.scalars);
-    cgSpread<<<blocksOf(slotItems, BLOCK), BLOCK, 0, stream>>>(x, n, c);
+    cgScale<<<1, 64, 0, stream>>>(c.partials, c.scalars, done);
+    cgSpread<<<blocksOf(slotItems, BLOCK), BLOCK, 0, stream>>>(x, n, c, done);
     switch (st->lanes) {
-    case 8: cgGather<8><<<blocksOf(n, BLOCK/8), BLOCK, 0, stream>>>(x, Ax, mass, n, aa, c); break;
-    case 32: cgGather<32><<<blocksOf(n, BLOCK/32), BLOCK, 0, stream>>>(x, Ax, mass, n, aa, c); break;
-    default: cgGather<64><<<blocksOf(n, BLOCK/64), BLOCK, 0, stream>>>(x, Ax, mass, n, aa, c); break;
+    case 8: cgGather<8><<<blocksOf(n, BLOCK/8), BLOCK, 0, stream>>>(x, Ax, mass, n, aa, c, done); break;
+    case 32: cgGather<32><<<blocksOf(n, BLOCK/32), BLOCK, 0, stream>>>(x, Ax, mass, n, aa, c, done); break;
+    default: cgGather<64><<<blocksOf(n, BLOCK/64), BLOCK, 0, stream>>>(x, Ax, mass, n, aa, c, done); break;
     }
-    cgClear<<<blocksOf(slotItems, BLOCK), BLOCK, 0, stream>>>(n, c);
+    cgClear<<<blocksOf(slotItems, BLOCK), BLOCK, 0, stream>>>(n, c, done);
 }
 
 bool converged(CentredState* st, hipStream_t stream) {
     double* s = st->hostScalars;
     check(hipMemcpyAsync(s, st->cg.scalars, NUM_SCALARS*sizeof(double), hipMemcpyDeviceToHost, stream), "download");
     check(hipStreamSynchronize(stream), "sync");
-    for (int k = 0; k < 3; k++)
-        if (s[RR + k] > CG_TOLERANCE*CG_TOLERANCE*s[NORM + k])
-            return false;
-    return true;
+    return residualConverged(s);
 }
 
 template <typename S>
@@ -670,27 +713,36 @@ void centredCoupleT(CentredState* st, const S* src, const uint32_t* mask, const 
                 g[0], g[1], g[2], c);
         break;
     }
-    cgMaxAbs<<<RED_BLOCKS, BLOCK, 0, stream>>>(c.F, n, c.partials);
-    multiply<S>(st, c.F, c.Ad, mass, aa, stream);
+    cgMaxAbs<<<RED_BLOCKS, BLOCK, 0, stream>>>(c.F, n, c.partials, nullptr);
+    multiply<S>(st, c.F, c.Ad, mass, aa, nullptr, stream);
     cgInit<<<RED_BLOCKS, BLOCK, 0, stream>>>(n, c, st->partialsB);
     cgFinishInit<<<1, 64, 0, stream>>>(c.partials, st->partialsB, c.scalars);
     check(hipGetLastError(), "launch");
-    // Each iteration starts with the new direction (d = z after cgInit), so that the last one computes none.
-    for (int iteration = 0; !converged(st, stream); iteration++) {
-        if (iteration == CG_MAX_ITERATIONS)
+    // Each iteration starts with the new direction (d = z after cgInit), so that the last one computes none.  The
+    // previous solve's count of iterations is enqueued at once (those after convergence do nothing), then the host
+    // reads the residual and adds one iteration at a time until it has converged.
+    const double* done = c.scalars + DONE;
+    int iteration = 0;
+    for (int batch = std::max(st->lastIterations, 1);; batch = 1) {
+        for (int b = 0; b < batch; b++, iteration++) {
+            if (iteration == 0)
+                cgMaxAbs<<<RED_BLOCKS, BLOCK, 0, stream>>>(c.d, n, c.partials, done);
+            else
+                cgDirection<<<RED_BLOCKS, BLOCK, 0, stream>>>(n, c, done);
+            multiply<S>(st, c.d, c.Ad, mass, aa, done, stream);
+            cgDot<<<RED_BLOCKS, BLOCK, 0, stream>>>(n, c, done);
+            cgFinishAlpha<<<1, 64, 0, stream>>>(c.partials, c.scalars, done);
+            cgUpdate<<<RED_BLOCKS, BLOCK, 0, stream>>>(n, c, done);
+            cgFinishBeta<<<1, 64, 0, stream>>>(c.partials, c.scalars, done);
+            check(hipGetLastError(), "launch");
+        }
+        if (converged(st, stream))
+            break;
+        if (iteration >= CG_MAX_ITERATIONS)
             throw std::runtime_error("LBMForce: the centred drag with the interpolation stencil did not converge in 1000 "
                     "iterations of the conjugate gradients");
-        if (iteration == 0)
-            cgMaxAbs<<<RED_BLOCKS, BLOCK, 0, stream>>>(c.d, n, c.partials);
-        else
-            cgDirection<<<RED_BLOCKS, BLOCK, 0, stream>>>(n, c);
-        multiply<S>(st, c.d, c.Ad, mass, aa, stream);
-        cgDot<<<RED_BLOCKS, BLOCK, 0, stream>>>(n, c);
-        cgFinishAlpha<<<1, 64, 0, stream>>>(c.partials, c.scalars);
-        cgUpdate<<<RED_BLOCKS, BLOCK, 0, stream>>>(n, c);
-        cgFinishBeta<<<1, 64, 0, stream>>>(c.partials, c.scalars);
-        check(hipGetLastError(), "launch");
     }
+    st->lastIterations = (int) st->hostScalars[ITERS];
     switch (st->lanes) {
     case 8: cgForces<8><<<groups, BLOCK, 0, stream>>>(n, a, c, particleForce, reactionFixed, touched, partials); break;
     case 32: cgForces<32><<<groups, BLOCK, 0, stream>>>(n, a, c, particleForce, reactionFixed, touched, partials); break;
