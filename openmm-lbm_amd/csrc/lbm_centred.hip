@@ -310,25 +310,28 @@ __device__ __forceinline__ void combinePartials(const double* __restrict__ parti
         }
 }
 
-/* Fixed-point scale of the next spread: 2^(52 - e), 2^e >= max |x| (1 if x is zero). */
-__global__ void __launch_bounds__(64) cgScale(const double* __restrict__ partials, double* __restrict__ scalars,
-        const double* done) {
-    if (done != nullptr && *done != 0.0)
-        return;
-    double m[1];
-    combinePartials<1, true>(partials, m);
-    if (threadIdx.x != 0)
-        return;
-    int e = m[0] > 0 ? ilogb(m[0]) + 1 : 0;
-    scalars[SCALE] = ldexp(1.0, 52 - e);
-    scalars[INV_SCALE] = ldexp(1.0, e - 52);
-}
-
-/* spread[node] += w x (fixed point), one thread per particle slot. */
+/* spread[node] += w x (fixed point), one thread per particle slot.  The fixed-point scale, 2^(52 - e) with
+   2^e >= max |x| (1 if x is zero), is formed by the first wavefront of every work-group from the largest |x| of the
+   blocks in c.partials (one fixed combination, so every work-group has the same value); work-group 0 keeps it and its
+   inverse in the scalars for the gather. */
 __global__ void __launch_bounds__(BLOCK) cgSpread(const double* __restrict__ x, int n, CgArgs c,
         const double* done) {
     if (done != nullptr && *done != 0.0)
         return;
+    __shared__ double sharedScale;
+    if (threadIdx.x < 64) {
+        double m[1];
+        combinePartials<1, true>(c.partials, m);
+        if (threadIdx.x == 0) {
+            int e = m[0] > 0 ? ilogb(m[0]) + 1 : 0;
+            sharedScale = ldexp(1.0, 52 - e);
+            if (blockIdx.x == 0) {
+                c.scalars[SCALE] = sharedScale;
+                c.scalars[INV_SCALE] = ldexp(1.0, e - 52);
+            }
+        }
+    }
+    __syncthreads();
     size_t t = (size_t) blockIdx.x*BLOCK + threadIdx.x;
     if (t >= (size_t) n*c.slots)
         return;
@@ -337,7 +340,7 @@ __global__ void __launch_bounds__(BLOCK) cgSpread(const double* __restrict__ x, 
         return;
     size_t i = t/c.slots;
     uint32_t node = c.slotNode[t];
-    double scale = c.scalars[SCALE];
+    const double scale = sharedScale;
     for (int k = 0; k < 3; k++) {
         long long v = llrint(w*x[3*i + k]*scale);
         if (v != 0)
@@ -439,10 +442,17 @@ __global__ void __launch_bounds__(64) cgFinishInit(const double* __restrict__ pa
     scalars[ITERS] = 0.0;
 }
 
-/* d . Ad -> alpha = r.z/(d.Ad). */
+/* d . Ad into c.partials (alpha = r.z/(d.Ad) follows in cgUpdate); and the node sums of the product, read by the
+   gather, cleared for the next spread. */
 __global__ void __launch_bounds__(BLOCK) cgDot(int n, CgArgs c, const double* done) {
     if (done != nullptr && *done != 0.0)
         return;
+    for (size_t t = (size_t) blockIdx.x*BLOCK + threadIdx.x; t < (size_t) n*c.slots; t += (size_t) gridDim.x*BLOCK)
+        if (c.slotW[t] != 0.0) {
+            uint32_t node = c.slotNode[t];
+            for (int k = 0; k < 3; k++)
+                c.spread[3*(size_t) node + k] = 0ull;
+        }
     double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     for (int i = blockIdx.x*BLOCK + threadIdx.x; i < n; i += gridDim.x*BLOCK)
         for (int k = 0; k < 3; k++)
@@ -450,26 +460,29 @@ __global__ void __launch_bounds__(BLOCK) cgDot(int n, CgArgs c, const double* do
     reduceBlock(v, 3, c.partials);
 }
 
-__global__ void __launch_bounds__(64) cgFinishAlpha(const double* __restrict__ partials, double* __restrict__ scalars,
+/* alpha = r.z/(d.Ad), from the blocks' d.Ad in c.partials (formed by the first wavefront of every work-group, one
+   fixed combination; work-group 0 keeps it in the scalars); F += alpha d, r -= alpha Ad, z = r/diag; partialsOut:
+   r.z (new), r.r (a buffer of its own: other work-groups may still be reading c.partials). */
+__global__ void __launch_bounds__(BLOCK) cgUpdate(int n, CgArgs c, double* __restrict__ partialsOut,
         const double* done) {
     if (done != nullptr && *done != 0.0)
         return;
-    double v[3];
-    combinePartials<3>(partials, v);
-    if (threadIdx.x != 0)
-        return;
-    for (int k = 0; k < 3; k++) {
-        scalars[DAD + k] = v[k];
-        scalars[ALPHA + k] = v[k] > 0 ? scalars[RZ + k]/v[k] : 0.0;
+    __shared__ double sharedAlpha[3];
+    if (threadIdx.x < 64) {
+        double dad[3];
+        combinePartials<3>(c.partials, dad);
+        if (threadIdx.x == 0)
+            for (int k = 0; k < 3; k++) {
+                sharedAlpha[k] = dad[k] > 0 ? c.scalars[RZ + k]/dad[k] : 0.0;
+                if (blockIdx.x == 0) {
+                    c.scalars[DAD + k] = dad[k];
+                    c.scalars[ALPHA + k] = sharedAlpha[k];
+                }
+            }
     }
-}
-
-/* F += alpha d, r -= alpha Ad, z = r/diag; partials: r.z (new), r.r. */
-__global__ void __launch_bounds__(BLOCK) cgUpdate(int n, CgArgs c, const double* done) {
-    if (done != nullptr && *done != 0.0)
-        return;
+    __syncthreads();
     double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    double alpha[3] = {c.scalars[ALPHA], c.scalars[ALPHA + 1], c.scalars[ALPHA + 2]};
+    double alpha[3] = {sharedAlpha[0], sharedAlpha[1], sharedAlpha[2]};
     for (int i = blockIdx.x*BLOCK + threadIdx.x; i < n; i += gridDim.x*BLOCK)
         for (int k = 0; k < 3; k++) {
             size_t t = 3*(size_t) i + k;
@@ -481,7 +494,7 @@ __global__ void __launch_bounds__(BLOCK) cgUpdate(int n, CgArgs c, const double*
             v[k] += r*z;
             v[3 + k] += r*r;
         }
-    reduceBlock(v, 6, c.partials);
+    reduceBlock(v, 6, partialsOut);
 }
 
 /* beta = r.z(new)/r.z(old), r.z <- r.z(new), r.r. */
@@ -654,21 +667,22 @@ void centredDestroy(CentredState* st) {
 namespace {
 
 /* Ax = A x.  The largest |x| of each block must be in c.partials (cgMaxAbs, or cgDirection for x = d).  done: the
-   solve's convergence flag inside the iteration loop (nullptr outside it). */
+   solve's convergence flag inside the iteration loop (nullptr outside it), where cgDot clears the node sums (clear =
+   false); outside it cgClear does. */
 template <typename S>
 void multiply(CentredState* st, const double* x, double* Ax, const double* mass, double aa, const double* done,
-        hipStream_t stream) {
+        bool clear, hipStream_t stream) {
     CgArgs& c = st->cg;
     int n = st->n;
     size_t slotItems = (size_t) n*c.slots;
-    cgScale<<<1, 64, 0, stream>>>(c.partials, c.scalars, done);
     cgSpread<<<blocksOf(slotItems, BLOCK), BLOCK, 0, stream>>>(x, n, c, done);
     switch (st->lanes) {
     case 8: cgGather<8><<<blocksOf(n, BLOCK/8), BLOCK, 0, stream>>>(x, Ax, mass, n, aa, c, done); break;
     case 32: cgGather<32><<<blocksOf(n, BLOCK/32), BLOCK, 0, stream>>>(x, Ax, mass, n, aa, c, done); break;
     default: cgGather<64><<<blocksOf(n, BLOCK/64), BLOCK, 0, stream>>>(x, Ax, mass, n, aa, c, done); break;
     }
-    cgClear<<<blocksOf(slotItems, BLOCK), BLOCK, 0, stream>>>(n, c, done);
+    if (clear)
+        cgClear<<<blocksOf(slotItems, BLOCK), BLOCK, 0, stream>>>(n, c, done);
 }
 
 bool converged(CentredState* st, hipStream_t stream) {
@@ -714,7 +728,7 @@ void centredCoupleT(CentredState* st, const S* src, const uint32_t* mask, const 
         break;
     }
     cgMaxAbs<<<RED_BLOCKS, BLOCK, 0, stream>>>(c.F, n, c.partials, nullptr);
-    multiply<S>(st, c.F, c.Ad, mass, aa, nullptr, stream);
+    multiply<S>(st, c.F, c.Ad, mass, aa, nullptr, true, stream);
     cgInit<<<RED_BLOCKS, BLOCK, 0, stream>>>(n, c, st->partialsB);
     cgFinishInit<<<1, 64, 0, stream>>>(c.partials, st->partialsB, c.scalars);
     check(hipGetLastError(), "launch");
@@ -729,11 +743,10 @@ void centredCoupleT(CentredState* st, const S* src, const uint32_t* mask, const 
                 cgMaxAbs<<<RED_BLOCKS, BLOCK, 0, stream>>>(c.d, n, c.partials, done);
             else
                 cgDirection<<<RED_BLOCKS, BLOCK, 0, stream>>>(n, c, done);
-            multiply<S>(st, c.d, c.Ad, mass, aa, done, stream);
+            multiply<S>(st, c.d, c.Ad, mass, aa, done, false, stream);
             cgDot<<<RED_BLOCKS, BLOCK, 0, stream>>>(n, c, done);
-            cgFinishAlpha<<<1, 64, 0, stream>>>(c.partials, c.scalars, done);
-            cgUpdate<<<RED_BLOCKS, BLOCK, 0, stream>>>(n, c, done);
-            cgFinishBeta<<<1, 64, 0, stream>>>(c.partials, c.scalars, done);
+            cgUpdate<<<RED_BLOCKS, BLOCK, 0, stream>>>(n, c, st->partialsB, done);
+            cgFinishBeta<<<1, 64, 0, stream>>>(st->partialsB, c.scalars, done);
             check(hipGetLastError(), "launch");
         }
         if (converged(st, stream))
