@@ -596,6 +596,7 @@ struct CentredState {
     double* partialsB = nullptr;
     double* hostScalars = nullptr;        // pinned: the convergence check reads the scalars
     int lastIterations = 1;               // iterations of the previous solve: enqueued before the first check
+    bool planned = true;                  // false (LBM_AMD_CG_PLANNED=0): the host reads the residual after each
     std::vector<void*> buffers;
 };
 
@@ -606,6 +607,8 @@ CentredState* centredCreate(int numParticles, int stencil, uint32_t numNodes) {
         st->n = numParticles;
         st->stencil = stencil;
         st->numNodes = numNodes;
+        if (const char* plannedEnv = getenv("LBM_AMD_CG_PLANNED"))
+            st->planned = plannedEnv[0] != '0';
         int w = stencilWidth(stencil);
         st->slots = w*w*w;
         st->lanes = coupleLanes(stencil);
@@ -737,7 +740,7 @@ void centredCoupleT(CentredState* st, const S* src, const uint32_t* mask, const 
     // reads the residual and adds one iteration at a time until it has converged.
     const double* done = c.scalars + DONE;
     int iteration = 0;
-    for (int batch = std::max(st->lastIterations, 1);; batch = 1) {
+    for (int batch = st->planned ? std::max(st->lastIterations, 1) : 1;; batch = 1) {
         for (int b = 0; b < batch; b++, iteration++) {
             if (iteration == 0)
                 cgMaxAbs<<<RED_BLOCKS, BLOCK, 0, stream>>>(c.d, n, c.partials, done);

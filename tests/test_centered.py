@@ -228,6 +228,28 @@ def test_centred_parity_with_fluctuations(stencil):
         assert worst[key] < 1e-5, (key, worst)
 
 
+@pytest.mark.gpu
+@pytest.mark.parametrize("stencil", [LBMForce.Keys, LBMForce.Trilinear], ids=["keys", "trilinear"])
+def test_cg_iterations_enqueued_ahead_are_bit_identical(stencil, monkeypatch):
+    """The conjugate gradients enqueue the previous solve's count of iterations before the host reads the residual,
+    and the device skips those past convergence.  With fluctuating fluid and particle noise the count changes from step
+    to step; the run must be, bit for bit, the one whose host reads the residual after every iteration
+    (LBM_AMD_CG_PLANNED=0, read when the context is created)."""
+    rng = np.random.default_rng(37)
+    n = 10
+    pos = rng.uniform(0, n*DX, size=(150, 3))
+    v0 = rng.normal(scale=0.3, size=(150, 3))
+    runs = []
+    for planned in ("1", "0"):
+        monkeypatch.setenv("LBM_AMD_CG_PLANNED", planned)
+        c = centred_setup([100.0]*150, pos, v0, 20.0, stencil=stencil, n=n, temperature=300.0, fluct=True,
+                          seed=9).build(None)
+        c.step(30)
+        runs.append((c.getFluidState(), c.positions.copy(), c.velocities.copy(), c.getForces()))
+    for a, b in zip(*runs):
+        assert np.array_equal(a, b)
+
+
 SOLVE_BACKENDS = [pytest.param("oracle", id="oracle"),
                   pytest.param("fp64", id="gpu-fp64", marks=pytest.mark.gpu),
                   pytest.param("fp32", id="gpu-fp32", marks=pytest.mark.gpu)]
