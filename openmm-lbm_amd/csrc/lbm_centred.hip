@@ -188,7 +188,17 @@ struct CgArgs {
     double* partials;        // [RED_BLOCKS][8]
     double* scalars;
     int slots;               // w^3
+    // Sparse particles: a product spreads only the slots at nodes that several slots share (sharedNode); a slot alone
+    // at its node contributes (w/rho) w x of its own particle directly.  seen/shared: node bits, cleared by cgForces.
+    int split;
+    uint32_t* seen;
+    uint32_t* shared;
 };
+
+/* Whether a slot's node takes part in the spread of a product: every node without the split, else the shared ones. */
+__device__ __forceinline__ bool spreadsAt(const CgArgs& c, uint32_t node) {
+    return !c.split || ((c.shared[node >> 5] >> (node & 31)) & 1u);
+}
 
 /* One group of LANES lanes per particle, one lane per stencil slot: the slot's node, weight and w/rho, the particle's
    right-hand side, diagonal and F = rhs/diagonal. */
@@ -261,6 +271,18 @@ __global__ void __launch_bounds__(BLOCK) cgSetup(const S* __restrict__ src, cons
             c.F[3*(size_t) i + k] = b*(1.0/diag);
         }
     }
+}
+
+/* The nodes that several fluid slots share (with the split): each slot sets its node's bit in seen, and in shared if
+   the bit was already set.  One thread per particle slot. */
+__global__ void __launch_bounds__(BLOCK) cgMarkShared(int n, CgArgs c) {
+    size_t t = (size_t) blockIdx.x*BLOCK + threadIdx.x;
+    if (t >= (size_t) n*c.slots || c.slotWInvRho[t] == 0.0)
+        return;
+    uint32_t node = c.slotNode[t];
+    uint32_t bit = 1u << (node & 31);
+    if (atomicOr(c.seen + (node >> 5), bit) & bit)
+        atomicOr(c.shared + (node >> 5), bit);
 }
 
 /* Every component's residual within CG_TOLERANCE of its right-hand side (the host's test and the device's). */
@@ -340,6 +362,8 @@ __global__ void __launch_bounds__(BLOCK) cgSpread(const double* __restrict__ x, 
         return;
     size_t i = t/c.slots;
     uint32_t node = c.slotNode[t];
+    if (!spreadsAt(c, node))
+        return;
     const double scale = sharedScale;
     for (int k = 0; k < 3; k++) {
         long long v = llrint(w*x[3*i + k]*scale);
@@ -364,9 +388,17 @@ __global__ void __launch_bounds__(BLOCK) cgGather(const double* __restrict__ x, 
         double winv = c.slotWInvRho[sl];
         if (winv != 0.0) {
             uint32_t node = c.slotNode[sl];
-            double inv = c.scalars[INV_SCALE];
-            for (int k = 0; k < 3; k++)
-                s[k] = (double) (long long) c.spread[3*(size_t) node + k]*inv*winv;
+            if (spreadsAt(c, node)) {
+                double inv = c.scalars[INV_SCALE];
+                for (int k = 0; k < 3; k++)
+                    s[k] = (double) (long long) c.spread[3*(size_t) node + k]*inv*winv;
+            }
+            else {
+                // Alone at its node: the node's sum is this slot's own w x.
+                double ww = c.slotW[sl]*winv;
+                for (int k = 0; k < 3; k++)
+                    s[k] = ww*x[3*(size_t) i + k];
+            }
         }
     }
 #pragma unroll
@@ -385,6 +417,8 @@ __global__ void __launch_bounds__(BLOCK) cgClear(int n, CgArgs c, const double* 
     if (t >= (size_t) n*c.slots || c.slotW[t] == 0.0)
         return;
     uint32_t node = c.slotNode[t];
+    if (!spreadsAt(c, node))
+        return;
     for (int k = 0; k < 3; k++)
         c.spread[3*(size_t) node + k] = 0ull;
 }
@@ -450,8 +484,9 @@ __global__ void __launch_bounds__(BLOCK) cgDot(int n, CgArgs c, const double* do
     for (size_t t = (size_t) blockIdx.x*BLOCK + threadIdx.x; t < (size_t) n*c.slots; t += (size_t) gridDim.x*BLOCK)
         if (c.slotW[t] != 0.0) {
             uint32_t node = c.slotNode[t];
-            for (int k = 0; k < 3; k++)
-                c.spread[3*(size_t) node + k] = 0ull;
+            if (spreadsAt(c, node))
+                for (int k = 0; k < 3; k++)
+                    c.spread[3*(size_t) node + k] = 0ull;
         }
     double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     for (int i = blockIdx.x*BLOCK + threadIdx.x; i < n; i += gridDim.x*BLOCK)
@@ -567,6 +602,16 @@ __global__ void __launch_bounds__(BLOCK) cgForces(int n, ParticleArgs a, CgArgs 
                 flagged = addReaction(node, c.slotSolid[sl] != 0, r, a, reactionFixed, wallAcc);
             }
         }
+        // The node bits of the split, cleared for the next solve (every word that cgMarkShared set holds a bit of one
+        // of these slots; nothing reads them after the solve).
+        if (c.split && lane < c.slots) {
+            size_t sl = (size_t) i*c.slots + lane;
+            if (c.slotWInvRho[sl] != 0.0) {
+                uint32_t at = c.slotNode[sl] >> 5;
+                c.seen[at] = 0u;
+                c.shared[at] = 0u;
+            }
+        }
     }
     // The slots of a particle are its lanes, x fastest, as in the explicit coupling: one OR per run of a row's lanes.
     if (a.isStep)
@@ -639,12 +684,26 @@ CentredState* centredCreate(int numParticles, int stencil, uint32_t numNodes) {
             c.slotWInvRho = alloc<double>((size_t) n*st->slots);
             c.slotSolid = alloc<uint8_t>((size_t) n*st->slots);
             c.spread = alloc<unsigned long long>(3*(size_t) numNodes);
+            // The split for sparse particles (stencil slots under a quarter of the nodes: few share a node);
+            // LBM_AMD_CG_SPLIT=0 or 1 forces the choice.
+            bool sparse = (uint64_t) numParticles*(uint64_t) st->slots*4 <= (uint64_t) numNodes;
+            const char* splitEnv = getenv("LBM_AMD_CG_SPLIT");
+            c.split = (splitEnv ? splitEnv[0] == '1' : sparse) ? 1 : 0;
+            c.seen = nullptr;
+            c.shared = nullptr;
+            if (c.split) {
+                size_t words = ((size_t) numNodes + 31)/32;
+                c.seen = alloc<uint32_t>(words);
+                c.shared = alloc<uint32_t>(words);
+                check(hipMemset(c.seen, 0, words*sizeof(uint32_t)), "memset");
+                check(hipMemset(c.shared, 0, words*sizeof(uint32_t)), "memset");
+            }
             c.partials = alloc<double>(8*RED_BLOCKS);
             c.scalars = alloc<double>(NUM_SCALARS);
             st->partialsB = alloc<double>(8*RED_BLOCKS);
             check(hipMemset(c.spread, 0, 3*(size_t) numNodes*sizeof(unsigned long long)), "memset");
             st->buffers = {c.rhs, c.diag, c.F, c.r, c.z, c.d, c.Ad, c.slotNode, c.slotW, c.slotWInvRho, c.slotSolid,
-                           c.spread, c.partials, c.scalars, st->partialsB};
+                           c.spread, c.partials, c.scalars, st->partialsB, c.seen, c.shared};
             check(hipHostMalloc((void**) &st->hostScalars, NUM_SCALARS*sizeof(double), hipHostMallocDefault),
                     "pinned scalars");
         }
@@ -730,6 +789,8 @@ void centredCoupleT(CentredState* st, const S* src, const uint32_t* mask, const 
                 g[0], g[1], g[2], c);
         break;
     }
+    if (c.split)
+        cgMarkShared<<<blocksOf((size_t) n*c.slots, BLOCK), BLOCK, 0, stream>>>(n, c);
     cgMaxAbs<<<RED_BLOCKS, BLOCK, 0, stream>>>(c.F, n, c.partials, nullptr);
     multiply<S>(st, c.F, c.Ad, mass, aa, nullptr, true, stream);
     cgInit<<<RED_BLOCKS, BLOCK, 0, stream>>>(n, c, st->partialsB);

@@ -212,11 +212,16 @@ def test_checkpoint_refused_by_the_other_drag(stencil):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("stencil", [LBMForce.NearestNode, LBMForce.Keys])
-def test_centred_parity_with_fluctuations(stencil):
+@pytest.mark.parametrize("stencil,split", [(LBMForce.NearestNode, None), (LBMForce.Keys, None), (LBMForce.Keys, "1"),
+                                           (LBMForce.Trilinear, "1")],
+                         ids=["nearest", "keys", "keys-split", "trilinear-split"])
+def test_centred_parity_with_fluctuations(stencil, split, monkeypatch):
     """The GPU against the oracle, 200 steps, dense particles sharing nodes and stencils, fluctuating fluid and particle
     noise on the same Philox streams: fields, forces and positions to 1e-5 (the FP32 criterion; measured 7e-9, 4.4e-7,
-    1.2e-7 and 3.2e-8)."""
+    1.2e-7 and 3.2e-8).  These particles are dense, so the conjugate gradients spread every slot by default; with the
+    split forced (LBM_AMD_CG_SPLIT=1) the shared nodes are found anew at every step."""
+    if split is not None:
+        monkeypatch.setenv("LBM_AMD_CG_SPLIT", split)
     rng = np.random.default_rng(31)
     n = 10
     pos = rng.uniform(0, n*DX, size=(150, 3))
@@ -259,6 +264,22 @@ SOLVE_BACKENDS = [pytest.param("oracle", id="oracle"),
 @pytest.mark.parametrize("stencil", [LBMForce.Trilinear, LBMForce.ThreePoint, LBMForce.Keys])
 @pytest.mark.parametrize("backend", SOLVE_BACKENDS)
 def test_centred_stencil_solve(backend, stencil, open_z):
+    stencil_solve(backend, stencil, open_z)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("open_z", [False, True], ids=["periodic", "open-z"])
+@pytest.mark.parametrize("stencil", [LBMForce.Trilinear, LBMForce.ThreePoint, LBMForce.Keys])
+@pytest.mark.parametrize("split", ["0", "1"], ids=["all-spread", "split"])
+def test_centred_stencil_solve_with_and_without_the_split(split, stencil, open_z, monkeypatch):
+    """The same solve with the products spread over every slot (LBM_AMD_CG_SPLIT=0) and over the shared nodes only,
+    the slots alone at their nodes taken directly (=1); the default picks by density, so this covers both on every
+    stencil, in FP64 storage."""
+    monkeypatch.setenv("LBM_AMD_CG_SPLIT", split)
+    stencil_solve("fp64", stencil, open_z)
+
+
+def stencil_solve(backend, stencil, open_z):
     """TestLBMStencils.h:341-491: the coupling forces of the first step against the solution, by Gaussian elimination,
     of (1 + a) F_k + a m_k sum_l K_kl F_l = -gamma m_k (v~_k - U_k) (lattice units, m_k = m/m_c, a = gamma/2,
     K_kl = sum_j xi_jk xi_jl/rho_j and U_k = sum_j xi_jk u_j over the fluid nodes), built here from the fluid state, the
