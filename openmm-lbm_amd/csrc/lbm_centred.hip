@@ -273,16 +273,45 @@ __global__ void __launch_bounds__(BLOCK) cgSetup(const S* __restrict__ src, cons
     }
 }
 
-/* The nodes that several fluid slots share (with the split): each slot sets its node's bit in seen, and in shared if
-   the bit was already set.  One thread per particle slot. */
+/* The nodes that several fluid slots share (with the split), one group of LANES lanes per particle, one lane per slot:
+   every node's bit goes into seen, and into shared where it was set already.  The x-consecutive slots of a stencil row
+   mostly share a word of the bit arrays, so the first lane of each run of lanes on one word sets the whole run's bits
+   with one atomic OR (as markTouched does) and, from the bits that OR returns and any bit two lanes of the run hold,
+   the run's shared bits with another. */
+template <int LANES>
 __global__ void __launch_bounds__(BLOCK) cgMarkShared(int n, CgArgs c) {
-    size_t t = (size_t) blockIdx.x*BLOCK + threadIdx.x;
-    if (t >= (size_t) n*c.slots || c.slotWInvRho[t] == 0.0)
-        return;
-    uint32_t node = c.slotNode[t];
-    uint32_t bit = 1u << (node & 31);
-    if (atomicOr(c.seen + (node >> 5), bit) & bit)
-        atomicOr(c.shared + (node >> 5), bit);
+    constexpr int GROUPS = BLOCK/LANES;
+    constexpr int W = LANES == 64 ? 4 : LANES == 32 ? 3 : 2;    // slots per stencil row
+    const int lane = threadIdx.x % LANES;
+    const int i = blockIdx.x*GROUPS + threadIdx.x/LANES;
+    uint32_t node = 0;
+    bool live = false;
+    if (i < n && lane < c.slots) {
+        size_t sl = (size_t) i*c.slots + lane;
+        live = c.slotWInvRho[sl] != 0.0;
+        node = c.slotNode[sl];
+    }
+    const uint32_t word = node >> 5;
+    const uint32_t bit = live ? 1u << (node & 31) : 0u;
+    uint32_t bits = bit, twice = 0u;
+    const int x = lane % W;
+#pragma unroll
+    for (int d = 1; d < W; d++) {
+        uint32_t w2 = __shfl_down(word, d, 64);
+        uint32_t b2 = __shfl_down(bit, d, 64);
+        if (x + d < W && w2 == word) {
+            twice |= bits & b2;
+            bits |= b2;
+        }
+    }
+    uint32_t before = __shfl_up(word, 1, 64);
+    bool first = x == 0 || before != word;
+    if (first && bits != 0u) {
+        uint32_t old = atomicOr(c.seen + word, bits);
+        uint32_t both = (old & bits) | twice;
+        if (both != 0u)
+            atomicOr(c.shared + word, both);
+    }
 }
 
 /* Every component's residual within CG_TOLERANCE of its right-hand side (the host's test and the device's). */
@@ -790,7 +819,11 @@ void centredCoupleT(CentredState* st, const S* src, const uint32_t* mask, const 
         break;
     }
     if (c.split)
-        cgMarkShared<<<blocksOf((size_t) n*c.slots, BLOCK), BLOCK, 0, stream>>>(n, c);
+        switch (st->lanes) {
+        case 8: cgMarkShared<8><<<groups, BLOCK, 0, stream>>>(n, c); break;
+        case 32: cgMarkShared<32><<<groups, BLOCK, 0, stream>>>(n, c); break;
+        default: cgMarkShared<64><<<groups, BLOCK, 0, stream>>>(n, c); break;
+        }
     cgMaxAbs<<<RED_BLOCKS, BLOCK, 0, stream>>>(c.F, n, c.partials, nullptr);
     multiply<S>(st, c.F, c.Ad, mass, aa, nullptr, true, stream);
     cgInit<<<RED_BLOCKS, BLOCK, 0, stream>>>(n, c, st->partialsB);
