@@ -184,7 +184,9 @@ struct CgArgs {
     double* slotW;
     double* slotWInvRho;     // w/rho, 0 at solid nodes
     uint8_t* slotSolid;
-    unsigned long long* spread;   // [node][3] fixed point
+    uint8_t* slotSpreads;    // 1 where the slot takes part in the spread of a product (set by the solve's first one)
+    unsigned long long* spread;        // [node][3] fixed point: the node sums of the current product
+    unsigned long long* spreadOther;   // the other of the two arrays: the previous product's, cleared by the gather
     double* partials;        // [RED_BLOCKS][8]
     double* scalars;
     int slots;               // w^3
@@ -364,9 +366,9 @@ __device__ __forceinline__ void combinePartials(const double* __restrict__ parti
 /* spread[node] += w x (fixed point), one thread per particle slot.  The fixed-point scale, 2^(52 - e) with
    2^e >= max |x| (1 if x is zero), is formed by the first wavefront of every work-group from the largest |x| of the
    blocks in c.partials (one fixed combination, so every work-group has the same value); work-group 0 keeps it and its
-   inverse in the scalars for the gather. */
+   inverse in the scalars for the gather.  The solve's first product (first = 1) records which slots spread. */
 __global__ void __launch_bounds__(BLOCK) cgSpread(const double* __restrict__ x, int n, CgArgs c,
-        const double* done) {
+        const double* done, int first) {
     if (done != nullptr && *done != 0.0)
         return;
     __shared__ double sharedScale;
@@ -387,12 +389,17 @@ __global__ void __launch_bounds__(BLOCK) cgSpread(const double* __restrict__ x, 
     if (t >= (size_t) n*c.slots)
         return;
     double w = c.slotW[t];
-    if (w == 0.0)
+    uint32_t node = c.slotNode[t];
+    bool spreads;
+    if (first) {
+        spreads = w != 0.0 && spreadsAt(c, node);
+        c.slotSpreads[t] = spreads ? 1 : 0;
+    }
+    else
+        spreads = c.slotSpreads[t] != 0;
+    if (!spreads)
         return;
     size_t i = t/c.slots;
-    uint32_t node = c.slotNode[t];
-    if (!spreadsAt(c, node))
-        return;
     const double scale = sharedScale;
     for (int k = 0; k < 3; k++) {
         long long v = llrint(w*x[3*i + k]*scale);
@@ -415,9 +422,14 @@ __global__ void __launch_bounds__(BLOCK) cgGather(const double* __restrict__ x, 
     if (active && lane < c.slots) {
         size_t sl = (size_t) i*c.slots + lane;
         double winv = c.slotWInvRho[sl];
+        const bool spreads = c.slotSpreads[sl] != 0;
+        const uint32_t node = c.slotNode[sl];
+        // The previous product's node sums, which every reader has read, cleared for the next product.
+        if (spreads && c.spreadOther != nullptr)
+            for (int k = 0; k < 3; k++)
+                c.spreadOther[3*(size_t) node + k] = 0ull;
         if (winv != 0.0) {
-            uint32_t node = c.slotNode[sl];
-            if (spreadsAt(c, node)) {
+            if (spreads) {
                 double inv = c.scalars[INV_SCALE];
                 for (int k = 0; k < 3; k++)
                     s[k] = (double) (long long) c.spread[3*(size_t) node + k]*inv*winv;
@@ -437,19 +449,6 @@ __global__ void __launch_bounds__(BLOCK) cgGather(const double* __restrict__ x, 
     if (active && lane == 0)
         for (int k = 0; k < 3; k++)
             Ax[3*(size_t) i + k] = x[3*(size_t) i + k]*((1.0 + aa)/mass[i]) + s[k]*aa;
-}
-
-__global__ void __launch_bounds__(BLOCK) cgClear(int n, CgArgs c, const double* done) {
-    if (done != nullptr && *done != 0.0)
-        return;
-    size_t t = (size_t) blockIdx.x*BLOCK + threadIdx.x;
-    if (t >= (size_t) n*c.slots || c.slotW[t] == 0.0)
-        return;
-    uint32_t node = c.slotNode[t];
-    if (!spreadsAt(c, node))
-        return;
-    for (int k = 0; k < 3; k++)
-        c.spread[3*(size_t) node + k] = 0ull;
 }
 
 /* Block sums of up to 8 per-particle quantities into partials[block][0..7], fixed order. */
@@ -505,18 +504,10 @@ __global__ void __launch_bounds__(64) cgFinishInit(const double* __restrict__ pa
     scalars[ITERS] = 0.0;
 }
 
-/* d . Ad into c.partials (alpha = r.z/(d.Ad) follows in cgUpdate); and the node sums of the product, read by the
-   gather, cleared for the next spread. */
+/* d . Ad into c.partials (alpha = r.z/(d.Ad) follows in cgUpdate). */
 __global__ void __launch_bounds__(BLOCK) cgDot(int n, CgArgs c, const double* done) {
     if (done != nullptr && *done != 0.0)
         return;
-    for (size_t t = (size_t) blockIdx.x*BLOCK + threadIdx.x; t < (size_t) n*c.slots; t += (size_t) gridDim.x*BLOCK)
-        if (c.slotW[t] != 0.0) {
-            uint32_t node = c.slotNode[t];
-            if (spreadsAt(c, node))
-                for (int k = 0; k < 3; k++)
-                    c.spread[3*(size_t) node + k] = 0ull;
-        }
     double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     for (int i = blockIdx.x*BLOCK + threadIdx.x; i < n; i += gridDim.x*BLOCK)
         for (int k = 0; k < 3; k++)
@@ -631,14 +622,19 @@ __global__ void __launch_bounds__(BLOCK) cgForces(int n, ParticleArgs a, CgArgs 
                 flagged = addReaction(node, c.slotSolid[sl] != 0, r, a, reactionFixed, wallAcc);
             }
         }
-        // The node bits of the split, cleared for the next solve (every word that cgMarkShared set holds a bit of one
-        // of these slots; nothing reads them after the solve).
-        if (c.split && lane < c.slots) {
+        // Both arrays of node sums and the node bits of the split, cleared for the next solve (every word that a
+        // spread or cgMarkShared set belongs to one of these slots; nothing reads them after the solve).
+        if (lane < c.slots) {
             size_t sl = (size_t) i*c.slots + lane;
-            if (c.slotWInvRho[sl] != 0.0) {
-                uint32_t at = c.slotNode[sl] >> 5;
-                c.seen[at] = 0u;
-                c.shared[at] = 0u;
+            uint32_t at = c.slotNode[sl];
+            if (c.slotSpreads[sl] != 0)
+                for (int k = 0; k < 3; k++) {
+                    c.spread[3*(size_t) at + k] = 0ull;
+                    c.spreadOther[3*(size_t) at + k] = 0ull;
+                }
+            if (c.split && c.slotWInvRho[sl] != 0.0) {
+                c.seen[at >> 5] = 0u;
+                c.shared[at >> 5] = 0u;
             }
         }
     }
@@ -667,6 +663,7 @@ struct CentredState {
     int sortBits = 1;
     // Stencils: conjugate gradients.
     CgArgs cg{};
+    unsigned long long* sums[2] = {nullptr, nullptr};   // the two arrays of node sums, used in turn by the products
     double* partialsB = nullptr;
     double* hostScalars = nullptr;        // pinned: the convergence check reads the scalars
     int lastIterations = 1;               // iterations of the previous solve: enqueued before the first check
@@ -712,7 +709,11 @@ CentredState* centredCreate(int numParticles, int stencil, uint32_t numNodes) {
             c.slotW = alloc<double>((size_t) n*st->slots);
             c.slotWInvRho = alloc<double>((size_t) n*st->slots);
             c.slotSolid = alloc<uint8_t>((size_t) n*st->slots);
-            c.spread = alloc<unsigned long long>(3*(size_t) numNodes);
+            c.slotSpreads = alloc<uint8_t>((size_t) n*st->slots);
+            st->sums[0] = alloc<unsigned long long>(3*(size_t) numNodes);
+            st->sums[1] = alloc<unsigned long long>(3*(size_t) numNodes);
+            c.spread = st->sums[0];
+            c.spreadOther = st->sums[1];
             // The split for sparse particles (stencil slots under a quarter of the nodes: few share a node);
             // LBM_AMD_CG_SPLIT=0 or 1 forces the choice.
             bool sparse = (uint64_t) numParticles*(uint64_t) st->slots*4 <= (uint64_t) numNodes;
@@ -730,9 +731,11 @@ CentredState* centredCreate(int numParticles, int stencil, uint32_t numNodes) {
             c.partials = alloc<double>(8*RED_BLOCKS);
             c.scalars = alloc<double>(NUM_SCALARS);
             st->partialsB = alloc<double>(8*RED_BLOCKS);
-            check(hipMemset(c.spread, 0, 3*(size_t) numNodes*sizeof(unsigned long long)), "memset");
+            check(hipMemset(st->sums[0], 0, 3*(size_t) numNodes*sizeof(unsigned long long)), "memset");
+            check(hipMemset(st->sums[1], 0, 3*(size_t) numNodes*sizeof(unsigned long long)), "memset");
             st->buffers = {c.rhs, c.diag, c.F, c.r, c.z, c.d, c.Ad, c.slotNode, c.slotW, c.slotWInvRho, c.slotSolid,
-                           c.spread, c.partials, c.scalars, st->partialsB, c.seen, c.shared};
+                           c.slotSpreads, st->sums[0], st->sums[1], c.partials, c.scalars, st->partialsB, c.seen,
+                           c.shared};
             check(hipHostMalloc((void**) &st->hostScalars, NUM_SCALARS*sizeof(double), hipHostMallocDefault),
                     "pinned scalars");
         }
@@ -757,23 +760,23 @@ void centredDestroy(CentredState* st) {
 
 namespace {
 
-/* Ax = A x.  The largest |x| of each block must be in c.partials (cgMaxAbs, or cgDirection for x = d).  done: the
-   solve's convergence flag inside the iteration loop (nullptr outside it), where cgDot clears the node sums (clear =
-   false); outside it cgClear does. */
+/* Ax = A x, with the node sums in array `sums` (the gather clears the other, the previous product's).  The largest |x|
+   of each block must be in c.partials (cgMaxAbs, or cgDirection for x = d).  done: the solve's convergence flag inside
+   the iteration loop (nullptr outside it); first: the solve's first product. */
 template <typename S>
 void multiply(CentredState* st, const double* x, double* Ax, const double* mass, double aa, const double* done,
-        bool clear, hipStream_t stream) {
-    CgArgs& c = st->cg;
+        int sums, bool first, hipStream_t stream) {
+    CgArgs c = st->cg;
+    c.spread = st->sums[sums];
+    c.spreadOther = st->sums[1 - sums];
     int n = st->n;
     size_t slotItems = (size_t) n*c.slots;
-    cgSpread<<<blocksOf(slotItems, BLOCK), BLOCK, 0, stream>>>(x, n, c, done);
+    cgSpread<<<blocksOf(slotItems, BLOCK), BLOCK, 0, stream>>>(x, n, c, done, first ? 1 : 0);
     switch (st->lanes) {
     case 8: cgGather<8><<<blocksOf(n, BLOCK/8), BLOCK, 0, stream>>>(x, Ax, mass, n, aa, c, done); break;
     case 32: cgGather<32><<<blocksOf(n, BLOCK/32), BLOCK, 0, stream>>>(x, Ax, mass, n, aa, c, done); break;
     default: cgGather<64><<<blocksOf(n, BLOCK/64), BLOCK, 0, stream>>>(x, Ax, mass, n, aa, c, done); break;
     }
-    if (clear)
-        cgClear<<<blocksOf(slotItems, BLOCK), BLOCK, 0, stream>>>(n, c, done);
 }
 
 bool converged(CentredState* st, hipStream_t stream) {
@@ -825,7 +828,7 @@ void centredCoupleT(CentredState* st, const S* src, const uint32_t* mask, const 
         default: cgMarkShared<64><<<groups, BLOCK, 0, stream>>>(n, c); break;
         }
     cgMaxAbs<<<RED_BLOCKS, BLOCK, 0, stream>>>(c.F, n, c.partials, nullptr);
-    multiply<S>(st, c.F, c.Ad, mass, aa, nullptr, true, stream);
+    multiply<S>(st, c.F, c.Ad, mass, aa, nullptr, 1, true, stream);
     cgInit<<<RED_BLOCKS, BLOCK, 0, stream>>>(n, c, st->partialsB);
     cgFinishInit<<<1, 64, 0, stream>>>(c.partials, st->partialsB, c.scalars);
     check(hipGetLastError(), "launch");
@@ -840,7 +843,7 @@ void centredCoupleT(CentredState* st, const S* src, const uint32_t* mask, const 
                 cgMaxAbs<<<RED_BLOCKS, BLOCK, 0, stream>>>(c.d, n, c.partials, done);
             else
                 cgDirection<<<RED_BLOCKS, BLOCK, 0, stream>>>(n, c, done);
-            multiply<S>(st, c.d, c.Ad, mass, aa, done, false, stream);
+            multiply<S>(st, c.d, c.Ad, mass, aa, done, iteration & 1, false, stream);
             cgDot<<<RED_BLOCKS, BLOCK, 0, stream>>>(n, c, done);
             cgUpdate<<<RED_BLOCKS, BLOCK, 0, stream>>>(n, c, st->partialsB, done);
             cgFinishBeta<<<1, 64, 0, stream>>>(st->partialsB, c.scalars, done);
