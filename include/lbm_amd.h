@@ -157,7 +157,9 @@ int lbm_compute_lattice_parameters(const lbm_force_settings* settings, const dou
  * lattice-wide moment field) and LBM_AMD_STENCIL_SETUP=0|1 (each particle's stencil formed once), by default tiles,
  * field and setup for dense particles (stencil slots above a quarter of the nodes) and none of them otherwise; for the
  * centred drag with a stencil, LBM_AMD_CG_PLANNED=0 makes the host read the conjugate gradients' residual after every
- * iteration instead of after the previous solve's count of them (default 1).
+ * iteration instead of after the previous solve's count of them (default 1), and LBM_AMD_CG_SPLIT=0|1 spreads every
+ * slot in their products or only those at nodes several slots share (default: the latter for sparse particles; the
+ * results agree to the solver's tolerance, not bit for bit).
  */
 int lbm_create(const lbm_params* params, const int* solid_nodes, int num_solid, const int* particles,
         const double* particle_masses, int num_particles, int num_system_particles, lbm_context** out);
