@@ -190,8 +190,9 @@ struct CgArgs {
     double* partials;        // [RED_BLOCKS][8]
     double* scalars;
     int slots;               // w^3
-    // Sparse particles: a product spreads only the slots at nodes that several slots share (sharedNode); a slot alone
-    // at its node contributes (w/rho) w x of its own particle directly.  seen/shared: node bits, cleared by cgForces.
+    // Sparse particles (split = 1): a product spreads only the slots at nodes that several slots share (spreadsAt); a
+    // slot alone at its node contributes (w/rho) w x of its own particle directly.  seen/shared: node bits set by
+    // cgMarkShared and cleared by cgForces.
     int split;
     uint32_t* seen;
     uint32_t* shared;
