@@ -16,7 +16,9 @@
  *    conjugate gradients preconditioned with the diagonal, from the diagonal solution, to a residual of 1e-13 of the
  *    right-hand side.  A product spreads the vector onto the nodes and interpolates it back; the spread accumulates in
  *    64-bit fixed point with a scale set from the largest entry of the vector (2^52 / that entry), so the sums are
- *    exact to 2^-52 of it and independent of the order of the atomics.  Dot products reduce over a fixed grid in a
+ *    exact to 2^-52 of it and independent of the order of the atomics.  With sparse particles only the slots at nodes
+ *    that several slots share are spread; a slot alone at its node takes its own particle's w x directly.  Two arrays
+ *    of node sums serve the products in turn, each gather clearing the previous product's.  Dot products reduce over a fixed grid in a
  *    fixed order.  The device tests the residual after every iteration and the kernels of later iterations then skip
  *    their work; the host enqueues as many iterations as the previous solve took before it reads the residual, then
  *    one at a time, so the iterations and results are those of a check after every iteration without a round trip
