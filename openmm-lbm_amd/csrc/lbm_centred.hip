@@ -95,6 +95,23 @@ __device__ __forceinline__ bool addReaction(uint32_t node, bool solid, const dou
     return any;
 }
 
+/* addReaction for a fluid node that no other slot of the step reaches (the CG split's unshared nodes): its
+   accumulators are zero (the lattice pass consumed and cleared them), so a store puts the same bits as the atomic add. */
+__device__ __forceinline__ bool storeReaction(uint32_t node, const double r[3], const ParticleArgs& a,
+        unsigned long long* reactionFixed) {
+    bool any = false;
+    for (int k = 0; k < 3; k++) {
+        if (!reactionInRange(r[k], a.overflow))
+            continue;
+        long long fixed = llrint(r[k]*FIXED_SCALE);
+        if (fixed != 0) {
+            reactionFixed[reactionIndex(node, k, a.g.n, a.reactionPlanes)] = (unsigned long long) fixed;
+            any = true;
+        }
+    }
+    return any;
+}
+
 __device__ __forceinline__ void blockPartials(const double acc[3], double* partials) {
     if (partials != nullptr)
         storeWallPartial<BLOCK>(acc, partials);
@@ -622,7 +639,12 @@ __global__ void __launch_bounds__(BLOCK) cgForces(int n, ParticleArgs a, CgArgs 
             node = c.slotNode[sl];
             if (w != 0.0) {
                 double r[3] = {-w*F[0], -w*F[1], -w*F[2]};
-                flagged = addReaction(node, c.slotSolid[sl] != 0, r, a, reactionFixed, wallAcc);
+                bool solid = c.slotSolid[sl] != 0;
+                // With the split, a fluid slot that spreads nothing is alone at its node: a store, not an atomic.
+                if (c.split && !solid && c.slotSpreads[sl] == 0 && c.slotWInvRho[sl] != 0.0)
+                    flagged = storeReaction(node, r, a, reactionFixed);
+                else
+                    flagged = addReaction(node, solid, r, a, reactionFixed, wallAcc);
             }
         }
         // Both arrays of node sums and the node bits of the split, cleared for the next solve (every word that a
