@@ -295,11 +295,8 @@ __global__ void __launch_bounds__(BLOCK) cgSetup(const S* __restrict__ src, cons
     }
 }
 
-/* The nodes that several fluid slots share (with the split), one group of LANES lanes per particle, one lane per slot:
-   every node's bit goes into seen, and into shared where it was set already.  The x-consecutive slots of a stencil row
-   mostly share a word of the bit arrays, so the first lane of each run of lanes on one word sets the whole run's bits
-   with one atomic OR (as markTouched does) and, from the bits that OR returns and any bit two lanes of the run hold,
-   the run's shared bits with another. */
+/* The nodes that several fluid slots share (with the split), one group of LANES lanes per particle, one lane per slot
+   (markSharedRun: one atomic OR per run of a stencil row's lanes on one word). */
 template <int LANES>
 __global__ void __launch_bounds__(BLOCK) cgMarkShared(int n, CgArgs c) {
     constexpr int GROUPS = BLOCK/LANES;
@@ -313,27 +310,7 @@ __global__ void __launch_bounds__(BLOCK) cgMarkShared(int n, CgArgs c) {
         live = c.slotWInvRho[sl] != 0.0;
         node = c.slotNode[sl];
     }
-    const uint32_t word = node >> 5;
-    const uint32_t bit = live ? 1u << (node & 31) : 0u;
-    uint32_t bits = bit, twice = 0u;
-    const int x = lane % W;
-#pragma unroll
-    for (int d = 1; d < W; d++) {
-        uint32_t w2 = __shfl_down(word, d, 64);
-        uint32_t b2 = __shfl_down(bit, d, 64);
-        if (x + d < W && w2 == word) {
-            twice |= bits & b2;
-            bits |= b2;
-        }
-    }
-    uint32_t before = __shfl_up(word, 1, 64);
-    bool first = x == 0 || before != word;
-    if (first && bits != 0u) {
-        uint32_t old = atomicOr(c.seen + word, bits);
-        uint32_t both = (old & bits) | twice;
-        if (both != 0u)
-            atomicOr(c.shared + word, both);
-    }
+    markSharedRun<W>(c.seen, c.shared, node, live, lane);
 }
 
 /* Every component's residual within CG_TOLERANCE of its right-hand side (the host's test and the device's). */

@@ -356,6 +356,43 @@ __device__ __forceinline__ bool spreadReaction(const StencilSlot& sl, const doub
 }
 
 /*
+ * Marks the nodes that several slots reach: each live slot's node bit goes into seen, and into shared where it was set
+ * already.  The W slots of a stencil row are consecutive lanes on x-consecutive nodes, which mostly share a word: the
+ * first lane of each run of lanes on one word sets the run's bits with one atomic OR and, from the bits it returns and
+ * any bit two lanes of the run hold, the run's shared bits with another.  A word the run visits twice (a lattice
+ * narrower than the stencil) may flag a node shared that is not, which only costs an atomic.  Every lane of the
+ * wavefront calls it.
+ */
+template <int W>
+__device__ __forceinline__ void markSharedRun(uint32_t* __restrict__ seen, uint32_t* __restrict__ shared, uint32_t node,
+        bool live, int lane) {
+    const uint32_t word = node >> 5;
+    const uint32_t bit = live ? 1u << (node & 31) : 0u;
+    uint32_t bits = bit, twice = 0u;
+    bool first = true;
+    if constexpr (W > 1) {
+        const int x = lane % W;
+#pragma unroll
+        for (int d = 1; d < W; d++) {
+            uint32_t w2 = __shfl_down(word, d, 64);
+            uint32_t b2 = __shfl_down(bit, d, 64);
+            if (x + d < W && w2 == word) {
+                twice |= bits & b2;
+                bits |= b2;
+            }
+        }
+        uint32_t before = __shfl_up(word, 1, 64);
+        first = x == 0 || before != word;
+    }
+    if (first && bits != 0u) {
+        uint32_t old = atomicOr(seen + word, bits);
+        uint32_t both = (old & bits) | twice;
+        if (both != 0u)
+            atomicOr(shared + word, both);
+    }
+}
+
+/*
  * Flags the nodes of the slots with `flagged` in the touched mask (bit node & 31 of word node >> 5).  The W slots of
  * a stencil row are consecutive lanes on x-consecutive nodes, which mostly share a mask word: the first lane of each
  * run of lanes on one word ORs in the bits of the row's lanes on that word, one atomic instead of up to W.  A bit may
