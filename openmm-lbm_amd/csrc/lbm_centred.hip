@@ -24,8 +24,9 @@
  *    one at a time, so the iterations and results are those of a check after every iteration without a round trip
  *    to the host for each.
  *
- * The reaction goes into the same fixed-point accumulators as the explicit drag (consumed by the fused pass), and into
- * the wall partial sums at solid nodes.
+ * The reaction goes into the same fixed-point accumulators as the explicit drag (consumed by the fused pass; with the
+ * split, stored where no other slot reaches the node, since those accumulators are zero, and added atomically where
+ * several do), and into the wall partial sums at solid nodes.
  */
 #include "lbm_coupling.h"
 #include <hipcub/hipcub.hpp>
