@@ -854,9 +854,20 @@ void centredCoupleT(CentredState* st, const S* src, const uint32_t* mask, const 
         }
         if (converged(st, stream))
             break;
-        if (iteration >= CG_MAX_ITERATIONS)
+        if (iteration >= CG_MAX_ITERATIONS) {
+            // cgForces, which clears the node sums and bits after a solve, does not run: clear them here, so that a
+            // caller who goes on after the error starts the next solve from zero.
+            const size_t sumBytes = 3*(size_t) st->numNodes*sizeof(unsigned long long);
+            check(hipMemsetAsync(st->sums[0], 0, sumBytes, stream), "memset");
+            check(hipMemsetAsync(st->sums[1], 0, sumBytes, stream), "memset");
+            if (c.split) {
+                const size_t bitBytes = (((size_t) st->numNodes + 31)/32)*sizeof(uint32_t);
+                check(hipMemsetAsync(c.seen, 0, bitBytes, stream), "memset");
+                check(hipMemsetAsync(c.shared, 0, bitBytes, stream), "memset");
+            }
             throw std::runtime_error("LBMForce: the centred drag with the interpolation stencil did not converge in 1000 "
                     "iterations of the conjugate gradients");
+        }
     }
     st->lastIterations = (int) st->hostScalars[ITERS];
     switch (st->lanes) {
