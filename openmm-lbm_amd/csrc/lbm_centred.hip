@@ -334,7 +334,7 @@ __global__ void __launch_bounds__(BLOCK) cgMaxAbs(const double* __restrict__ x, 
     lds[threadIdx.x] = m;
     __syncthreads();
     for (int s = BLOCK/2; s > 0; s >>= 1) {
-        if (threadIdx.x < s)
+        if ((int) threadIdx.x < s)
             lds[threadIdx.x] = fmax(lds[threadIdx.x], lds[threadIdx.x + s]);
         __syncthreads();
     }
@@ -456,7 +456,7 @@ __device__ __forceinline__ void reduceBlock(double v[8], int count, double* part
         lds[q][threadIdx.x] = v[q];
     __syncthreads();
     for (int s = BLOCK/2; s > 0; s >>= 1) {
-        if (threadIdx.x < s)
+        if ((int) threadIdx.x < s)
             for (int q = 0; q < count; q++)
                 lds[q][threadIdx.x] += lds[q][threadIdx.x + s];
         __syncthreads();
@@ -587,7 +587,7 @@ __global__ void __launch_bounds__(BLOCK) cgDirection(int n, CgArgs c, const doub
     lds[threadIdx.x] = m;
     __syncthreads();
     for (int s = BLOCK/2; s > 0; s >>= 1) {
-        if (threadIdx.x < s)
+        if ((int) threadIdx.x < s)
             lds[threadIdx.x] = fmax(lds[threadIdx.x], lds[threadIdx.x + s]);
         __syncthreads();
     }
