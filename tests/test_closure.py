@@ -49,7 +49,8 @@ def beads(rng, n, count):
     return pos, rng.normal(scale=0.2, size=(count, 3))
 
 
-@pytest.mark.parametrize("storage", [LBMForce.FP64, LBMForce.FP32])
+@pytest.mark.parametrize("storage", [LBMForce.FP64, LBMForce.FP32, LBMForce.FP16, LBMForce.BF16],
+                         ids=["fp64", "fp32", "fp16", "bf16"])
 def test_features_and_targets_match_the_oracle(storage):
     n = 12
     rng = np.random.default_rng(5)
