@@ -37,12 +37,12 @@ def make_replica(grid, tau, beads, kT, seed, stencil=LBMForce.Trilinear, dx=1.0,
     return Replica(force, (nx*dx, ny*dx, nz*dx), dt, np.full(beads, 100.0), pos, vel)
 
 
-def uneven_replicas():
+def uneven_replicas(stencil=LBMForce.Trilinear):
     return [
-        make_replica((12, 12, 12), 0.62, 9, 2.5, 11, rng_seed=1),
-        make_replica((16, 12, 10), 0.95, 30, 1.5, 12345, rng_seed=2),
-        make_replica((8, 8, 8), 0.55, 0, 3.0, 777, rng_seed=3),
-        make_replica((20, 9, 14), 0.8, 3, 2.0, 31, rng_seed=4),
+        make_replica((12, 12, 12), 0.62, 9, 2.5, 11, stencil=stencil, rng_seed=1),
+        make_replica((16, 12, 10), 0.95, 30, 1.5, 12345, stencil=stencil, rng_seed=2),
+        make_replica((8, 8, 8), 0.55, 0, 3.0, 777, stencil=stencil, rng_seed=3),
+        make_replica((20, 9, 14), 0.8, 3, 2.0, 31, stencil=stencil, rng_seed=4),
     ]
 
 
@@ -67,8 +67,10 @@ def run_single(replica, steps):
 
 
 @pytest.mark.gpu
-def test_batch_matches_single_replicas_bit_for_bit():
-    replicas = uneven_replicas()
+@pytest.mark.parametrize("stencil", [LBMForce.Trilinear, LBMForce.Keys, LBMForce.ThreePoint, LBMForce.NearestNode],
+                         ids=["trilinear", "keys", "threepoint", "nearest"])
+def test_batch_matches_single_replicas_bit_for_bit(stencil):
+    replicas = uneven_replicas(stencil)
     batch = Batch(replicas)
     nodes, _ = batch.info()
     assert nodes == sum(int(np.prod(r.force.getGridSize())) for r in replicas)
