@@ -86,26 +86,27 @@ __global__ void __launch_bounds__(BLOCK) closureFeatures(const S* __restrict__ s
     int part = active ? particles[i] : 0;
     const double* x = positions + 3*(size_t) part;
     int size[3] = {(int) a.g.nx, (int) a.g.ny, (int) a.g.nz};
-    // The wrapped coordinate, the nearest node (FP64, as the coupling computes them) and the particle's frame by the
-    // group's first lane, shared with the others.
-    double sw[3] = {0, 0, 0};
-    int near[3] = {0, 0, 0};
-    int code = 0;
-    if (active && lane == 0) {
-        double o[3];
-        for (int k = 0; k < 3; k++) {
-            sw[k] = wrapCoordinate(x[k], a.dx, size[k]);
-            near[k] = nearestIndex(sw[k], size[k]);
-            o[k] = sw[k] - floor(sw[k] + 0.5);
-        }
-        code = frameCode(o);
-        frame[i] = (uint8_t) code;
+    // The wrapped coordinate and the nearest node of axis k (FP64, as the coupling computes them) by the group's lane k,
+    // shared with the others; every lane then forms the particle's frame from the three offsets.  The axis length is
+    // selected, not indexed by the lane (a lane-varying index into the kernel argument's fields would be read once for
+    // the whole wavefront).
+    double swLane = 0.0;
+    int nearLane = 0;
+    if (active && lane < 3) {
+        const int axisSize = lane == 0 ? size[0] : (lane == 1 ? size[1] : size[2]);
+        swLane = wrapCoordinate(x[lane], a.dx, axisSize);
+        nearLane = nearestIndex(swLane, axisSize);
     }
+    double sw[3], o[3];
+    int near[3];
     for (int k = 0; k < 3; k++) {
-        sw[k] = __shfl(sw[k], 0, FEATURE_LANES);
-        near[k] = __shfl(near[k], 0, FEATURE_LANES);
+        sw[k] = __shfl(swLane, k, FEATURE_LANES);
+        near[k] = __shfl(nearLane, k, FEATURE_LANES);
+        o[k] = sw[k] - floor(sw[k] + 0.5);
     }
-    code = __shfl(code, 0, FEATURE_LANES);
+    const int code = frameCode(o);
+    if (active && lane == 0)
+        frame[i] = (uint8_t) code;
     const Frame fr = frameOf((unsigned) code);
     if (active) {
         int openAxis[3] = {a.g.openX, a.g.openY, a.g.openZ};
