@@ -67,7 +67,7 @@ def cpu_model():
 
 
 def host_threads():
-    """The host cores this process may use (its affinity), at most 16 (the CPU share of a one-GPU box)."""
+    """The host cores this process may use (its affinity), at most 16 (a one-GPU host's share of the cores)."""
     try:
         count = len(os.sched_getaffinity(0))
     except AttributeError:

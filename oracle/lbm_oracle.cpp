@@ -3,7 +3,7 @@
  *
  * A plain double-precision restatement of the reference's Reference platform for the
  * one-GPU LBMForce path, used by tests/, the smoke check at the repository root and bench.py's CPU
- * baseline as the checker.  Nothing in the product library links or calls it.
+ * baseline as the reference results.  Nothing in the product library links or calls it.
  *
  * It follows, step for step, platforms/reference/src/ReferenceLBMKernels.cpp (push
  * streaming into a single population array, moments, removal, explicit coupling at the

@@ -1,6 +1,6 @@
 """CPU ORACLE OF THE LEARNED COUPLING CLOSURE -- TEST INFRASTRUCTURE ONLY.
 
-Used by tests/ as the checker of the gfx950 closure kernels; nothing in the product imports it.
+Used by tests/ as the reference results for the gfx950 closure kernels; nothing in the product imports it.
 
 The closure has no counterpart in the reference.  What it is built from does:
 

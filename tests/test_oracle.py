@@ -1,4 +1,4 @@
-"""Pins the FP64 CPU oracle (oracle/lbm_oracle.cpp) before it is trusted as the checker of the GPU path.
+"""Pins the FP64 CPU oracle (oracle/lbm_oracle.cpp) before it is trusted as the reference for the GPU path.
 
  - Philox4x32-10 against the published known-answer vectors of Random123 (kat_vectors, philox4x32_10).
  - The D3Q19 algebra against the reference's own openmmapi/include/internal/D3Q19.h, compiled into oracle/_ref.
