@@ -225,6 +225,13 @@ def batch_study(steps, warmup, sequential_steps=20):
     # Touched nodes of the trilinear stencil: at most 8 per bead.
     bpn = bytes_per_node("fp32", False, min(1.0, 8*beads/nodes))
     achieved = bpn*nodes/(fused_ms*1e-3)/1e9
+    # HBM bytes per launch of fusedBatch at the picked work-group size, measured by rocprofv3 --pmc FETCH_SIZE /
+    # WRITE_SIZE (scaled by a copy of known size) on this workload.
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "fusedBatch_config4_fp32.traffic.json")
+    if os.path.exists(tfile):
+        shape = json.load(open(tfile))["per_threads"].get(str(best[0]))
+        traffic = round(shape["traffic_bytes_per_launch"]) if shape is not None else None
 
     # The same replicas one after another through the single-lattice path (one GPU, one replica at a time).
     seq_ms = 0.0
@@ -260,9 +267,12 @@ def batch_study(steps, warmup, sequential_steps=20):
         "config": dict(description, autotune_ms_per_step={f"{f}x{c}": round(v, 4) for (f, c), v in tune.items()},
                        fused_threads=best[0], coupling_threads=best[1]),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved/HBM_PEAK_GBS, 4), "traffic": None, "kernel": "fusedBatch",
+                     "frac": round(achieved/HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": "fusedBatch",
                      "kernel_ms": round(fused_ms, 4), "bytes_per_node": bpn,
-                     "algorithmic_bytes_per_launch": round(bpn*nodes)},
+                     "algorithmic_bytes_per_launch": round(bpn*nodes),
+                     "traffic_source": ("profiles/fusedBatch_config4_fp32.traffic.json: rocprofv3 --pmc FETCH_SIZE and "
+                                        "WRITE_SIZE of the same workload, scaled by a copy of known size")
+                                       if traffic is not None else None},
         "sequential_single_lattice": {"ms_per_step_all_replicas": round(seq_ms, 4),
                                       "speedup_of_batch": round(seq_ms/(1e3*elapsed/steps), 3),
                                       "steps_per_replica_timed": sequential_steps},
