@@ -25,9 +25,10 @@ typedef struct lbm_batch lbm_batch;
  * masses (Da), positions (nm), velocities (nm/ps): all particles, replica after replica (xyz per particle).
  * At most 65,535 replicas of at most 56.5 M nodes each (LBM_ERROR_INVALID beyond); periodic faces, no solid
  * nodes, the explicit drag, FP32 storage and no momentum removal (LBM_ERROR_UNSUPPORTED otherwise).
- * The environment variable LBM_AMD_BATCH_XCD (read here; default 1) sets which launches hand each of the 8 XCDs a
- * contiguous run of their tiles, so that the tiles of one replica share an XCD's L2: bit 0 the coupling, bit 1 the
- * fused pass.  It changes no result.
+ * The environment variable LBM_AMD_BATCH_XCD (read here; default 3) sets which launches group their tiles by XCD, so
+ * that neighbouring tiles share one of the 8 XCDs' L2s: bit 0 the coupling (each XCD takes one contiguous run of the
+ * particle tiles), bit 1 the fused pass (each XCD takes runs of 64 consecutive node tiles in turn); 0 hands work-group
+ * b its own tile.  It changes no result.
  */
 int lbm_batch_create(int num_replicas, const lbm_params* params, const int* num_particles, const double* masses,
         const double* positions, const double* velocities, lbm_batch** out);

@@ -68,7 +68,8 @@ struct lbm_batch {
     uint32_t numTiles = 0;
     int threads = FUSED_THREADS;
     int coupleThreads = BLOCK;              // work-group size of the coupling (particles per tile: coupleThreads/lanes)
-    int xcdOrder = 1;                       // bit 0: coupling tiles, bit 1: fused tiles in XCD-contiguous runs
+    int xcdOrder = 3;                       // bit 0: coupling tiles in XCD-contiguous runs, bit 1: fused tiles in
+                                            // runs of XCD_CHUNK per XCD
     int numParticles = 0;
     uint16_t* particleReplica = nullptr;
     uint16_t* particleTiles = nullptr;      // replica of each work-group of the coupling

@@ -170,7 +170,23 @@ __device__ __forceinline__ uint32_t xcdContiguousTile(uint32_t b, uint32_t n) {
     return x*q + (x < r ? x : r) + i;
 }
 
-/* Host-side launchers (defined in lbm_fluid.hip / lbm_particles.hip); xcdOrder: tiles by xcdContiguousTile. */
+/*
+ * The tile of work-group b of n when each XCD takes runs of XCD_CHUNK consecutive tiles in turn: XCD x's i-th
+ * work-group gets tile ((i / C) 8 + x) C + i mod C (C = XCD_CHUNK) over the first n - n mod 8C work-groups, the rest
+ * keep their own index; a bijection of [0, n).  Neighbouring tiles then run on one XCD at about the same time and
+ * share its L2 for the lines at their borders, while every XCD still streams through the whole array.
+ */
+constexpr uint32_t XCD_CHUNK = 64;
+__device__ __forceinline__ uint32_t xcdChunkedTile(uint32_t b, uint32_t n) {
+    const uint32_t full = n/(8*XCD_CHUNK)*(8*XCD_CHUNK);
+    if (b >= full)
+        return b;
+    const uint32_t x = b & 7u, i = b >> 3;
+    return ((i/XCD_CHUNK)*8 + x)*XCD_CHUNK + i%XCD_CHUNK;
+}
+
+/* Host-side launchers (defined in lbm_fluid.hip / lbm_particles.hip); xcdOrder: coupling tiles by xcdContiguousTile,
+   fused tiles by xcdChunkedTile. */
 void launchFusedBatch(int threads, bool fluct, const ReplicaDesc* table, const uint16_t* tileReplica, uint32_t numTiles,
         const int64_t* stepCounter, bool xcdOrder, hipStream_t stream);
 void launchCoupleBatch(int stencil, int threads, const ReplicaDesc* table, const uint16_t* particleTiles,

@@ -92,10 +92,12 @@ def test_batch_matches_single_replicas_bit_for_bit(stencil):
 @pytest.mark.gpu
 @pytest.mark.parametrize("order", ["1", "2", "3"])
 def test_batch_tile_orders_are_bit_identical(order, monkeypatch):
-    """The XCD-contiguous tile orders (LBM_AMD_BATCH_XCD, read when a batch is created) change which work-group
-    runs which tile, not the arithmetic: the batch's state after 23 steps matches the plain order's bit for bit.  A
-    replica of 700 beads gives the coupling 22 tiles, so the order is not the identity."""
-    replicas = uneven_replicas() + [make_replica((24, 24, 24), 0.7, 700, 2.5, 99, rng_seed=5)]
+    """The XCD tile orders (LBM_AMD_BATCH_XCD, read when a batch is created) change which work-group runs which tile,
+    not the arithmetic: the batch's state after 23 steps matches the plain order's bit for bit.  A replica of 700
+    beads gives the coupling 22 tiles, and with a 72^3 replica the fused pass has 770 tiles of 512 nodes (more than
+    one round of eight runs of 64), so neither order is the identity."""
+    replicas = uneven_replicas() + [make_replica((24, 24, 24), 0.7, 700, 2.5, 99, rng_seed=5),
+                                    make_replica((72, 72, 72), 0.7, 40, 2.0, 5, rng_seed=6)]
     states = []
     for env in ("0", order):
         monkeypatch.setenv("LBM_AMD_BATCH_XCD", env)
