@@ -155,7 +155,9 @@ int lbm_compute_lattice_parameters(const lbm_force_settings* settings, const dou
  * is bit-identical across them): LBM_AMD_ROW_BLOCK=1|2|4 (lattice rows per block of the population layout, default
  * 1); and, for the explicit drag, LBM_AMD_SPREAD=atomic|tiles (the reaction spread), LBM_AMD_FIELD=0|1 (stencils read a
  * lattice-wide moment field) and LBM_AMD_STENCIL_SETUP=0|1 (each particle's stencil formed once), by default tiles,
- * field and setup for dense particles (stencil slots above a quarter of the nodes) and none of them otherwise; for the
+ * field and setup for dense particles (stencil slots above a quarter of the nodes) and none of them otherwise, and
+ * LBM_AMD_COUPLE_XCD=0|1 (with FP16/BF16 storage, each of the 8 XCDs couples a contiguous run of the particles in
+ * their processing order rather than every 8th work-group's; default 1); for the
  * centred drag with a stencil, LBM_AMD_CG_PLANNED=0 makes the host read the conjugate gradients' residual after every
  * iteration instead of after the previous solve's count of them (default 1), and LBM_AMD_CG_SPLIT=0|1 spreads every
  * slot in their products or only those at nodes several slots share (default: the latter for sparse particles; the

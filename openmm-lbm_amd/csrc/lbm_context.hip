@@ -132,6 +132,7 @@ struct lbm_context {
     int numParticles = 0, numSystem = 0, numSlots = 0;
     bool dense = false;                   // denseCoupling(): index order, component-plane reactions
     bool wide = false;                    // 64-bit element indices in the fused pass (populations above 4 GiB)
+    bool coupleXcd = true;                // LBM_AMD_COUPLE_XCD: the coupling's particles in XCD-contiguous runs
     int fusedPath = FUSED_DIRECT;         // the fused-pass kernel (FusedPath)
     int* particles = nullptr;
     double* mass = nullptr;               // lattice units m/m_c
@@ -329,6 +330,7 @@ ParticleArgs particleArgs(lbm_context* c, int64_t step, bool isStep) {
     a.step = step;
     a.hasSolids = c->mask != nullptr;
     a.reactionPlanes = c->dense ? 1 : 0;
+    a.xcdOrder = c->coupleXcd ? 1 : 0;
     a.overflow = c->overflow;
     return a;
 }
@@ -684,6 +686,8 @@ void create(const lbm_params* params, const int* solid, int numSolid, const int*
         // LBM_AMD_SPREAD=atomic or =tiles forces the choice where it fits; by default the tiles serve dense particles
         // (the stencils cover more than a quarter of the lattice), whose concurrent atomics contend.
         const char* spreadEnv = getenv("LBM_AMD_SPREAD");
+        if (const char* xcdEnv = getenv("LBM_AMD_COUPLE_XCD"))
+            c->coupleXcd = xcdEnv[0] != '0';
         bool anyOpen = c->openAxis(0) || c->openAxis(1) || c->openAxis(2);
         bool fit = p.drag_scheme == LBM_DRAG_EXPLICIT && spreadTilesFit(p.nx, p.ny, p.nz, anyOpen);
         bool want = spreadEnv ? std::string(spreadEnv) == "tiles" : c->dense;

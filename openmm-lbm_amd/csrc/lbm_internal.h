@@ -74,6 +74,7 @@ struct ParticleArgs {
     int64_t step;
     int hasSolids;
     int reactionPlanes;     // layout of the reaction accumulators (reactionIndex)
+    int xcdOrder = 1;       // FP16/BF16 stencil coupling: each XCD takes a contiguous run of the particles (0: plain)
     unsigned int* overflow = nullptr;   // set to 1 by a reaction beyond REACTION_LIMIT (reactionInRange)
 };
 

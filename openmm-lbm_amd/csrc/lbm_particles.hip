@@ -249,7 +249,7 @@ __global__ void __launch_bounds__(BLOCK) coupleParticles(const S* __restrict__ s
     // Sparse beads with FP16/BF16 populations: each XCD takes a contiguous run of the work-groups, so that beads near
     // in the processing order (sorted by node) share an L2 for the 128 B lines, 64 nodes wide, that they read
     // (coupled step of 256^3 with 20,000 Keys beads 0.401 -> 0.397 ms; no gain with FP32's 32-node lines).
-    const uint32_t group = isReduced<S>() && !BINNED ? xcdContiguousTile(blockIdx.x, gridDim.x) : blockIdx.x;
+    const uint32_t group = isReduced<S>() && !BINNED && a.xcdOrder ? xcdContiguousTile(blockIdx.x, gridDim.x) : blockIdx.x;
     int rank = (int) group*GROUPS + threadIdx.x/LANES;    // position in the processing order
     bool active = rank < a.numParticles;
     int i = active && order != nullptr ? order[rank] : rank;

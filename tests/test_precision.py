@@ -163,6 +163,25 @@ def test_config3_against_fp64_oracle(config3_oracle, storage, profile_bound, for
     assert worst_force < force_bound
 
 
+@pytest.mark.parametrize("storage", REDUCED)
+def test_reduced_coupling_order_is_bit_identical(storage, monkeypatch):
+    """With FP16/BF16 storage each XCD couples a contiguous run of the beads in their processing order
+    (LBM_AMD_COUPLE_XCD, read when a context is created).  That changes which work-group couples which bead, not the
+    arithmetic: after 60 steps the fields and forces match the plain order's (every 8th work-group on each XCD) bit for
+    bit, since couplings are independent, the rounding is keyed by node and step and the reactions add as integers.
+    200 beads make 50 work-groups, so the order is not the identity."""
+    runs = []
+    for env in ("0", "1"):
+        monkeypatch.setenv("LBM_AMD_COUPLE_XCD", env)
+        setup = config3_setup()
+        ctx = setup.build(None, storage=storage)
+        ctx.step(60)
+        rho, u = ctx.getFluidFields()
+        runs.append((rho, u, ctx.getForces()))
+    for a, b in zip(*runs):
+        assert np.array_equal(a, b)
+
+
 def test_config3_full_size_fp16():
     """config3 at its full size (256^3, walls at z = 0 and 255, body force 1e-6, 20,000 Keys beads, fluctuations off,
     T = 0), 400 device-resident steps with FP16 storage against the same run with FP64 storage (which agrees with the
